@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark of the multigrid coarse-level hot path on one MI355X.
+
+A step is one application of the coarse Dirac stencil to one batch of
+synthetic right-hand sides (inputs already resident in HBM).  The headline
+(``value``) is rhs*sites/s of the headline leg; every other leg is reported
+under ``legs``.  ``roofline`` is computed from the algorithmic bytes/flops of
+the headline kernel and its HIP-event-timed launch duration; ``cpu_baseline``
+is the FP64 CPU oracle timed on the host cores on the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "grid-mg-bench_amd", "python"), os.path.join(ROOT, "oracle")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+import numpy as np  # noqa: E402
+
+METRIC = ("coarse-stencil rhs·sites/s (MRHS FP16/FP32 vs FP64) "
+          "+ MG solve s @ res 1e-10")
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3, "f16": 2500.0, "bf16": 2500.0}
+SEED = 1 | (2 << 16) | (3 << 32) | (4 << 48)
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Timer:
+    """Times K launches of `step` on `stream` with HIP events and the host clock."""
+
+    def __init__(self, mgc, stream):
+        self.mgc, self.stream = mgc, stream
+
+    def run(self, step, steps: int, warmup: int):
+        mgc = self.mgc
+        for _ in range(warmup):
+            step()
+        mgc.synchronize()
+        e0, e1 = mgc.Event(), mgc.Event()
+        t0 = time.perf_counter()
+        e0.record(self.stream)
+        for _ in range(steps):
+            step()
+        e1.record(self.stream)
+        mgc.synchronize()
+        t1 = time.perf_counter()
+        return (t1 - t0) / steps, e0.elapsed_ms(e1) * 1e-3 / steps
+
+
+def single_rhs_leg(mgc, stream, dims, N, prec, steps, warmup, check=True):
+    """Coarse M, one rhs: out(x) = sum_p A_p(x) in(x+d_p)."""
+    V = int(np.prod(dims))
+    wl = 16 if prec == mgc.F64 else 8
+    links = mgc.DeviceBuffer(9 * V * N * N * 16)
+    mgc.random_gaussian(links, 9 * V * N * N, mgc.F64, SEED, 1, stream)
+    op = mgc.CoarseOp(dims, N, prec)
+    op.set_links(links, mgc.LAYOUT_FLAT, invert=False, stream=stream)
+    vin = mgc.DeviceBuffer(V * N * wl)
+    vout = mgc.DeviceBuffer(V * N * wl)
+    mgc.random_gaussian(vin, V * N, prec, SEED, 2, stream)
+    stream.sync()
+    t_host, t_dev = Timer(mgc, stream).run(lambda: op.M(vin, vout, stream), steps, warmup)
+    bytes_alg = 9 * V * N * N * wl + V * N * 2 * wl
+    flops_alg = 8 * 9 * V * N * N
+    res = {
+        "workload": f"coarse M single-rhs {'x'.join(map(str, dims))} nbasis={N} "
+                    f"{'fp64' if prec == mgc.F64 else 'fp32'}",
+        "rhs_sites_per_s": V / t_dev,
+        "ms_per_step": t_host * 1e3,
+        "kernel_us": t_dev * 1e6,
+        "GBps": bytes_alg / t_dev / 1e9,
+        "hbm_frac": bytes_alg / t_dev / 1e9 / HBM_PEAK_GBS,
+        "bytes_per_launch": bytes_alg,
+        "flops_per_launch": flops_alg,
+        "dtype": "f64" if prec == mgc.F64 else "f32",
+    }
+    if check:
+        # parity spot check of the benchmarked operator against the CPU oracle
+        import oracle
+        h_links = links.to_numpy(np.complex128, (9, V, N, N))
+        h_in = vin.to_numpy(mgc.complex_dtype(prec), (V, N)).astype(np.complex128)
+        ref = oracle.coarse_apply(dims, N, h_links, h_in)
+        got = vout.to_numpy(mgc.complex_dtype(prec), (V, N)).astype(np.complex128)
+        res["rel_err_vs_oracle"] = oracle.relative_error(ref, got)
+    for b in (links, vin, vout):
+        b.close()
+    op.close()
+    return res
+
+
+def cpu_baseline(dims, N):
+    """FP64 oracle coarse M on the host cores, timed with the reference's
+    protocol (include/Benchmark_helpers.h:36-59: 10 calibration calls, then
+    min(1000, nSecMin*10/t10 + 1) calls, nSecMin = 5)."""
+    import oracle
+    V = int(np.prod(dims))
+    links = oracle.gaussian(9 * V * N * N, SEED, 1).reshape(9, V, N, N)
+    v = oracle.gaussian(V * N, SEED, 2).reshape(V, N)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        oracle.coarse_apply(dims, N, links, v)
+    t10 = time.perf_counter() - t0
+    n = int(min(1000, 5.0 * 10 / t10 + 1))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        oracle.coarse_apply(dims, N, links, v)
+    t = (time.perf_counter() - t0) / n
+    return {
+        "value": V / t,
+        "unit": "rhs·sites/s",
+        "cores": oracle.num_threads(),
+        "kind": "port",
+        "sample": f"FP64 oracle coarse M, {'x'.join(map(str, dims))} nbasis={N}, 1 rhs, "
+                  f"{n} timed calls after 10 calibration calls ({n * t:.1f} s)",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(the coarse operator shards by 4d domain decomposition; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import mgc
+    log(f"device: {mgc.device_name()}")
+    stream = mgc.Stream()
+    legs = {}
+    dims8 = (8, 8, 8, 8)
+    log("leg: single-rhs fp64 nbasis=32")
+    legs["M_1rhs_f64_n32"] = single_rhs_leg(mgc, stream, dims8, 32, mgc.F64, args.steps,
+                                            args.warmup)
+    log("leg: single-rhs fp32 nbasis=32")
+    legs["M_1rhs_f32_n32"] = single_rhs_leg(mgc, stream, dims8, 32, mgc.F32, args.steps,
+                                            args.warmup)
+    log("leg: single-rhs fp64 nbasis=64")
+    legs["M_1rhs_f64_n64"] = single_rhs_leg(mgc, stream, dims8, 64, mgc.F64, args.steps,
+                                            args.warmup)
+
+    head_key = "M_1rhs_f64_n32"
+    head = legs[head_key]
+    line = {
+        "metric": METRIC,
+        "value": head["rhs_sites_per_s"],
+        "unit": "rhs·sites/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head["dtype"],
+        "data": "synthetic (counter-based Gaussian links and vectors, seed {1,2,3,4})",
+        "config": {"workload": head["workload"], "headline_leg": head_key,
+                   "coarse_lattice": "8^4", "nbasis": 32, "nrhs": 1},
+        "roofline": {
+            "bound": "hbm",
+            "achieved": head["GBps"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": head["hbm_frac"],
+            "traffic": None,
+        },
+        "legs": legs,
+    }
+    if not args.no_cpu:
+        log("cpu baseline (FP64 oracle)")
+        line["cpu_baseline"] = cpu_baseline(dims8, 32)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

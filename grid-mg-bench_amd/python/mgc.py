@@ -1,0 +1,329 @@
+"""ctypes binding of libmgc.so (the C ABI declared in include/mgc.h).
+
+This is the Python face of the C ABI that the tests and the benchmark drive:
+every call goes straight to an ``extern "C"`` entry point of the HIP library.
+Device memory and streams come from the library's own helpers (no torch), so
+one HIP runtime serves the whole process.  A missing library raises
+``MgcError``: there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+F64, F32, F16, BF16 = 0, 1, 2, 3
+LAYOUT_FLAT, LAYOUT_SPLIT = 0, 1
+EVEN, ODD, FULL = 0, 1, -1
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libmgc.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "mgc.h")
+
+_c = ctypes
+_vp, _i, _sz, _u64 = _c.c_void_p, _c.c_int, _c.c_size_t, _c.c_uint64
+_ip = _c.POINTER(_c.c_int)
+
+# argument types of every entry point (return type is int unless noted)
+_SIGS = {
+    "mgc_last_error": ([], _c.c_char_p),
+    "mgc_version": ([], _i),
+    "mgc_device_count": ([_ip], _i),
+    "mgc_device_name": ([_c.c_char_p, _i], _i),
+    "mgc_malloc": ([_c.POINTER(_vp), _sz], _i),
+    "mgc_free": ([_vp], _i),
+    "mgc_memcpy_h2d": ([_vp, _vp, _sz, _vp], _i),
+    "mgc_memcpy_d2h": ([_vp, _vp, _sz, _vp], _i),
+    "mgc_memcpy_d2d": ([_vp, _vp, _sz, _vp], _i),
+    "mgc_fill_zero": ([_vp, _sz, _vp], _i),
+    "mgc_stream_create": ([_c.POINTER(_vp)], _i),
+    "mgc_stream_destroy": ([_vp], _i),
+    "mgc_stream_synchronize": ([_vp], _i),
+    "mgc_device_synchronize": ([], _i),
+    "mgc_event_create": ([_c.POINTER(_vp)], _i),
+    "mgc_event_destroy": ([_vp], _i),
+    "mgc_event_record": ([_vp, _vp], _i),
+    "mgc_event_synchronize": ([_vp], _i),
+    "mgc_event_elapsed_ms": ([_c.POINTER(_c.c_float), _vp, _vp], _i),
+    "mgc_random_gaussian": ([_vp, _sz, _i, _u64, _u64, _vp], _i),
+    "mgc_coarse_op_create": ([_c.POINTER(_vp), _ip, _i, _i, _i], _i),
+    "mgc_coarse_op_destroy": ([_vp], _i),
+    "mgc_coarse_op_info": ([_vp, _ip, _ip, _ip, _ip], _i),
+    "mgc_coarse_op_set_links": ([_vp, _i, _vp, _i, _vp], _i),
+    "mgc_coarse_op_get_links": ([_vp, _i, _vp, _vp], _i),
+    "mgc_coarse_op_get_self_inverse": ([_vp, _vp, _vp], _i),
+    "mgc_coarse_op_M": ([_vp, _vp, _vp, _vp], _i),
+    "mgc_coarse_op_Mdag": ([_vp, _vp, _vp, _vp], _i),
+    "mgc_coarse_op_Mdiag": ([_vp, _vp, _vp, _vp], _i),
+    "mgc_coarse_op_Mdir": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "mgc_coarse_op_MdirAll": ([_vp, _vp, _vp, _vp], _i),
+    "mgc_coarse_op_Dhop": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_DhopOE": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_DhopEO": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_Meooe": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_MeooeDag": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_Mooee": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_MooeeInv": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_MooeeDag": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_MooeeInvDag": ([_vp, _vp, _vp, _i, _vp], _i),
+}
+
+
+class MgcError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mgc error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Load libmgc.so once and attach argument types; raise when it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MgcError(-1, f"{LIB_PATH} not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(code: int) -> None:
+    if code != 0:
+        raise MgcError(code, lib().mgc_last_error().decode())
+
+
+def call(name: str, *args) -> None:
+    check(getattr(lib(), name)(*args))
+
+
+def declared_symbols() -> list:
+    """Names of every function declared in include/mgc.h."""
+    import re
+    text = open(HEADER_PATH).read()
+    return sorted(set(re.findall(r"\b(mgc_[a-zA-Z0-9_]+)\s*\(", text)))
+
+
+# ----------------------------------------------------------------------------
+# runtime helpers
+# ----------------------------------------------------------------------------
+def device_count() -> int:
+    n = _c.c_int(0)
+    call("mgc_device_count", _c.byref(n))
+    return n.value
+
+
+def device_name() -> str:
+    buf = _c.create_string_buffer(256)
+    call("mgc_device_name", buf, 256)
+    return buf.value.decode()
+
+
+def synchronize() -> None:
+    call("mgc_device_synchronize")
+
+
+class Stream:
+    def __init__(self):
+        h = _vp()
+        call("mgc_stream_create", _c.byref(h))
+        self.handle = h.value
+
+    def sync(self) -> None:
+        call("mgc_stream_synchronize", self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            call("mgc_stream_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        h = _vp()
+        call("mgc_event_create", _c.byref(h))
+        self.handle = h.value
+
+    def record(self, stream: Optional[Stream] = None) -> None:
+        call("mgc_event_record", self.handle, stream.handle if stream else None)
+
+    def sync(self) -> None:
+        call("mgc_event_synchronize", self.handle)
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = _c.c_float(0.0)
+        call("mgc_event_elapsed_ms", _c.byref(ms), self.handle, later.handle)
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                call("mgc_event_destroy", self.handle)
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """A device allocation owned by Python; freed on close()/garbage collection."""
+
+    def __init__(self, nbytes: int):
+        h = _vp()
+        call("mgc_malloc", _c.byref(h), _sz(int(nbytes)))
+        self.ptr = h.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_numpy(cls, arr: np.ndarray, stream: Optional[Stream] = None) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        buf = cls(arr.nbytes)
+        call("mgc_memcpy_h2d", buf.ptr, arr.ctypes.data, _sz(arr.nbytes),
+             stream.handle if stream else None)
+        return buf
+
+    def upload(self, arr: np.ndarray, stream: Optional[Stream] = None) -> None:
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("array larger than the device buffer")
+        call("mgc_memcpy_h2d", self.ptr, arr.ctypes.data, _sz(arr.nbytes),
+             stream.handle if stream else None)
+
+    def to_numpy(self, dtype, shape, stream: Optional[Stream] = None) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("requested more bytes than the buffer holds")
+        call("mgc_memcpy_d2h", out.ctypes.data, self.ptr, _sz(out.nbytes),
+             stream.handle if stream else None)
+        return out
+
+    def zero(self, stream: Optional[Stream] = None) -> None:
+        call("mgc_fill_zero", self.ptr, _sz(self.nbytes), stream.handle if stream else None)
+
+    def close(self) -> None:
+        if self.ptr:
+            call("mgc_free", self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def random_gaussian(buf: DeviceBuffer, n_complex: int, prec: int, seed: int, stream_id: int,
+                    stream: Optional[Stream] = None) -> None:
+    call("mgc_random_gaussian", buf.ptr, _sz(n_complex), prec, _u64(seed), _u64(stream_id),
+         stream.handle if stream else None)
+
+
+def complex_dtype(prec: int):
+    return {F64: np.complex128, F32: np.complex64}[prec]
+
+
+# ----------------------------------------------------------------------------
+# coarse operator
+# ----------------------------------------------------------------------------
+def _p(x):
+    """Pointer of a DeviceBuffer or raw int."""
+    if x is None:
+        return None
+    return x.ptr if isinstance(x, DeviceBuffer) else int(x)
+
+
+class CoarseOp:
+    """Handle of a coarse operator (the reference's CoarsenedMatrix)."""
+
+    def __init__(self, dims: Sequence[int], nbasis: int, prec: int = F64, hermitian: int = 0):
+        h = _vp()
+        d = (_c.c_int * 4)(*[int(v) for v in dims])
+        call("mgc_coarse_op_create", _c.byref(h), d, int(nbasis), int(prec), int(hermitian))
+        self.handle = h.value
+        self.dims = tuple(int(v) for v in dims)
+        self.N = int(nbasis)
+        self.prec = int(prec)
+        self.V = int(np.prod(self.dims))
+
+    def close(self) -> None:
+        if self.handle:
+            call("mgc_coarse_op_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_links(self, links: DeviceBuffer, layout: int = LAYOUT_FLAT, invert: bool = True,
+                  stream: Optional[Stream] = None) -> None:
+        call("mgc_coarse_op_set_links", self.handle, layout, _p(links), int(bool(invert)),
+             stream.handle if stream else None)
+
+    def get_links(self, out: DeviceBuffer, layout: int = LAYOUT_FLAT, stream=None) -> None:
+        call("mgc_coarse_op_get_links", self.handle, layout, _p(out),
+             stream.handle if stream else None)
+
+    def get_self_inverse(self, out: DeviceBuffer, stream=None) -> None:
+        call("mgc_coarse_op_get_self_inverse", self.handle, _p(out),
+             stream.handle if stream else None)
+
+    def _s(self, stream):
+        return stream.handle if stream else None
+
+    def M(self, i, o, stream=None):
+        call("mgc_coarse_op_M", self.handle, _p(i), _p(o), self._s(stream))
+
+    def Mdag(self, i, o, stream=None):
+        call("mgc_coarse_op_Mdag", self.handle, _p(i), _p(o), self._s(stream))
+
+    def Mdiag(self, i, o, stream=None):
+        call("mgc_coarse_op_Mdiag", self.handle, _p(i), _p(o), self._s(stream))
+
+    def Mdir(self, i, o, dir_, disp, stream=None):
+        call("mgc_coarse_op_Mdir", self.handle, _p(i), _p(o), int(dir_), int(disp),
+             self._s(stream))
+
+    def MdirAll(self, i, o8, stream=None):
+        call("mgc_coarse_op_MdirAll", self.handle, _p(i), _p(o8), self._s(stream))
+
+    def Dhop(self, i, o, dag=0, stream=None):
+        call("mgc_coarse_op_Dhop", self.handle, _p(i), _p(o), int(dag), self._s(stream))
+
+    def DhopOE(self, i, o, dag=0, stream=None):
+        call("mgc_coarse_op_DhopOE", self.handle, _p(i), _p(o), int(dag), self._s(stream))
+
+    def DhopEO(self, i, o, dag=0, stream=None):
+        call("mgc_coarse_op_DhopEO", self.handle, _p(i), _p(o), int(dag), self._s(stream))
+
+    def Meooe(self, i, o, in_cb, stream=None):
+        call("mgc_coarse_op_Meooe", self.handle, _p(i), _p(o), int(in_cb), self._s(stream))
+
+    def MeooeDag(self, i, o, in_cb, stream=None):
+        call("mgc_coarse_op_MeooeDag", self.handle, _p(i), _p(o), int(in_cb), self._s(stream))
+
+    def Mooee(self, i, o, cb=FULL, stream=None):
+        call("mgc_coarse_op_Mooee", self.handle, _p(i), _p(o), int(cb), self._s(stream))
+
+    def MooeeInv(self, i, o, cb=FULL, stream=None):
+        call("mgc_coarse_op_MooeeInv", self.handle, _p(i), _p(o), int(cb), self._s(stream))
+
+    def MooeeDag(self, i, o, cb=FULL, stream=None):
+        call("mgc_coarse_op_MooeeDag", self.handle, _p(i), _p(o), int(cb), self._s(stream))
+
+    def MooeeInvDag(self, i, o, cb=FULL, stream=None):
+        call("mgc_coarse_op_MooeeInvDag", self.handle, _p(i), _p(o), int(cb), self._s(stream))

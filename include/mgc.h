@@ -1,0 +1,168 @@
+/*
+ * mgc.h -- C ABI of the MI355X multigrid coarse-level library (libmgc.so).
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an `int` status (MGC_OK == 0, negative on error; the message of the
+ * last error on the calling thread is returned by mgc_last_error()).  No C++
+ * exception crosses this boundary.
+ *
+ * Memory and streams
+ *   - Field pointers passed to the apply/transfer calls are DEVICE pointers
+ *     owned by the caller.  The library only reads `in` and writes `out`.
+ *   - Every call that launches work takes a HIP stream (`void*`, a
+ *     hipStream_t; NULL is the null stream) and is asynchronous on it.
+ *   - A handle is used by one host thread at a time; distinct handles are
+ *     independent.
+ *
+ * Layouts (all complex numbers are interleaved re,im)
+ *   - Lattice sites are lexicographic with dimension 0 fastest:
+ *       x = x0 + L0*(x1 + L1*(x2 + L2*x3)).
+ *   - A half-checkerboard field of parity cb holds the sites with
+ *     (x0+x1+x2+x3)%2 == cb, site h = x>>1 (L0 must be even).
+ *   - Coarse vector: [site][N].  Multi-rhs coarse vector: [site][rhs][N]
+ *     (rhs index between site and basis index, as the 5d rhs grid of the
+ *     reference).
+ *   - Coarse links, FLAT layout: [point][site][row][col], N x N row-major
+ *     per (point, site).  SPLIT layout: [point][site][s1][s2][c1][c2] with
+ *     Ns_c = 2 chiralities and row n = s*(N/2)+c.
+ *   - Stencil points: 0..3 = +x,+y,+z,+t; 4..7 = -x,-y,-z,-t; 8 = self.
+ *     out(x) = sum_p A_p(x) in(x + delta_p), periodic.
+ *   - Fine spinor field: [site][spin 4][colour 3].  Gauge field:
+ *     [mu 4][site][3][3] row-major.  Basis (Aggregation subspace):
+ *     [nb][fine site][12], one fine field per basis vector.
+ */
+#ifndef MGC_H
+#define MGC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes, precisions, kinds                                    */
+/* ------------------------------------------------------------------ */
+enum {
+  MGC_OK = 0,
+  MGC_ERR_ARG = -1,       /* null pointer / bad enum / bad value        */
+  MGC_ERR_SHAPE = -2,     /* lattice / block / nbasis not conformable   */
+  MGC_ERR_PRECISION = -3, /* precision not supported by this call       */
+  MGC_ERR_HIP = -4,       /* a HIP runtime call failed                  */
+  MGC_ERR_STATE = -5,     /* call made before the data it needs was set */
+  MGC_ERR_SINGULAR = -6   /* a self link could not be inverted          */
+};
+
+enum { MGC_F64 = 0, MGC_F32 = 1, MGC_F16 = 2, MGC_BF16 = 3 };
+
+enum { MGC_LAYOUT_FLAT = 0, MGC_LAYOUT_SPLIT = 1 };
+
+/* parity argument of the checkerboarded calls */
+enum { MGC_EVEN = 0, MGC_ODD = 1, MGC_FULL = -1 };
+
+typedef struct mgc_coarse_op mgc_coarse_op;
+
+/* ------------------------------------------------------------------ */
+/* runtime helpers (device memory, streams, events)                   */
+/* ------------------------------------------------------------------ */
+const char* mgc_last_error(void);
+int mgc_version(void);
+int mgc_device_count(int* count);
+int mgc_device_name(char* buf, int len);
+int mgc_malloc(void** ptr, size_t bytes);
+int mgc_free(void* ptr);
+int mgc_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int mgc_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int mgc_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int mgc_fill_zero(void* ptr, size_t bytes, void* stream);
+int mgc_stream_create(void** stream);
+int mgc_stream_destroy(void* stream);
+int mgc_stream_synchronize(void* stream);
+int mgc_device_synchronize(void);
+int mgc_event_create(void** event);
+int mgc_event_destroy(void* event);
+int mgc_event_record(void* event, void* stream);
+int mgc_event_synchronize(void* event);
+int mgc_event_elapsed_ms(float* ms, void* start, void* stop);
+
+/* ------------------------------------------------------------------ */
+/* synthetic inputs: counter-based Gaussian RNG                        */
+/* ------------------------------------------------------------------ */
+/* Fill n complex numbers (precision prec) with independent re, im ~
+ * N(0, 1/2) from the counter-based generator keyed by (seed, stream);
+ * element i of the output is a pure function of (seed, stream, i).
+ * Replaces Grid's gaussian(GridParallelRNG&, Field&) as used by
+ * src/Benchmark_coarse_operator.cc:151 (random inputs of the benchmarks). */
+int mgc_random_gaussian(void* out, size_t n_complex, int prec, uint64_t seed,
+                        uint64_t stream, void* hip_stream);
+
+/* ------------------------------------------------------------------ */
+/* coarse operator (CoarsenedMatrix)                                  */
+/* ------------------------------------------------------------------ */
+/* Constructor.  Replaces CoarsenedMatrix(GridCartesian&, GridRedBlackCartesian&,
+ * int hermitian) of include/CoarsenedMatrixBaseline.h:557 and
+ * include/CoarsenedMatrixRework.h:82.  dims = coarse lattice extents,
+ * nbasis = N (even), prec = storage/compute precision of the links and of
+ * the single-rhs vectors (MGC_F64 or MGC_F32; MGC_F16/MGC_BF16 operators
+ * serve the multi-rhs call only). */
+int mgc_coarse_op_create(mgc_coarse_op** op, const int dims[4], int nbasis,
+                         int prec, int hermitian);
+int mgc_coarse_op_destroy(mgc_coarse_op* op);
+int mgc_coarse_op_info(const mgc_coarse_op* op, int dims[4], int* nbasis,
+                       int* prec, int* volume);
+
+/* Install the 9 link fields from a device buffer of complex doubles in the
+ * FLAT or SPLIT layout.  Replaces the direct writes of the public link
+ * members A[p] / Y_[p] (src/Benchmark_coarse_operator.cc:121-127) and the
+ * layout converters of include/Layout_converters.h:33-89.  Also computes the
+ * self-link inverses (InvertSelfStencilPoint, include/CoarsenedMatrixRework.h:640)
+ * when invert != 0. */
+int mgc_coarse_op_set_links(mgc_coarse_op* op, int layout, const void* links_f64,
+                            int invert, void* stream);
+/* Export the links (complex double, FLAT or SPLIT layout). */
+int mgc_coarse_op_get_links(const mgc_coarse_op* op, int layout, void* links_f64,
+                            void* stream);
+/* Export the inverse self links [site][N][N] row-major (complex double). */
+int mgc_coarse_op_get_self_inverse(const mgc_coarse_op* op, void* out_f64, void* stream);
+
+/* out = M in.  include/CoarsenedMatrixUpstream.h:546-607,
+ * include/CoarsenedMatrixRework.h:109-163, include/CoarsenedMatrixBaseline.h:314-344. */
+int mgc_coarse_op_M(mgc_coarse_op* op, const void* in, void* out, void* stream);
+/* out = Mdag in (G5C M G5C for Galerkin, M for hermitian).
+ * include/CoarsenedMatrixUpstream.h:609-621, include/CoarsenedMatrixRework.h:165-176. */
+int mgc_coarse_op_Mdag(mgc_coarse_op* op, const void* in, void* out, void* stream);
+/* out = A_8 in.  include/CoarsenedMatrixUpstream.h:738-742. */
+int mgc_coarse_op_Mdiag(mgc_coarse_op* op, const void* in, void* out, void* stream);
+/* out = A_p in(x+delta_p) for the point of (dir, disp); disp = 0 means self.
+ * include/CoarsenedMatrixUpstream.h:704-736, include/CoarsenedMatrixRework.h:239-286. */
+int mgc_coarse_op_Mdir(mgc_coarse_op* op, const void* in, void* out, int dir, int disp,
+                       void* stream);
+/* out[p] (8 consecutive coarse vectors) = A_p in(x+delta_p), p = 0..7.
+ * include/CoarsenedMatrixUpstream.h:691-703. */
+int mgc_coarse_op_MdirAll(mgc_coarse_op* op, const void* in, void* out8, void* stream);
+/* out = sum_{p<8} A_p in(x+delta_p) (dag: G5C Dhop G5C), full lattice.
+ * include/CoarsenedMatrixRework.h:210-217, 288-399. */
+int mgc_coarse_op_Dhop(mgc_coarse_op* op, const void* in, void* out, int dag, void* stream);
+/* Half-lattice hopping: in Even -> out Odd (DhopOE), in Odd -> out Even (DhopEO).
+ * include/CoarsenedMatrixRework.h:219-237. */
+int mgc_coarse_op_DhopOE(mgc_coarse_op* op, const void* in_even, void* out_odd, int dag,
+                         void* stream);
+int mgc_coarse_op_DhopEO(mgc_coarse_op* op, const void* in_odd, void* out_even, int dag,
+                         void* stream);
+/* Meooe / MeooeDag dispatch on the parity of `in`.  include/CoarsenedMatrixRework.h:186-200. */
+int mgc_coarse_op_Meooe(mgc_coarse_op* op, const void* in, void* out, int in_cb, void* stream);
+int mgc_coarse_op_MeooeDag(mgc_coarse_op* op, const void* in, void* out, int in_cb,
+                           void* stream);
+/* Self-link applications on the full lattice (cb = MGC_FULL) or a half
+ * lattice (cb = MGC_EVEN/MGC_ODD).  include/CoarsenedMatrixRework.h:202-208, 617-638. */
+int mgc_coarse_op_Mooee(mgc_coarse_op* op, const void* in, void* out, int cb, void* stream);
+int mgc_coarse_op_MooeeInv(mgc_coarse_op* op, const void* in, void* out, int cb, void* stream);
+int mgc_coarse_op_MooeeDag(mgc_coarse_op* op, const void* in, void* out, int cb, void* stream);
+int mgc_coarse_op_MooeeInvDag(mgc_coarse_op* op, const void* in, void* out, int cb,
+                              void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGC_H */

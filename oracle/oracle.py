@@ -1,0 +1,168 @@
+"""numpy face of the CPU FP64 oracle (liboracle.so, built from oracle/*.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
+the CPU-baseline leg of bench.py, never by the library it checks.  Each
+wrapper names the reference file:line its C function restates.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built (make -C oracle)")
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def _d(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def _dims(dims):
+    return (ctypes.c_int * 4)(*[int(v) for v in dims])
+
+
+def num_threads() -> int:
+    return int(lib().orc_num_threads())
+
+
+def volume(dims) -> int:
+    return int(np.prod(dims))
+
+
+def gaussian(n: int, seed: int, stream: int) -> np.ndarray:
+    """Counter-based Gaussian complex numbers, re/im ~ N(0, 1/2)."""
+    out = np.empty(n, dtype=np.complex128)
+    lib().orc_gaussian(_d(out.view(np.float64)), ctypes.c_size_t(n), ctypes.c_uint64(seed),
+                       ctypes.c_uint64(stream))
+    return out
+
+
+def point_from_dir_disp(dir_: int, disp: int) -> int:
+    return lib().orc_point_from_dir_disp(dir_, disp)
+
+
+def point_dir_disp(p: int):
+    d, s = ctypes.c_int(), ctypes.c_int()
+    lib().orc_point_dir_disp(p, ctypes.byref(d), ctypes.byref(s))
+    return d.value, s.value
+
+
+def site_shift(dims, x: int, dir_: int, disp: int) -> int:
+    return lib().orc_site_shift(_dims(dims), x, dir_, disp)
+
+
+def parity(dims, x: int) -> int:
+    return lib().orc_parity(_dims(dims), x)
+
+
+def half_index(dims, x: int) -> int:
+    return lib().orc_half_index(_dims(dims), x)
+
+
+def coarse_apply(dims, N, links, vin, mask=0x1FF, dag=0) -> np.ndarray:
+    """out = sum_{p in mask} A_p(x) in(x+d_p), G5C-sandwiched when dag.
+    include/CoarsenedMatrixBaseline.h:314-344, include/CoarsenedMatrixUpstream.h:609-742."""
+    links = np.ascontiguousarray(links, dtype=np.complex128)
+    vin = np.ascontiguousarray(vin, dtype=np.complex128)
+    out = np.zeros_like(vin)
+    lib().orc_coarse_apply(_dims(dims), N, _d(links.view(np.float64)), _d(vin.view(np.float64)),
+                           _d(out.view(np.float64)), ctypes.c_uint(mask), dag)
+    return out
+
+
+def coarse_apply_cb(dims, N, links, vin_half, out_cb, mask=0xFF, dag=0) -> np.ndarray:
+    """Half-lattice hopping, include/CoarsenedMatrixBaseline.h:448-554."""
+    links = np.ascontiguousarray(links, dtype=np.complex128)
+    vin_half = np.ascontiguousarray(vin_half, dtype=np.complex128)
+    out = np.zeros_like(vin_half)
+    lib().orc_coarse_apply_cb(_dims(dims), N, _d(links.view(np.float64)),
+                              _d(vin_half.view(np.float64)), _d(out.view(np.float64)), out_cb,
+                              ctypes.c_uint(mask), dag)
+    return out
+
+
+def site_matrix_apply(dims, N, mats, vin, cb=-1, adjoint=0) -> np.ndarray:
+    """Mooee family, include/CoarsenedMatrixBaseline.h:468-489."""
+    mats = np.ascontiguousarray(mats, dtype=np.complex128)
+    vin = np.ascontiguousarray(vin, dtype=np.complex128)
+    out = np.zeros_like(vin)
+    lib().orc_site_matrix_apply(_dims(dims), N, _d(mats.view(np.float64)),
+                                _d(vin.view(np.float64)), _d(out.view(np.float64)), cb, adjoint)
+    return out
+
+
+def invert(A: np.ndarray) -> np.ndarray:
+    A = np.ascontiguousarray(A, dtype=np.complex128)
+    N = A.shape[0]
+    out = np.empty_like(A)
+    if lib().orc_invert(N, _d(A.view(np.float64)), _d(out.view(np.float64))) != 0:
+        raise np.linalg.LinAlgError("singular")
+    return out
+
+
+def invert_self(dims, N, links) -> np.ndarray:
+    """InvertSelfStencilPoint, include/CoarsenedMatrixBaseline.h:735-768."""
+    links = np.ascontiguousarray(links, dtype=np.complex128)
+    out = np.empty((volume(dims), N, N), dtype=np.complex128)
+    if lib().orc_invert_self(_dims(dims), N, _d(links.view(np.float64)),
+                             _d(out.view(np.float64))) != 0:
+        raise np.linalg.LinAlgError("singular self link")
+    return out
+
+
+def links_flat_to_split(V, N, flat) -> np.ndarray:
+    """include/Layout_converters.h:62-89."""
+    flat = np.ascontiguousarray(flat, dtype=np.complex128)
+    out = np.empty(flat.size, dtype=np.complex128)
+    lib().orc_links_flat_to_split(V, N, _d(flat.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def links_split_to_flat(V, N, split) -> np.ndarray:
+    """include/Layout_converters.h:33-60."""
+    split = np.ascontiguousarray(split, dtype=np.complex128)
+    out = np.empty(split.size, dtype=np.complex128)
+    lib().orc_links_split_to_flat(V, N, _d(split.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def coarse_mrhs_M(dims, N, nrhs, links, vin) -> np.ndarray:
+    """include/CoarsenedMatrixUpstreamImprovedDirsaveLutMRHS.h:736-804."""
+    links = np.ascontiguousarray(links, dtype=np.complex128)
+    vin = np.ascontiguousarray(vin, dtype=np.complex128)
+    out = np.zeros_like(vin)
+    lib().orc_coarse_mrhs_M(_dims(dims), N, nrhs, _d(links.view(np.float64)),
+                            _d(vin.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def construct_remaining_links(dims, N, links, disp_have=-1) -> np.ndarray:
+    """include/CoarsenedMatrixUpstreamImprovedDirsave.h:999-1032 (disp_have=-1),
+    include/CoarsenedMatrixKernelsRework.h:141-167 (disp_have=+1)."""
+    out = np.array(links, dtype=np.complex128, copy=True, order="C")
+    lib().orc_construct_remaining_links(_dims(dims), N, _d(out.view(np.float64)), disp_have)
+    return out
+
+
+def relative_error(ref: np.ndarray, res: np.ndarray) -> float:
+    """||ref - res||_2 / ||ref||_2 (the square root of the reference's
+    norm2(diff)/norm2(ref), include/Benchmark_helpers.h:215-236)."""
+    ref = np.asarray(ref).ravel()
+    res = np.asarray(res).ravel()
+    n = np.linalg.norm(ref)
+    return float(np.linalg.norm(ref - res) / (n if n > 0 else 1.0))

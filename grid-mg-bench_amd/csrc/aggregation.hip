@@ -1,0 +1,455 @@
+// Block transfer operators (Aggregation): basis import/export, block
+// projection (restriction R), promotion (prolongation P), block modified
+// Gram-Schmidt in FP64, and the orthogonality check.
+//
+// Reference semantics (chirality-split form, Ns_c = 2, Ns_b = 2):
+//   include/AggregationKernelsRework.h:96-157   aggregateProjectFast
+//   include/AggregationKernelsRework.h:207-264  aggregatePromoteFast
+//   include/AggregationKernelsRework.h:266-341  aggregateOrthogonalise
+//   include/AggregationRework.h:64-146          class Aggregation
+//   include/CoarseningLookupTableRework.h:121-179 (block <-> fine site tables)
+#include "aggregation.hpp"
+
+#include <vector>
+
+namespace mgc {
+
+// ---------------------------------------------------------------------------
+// layout conversion
+// ---------------------------------------------------------------------------
+__global__ void k_import_basis(const cplx<double>* __restrict__ src, int nb, int Vf, int Vb,
+                               size_t total, const int* __restrict__ blk_fine,
+                               cplx<double>* __restrict__ dst) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < total; t += stride) {
+    const int i = (int)(t % nb);
+    size_t r = t / nb;
+    const int e = (int)(r % 12);
+    r /= 12;  // r = xc*Vb + xb
+    const int sf = blk_fine[r];
+    dst[t] = src[((size_t)i * Vf + sf) * 12 + e];
+  }
+}
+
+__global__ void k_export_basis(const cplx<double>* __restrict__ src, int nb, int Vf, int Vb,
+                               size_t total, const int* __restrict__ blk_fine,
+                               cplx<double>* __restrict__ dst) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < total; t += stride) {
+    const int i = (int)(t % nb);
+    size_t r = t / nb;
+    const int e = (int)(r % 12);
+    r /= 12;
+    const int sf = blk_fine[r];
+    dst[((size_t)i * Vf + sf) * 12 + e] = src[t];
+  }
+}
+
+__global__ void k_basis_to_f32(const cplx<double>* __restrict__ src, cplx<float>* __restrict__ dst,
+                               size_t n) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < n; t += stride) dst[t] = {(float)src[t].re, (float)src[t].im};
+}
+
+const cplx<float>* agg_basis32(mgc_aggregation* a, hipStream_t s) {
+  if (!a->basis32_valid) {
+    const size_t n = (size_t)a->Vc * a->Vb * 12 * a->nb;
+    if (!a->basis32) MGC_HIP(hipMalloc(&a->basis32, n * sizeof(cplx<float>)));
+    k_basis_to_f32<<<grid_for(n, 256, 16384), 256, 0, s>>>(a->basis64, a->basis32, n);
+    MGC_LAUNCH_CHECK();
+    a->basis32_valid = true;
+  }
+  return a->basis32;
+}
+
+// ---------------------------------------------------------------------------
+// project: c(xc)(rhs)(k, i) = sum_{xb, e in k} conj(U(xc,xb,e,i)) f(fine(xc,xb))(rhs)(e)
+// One workgroup per (coarse site, chunk of RC right-hand sides).  Thread
+// (i = t % nb, g = t / nb) walks rows q = g, g+G, ... of the block panel:
+// the nb threads of a group read one row's nb basis entries (contiguous) and
+// the G groups consecutive rows, so a wavefront streams the panel coalesced;
+// f(q) is a same-address broadcast load.  Group partial sums meet in LDS.
+// ---------------------------------------------------------------------------
+constexpr int RC = 4;  // right-hand sides per workgroup
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
+                                                 const cplx<R>* __restrict__ f,
+                                                 cplx<R>* __restrict__ c,
+                                                 const int* __restrict__ blk_fine, int nb, int Vb,
+                                                 int nrhs) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* s_red = reinterpret_cast<cplx<R>*>(smem_raw);  // [G][RC][2][nb]
+  const int xc = blockIdx.x;
+  const int r0 = blockIdx.y * RC;
+  const int nr = min(RC, nrhs - r0);
+  const int tid = threadIdx.x;
+  const int G = blockDim.x / nb;
+  const int i = tid % nb, g = tid / nb;
+  cplx<R> acc[RC][2];
+#pragma unroll
+  for (int r = 0; r < RC; ++r) acc[r][0] = acc[r][1] = {R(0), R(0)};
+  if (g < G) {
+    const int rows = Vb * 12;
+    const cplx<R>* Ub = U + (size_t)xc * rows * nb;
+    const int* bf = blk_fine + (size_t)xc * Vb;
+    for (int q = g; q < rows; q += G) {
+      const int xb = q / 12, e = q - xb * 12;
+      const int k = e / 6;
+      const cplx<R> u = Ub[(size_t)q * nb + i];
+      const cplx<R>* fr = f + ((size_t)bf[xb] * nrhs + r0) * 12 + e;
+#pragma unroll
+      for (int r = 0; r < RC; ++r)
+        if (r < nr) {
+          const cplx<R> x = fr[(size_t)r * 12];
+          if (k == 0) cmac_conj(acc[r][0], u, x);
+          else cmac_conj(acc[r][1], u, x);
+        }
+    }
+    for (int r = 0; r < nr; ++r)
+      for (int k = 0; k < 2; ++k) s_red[(((size_t)g * RC + r) * 2 + k) * nb + i] = acc[r][k];
+  }
+  __syncthreads();
+  for (int o = tid; o < nr * 2 * nb; o += blockDim.x) {
+    const int r = o / (2 * nb), kn = o - r * 2 * nb;  // kn = k*nb + i
+    cplx<R> t = {R(0), R(0)};
+    for (int gg = 0; gg < G; ++gg) t = cadd(t, s_red[((size_t)gg * RC + r) * 2 * nb + kn]);
+    c[((size_t)xc * nrhs + r0 + r) * 2 * nb + kn] = t;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// promote: f(fine(xc,xb))(rhs)(e) = sum_i c(xc)(rhs)(e/6, i) U(xc,xb,e,i)
+// W lanes (W = next power of two >= nb, W <= 64) per row: each lane loads one
+// basis entry, multiplies by the LDS-resident coarse coefficient, and the W
+// lanes reduce with cross-lane shuffles.
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
+                                                 const cplx<R>* __restrict__ c,
+                                                 cplx<R>* __restrict__ f,
+                                                 const int* __restrict__ blk_fine, int nb, int Vb,
+                                                 int nrhs, int W) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* s_c = reinterpret_cast<cplx<R>*>(smem_raw);  // [RC][2][nb]
+  const int xc = blockIdx.x;
+  const int r0 = blockIdx.y * RC;
+  const int nr = min(RC, nrhs - r0);
+  const int tid = threadIdx.x;
+  for (int o = tid; o < nr * 2 * nb; o += blockDim.x)
+    s_c[o] = c[((size_t)xc * nrhs + r0) * 2 * nb + o];
+  __syncthreads();
+  const int lane = tid % W;
+  const int rpp = blockDim.x / W;  // rows per pass
+  const int rows = Vb * 12;
+  const cplx<R>* Ub = U + (size_t)xc * rows * nb;
+  const int* bf = blk_fine + (size_t)xc * Vb;
+  for (int q0 = 0; q0 < rows; q0 += rpp) {
+    const int q = q0 + tid / W;
+    const bool active = q < rows;
+    const int qq = active ? q : 0;
+    const int xb = qq / 12, e = qq - xb * 12;
+    const int k = e / 6;
+    cplx<R> u = {R(0), R(0)};
+    if (active && lane < nb) u = Ub[(size_t)qq * nb + lane];
+    cplx<R> acc[RC];
+#pragma unroll
+    for (int r = 0; r < RC; ++r) {
+      acc[r] = {R(0), R(0)};
+      if (r < nr && lane < nb) cmac(acc[r], s_c[(r * 2 + k) * nb + lane], u);
+    }
+    for (int off = W / 2; off > 0; off >>= 1)
+#pragma unroll
+      for (int r = 0; r < RC; ++r) {
+        acc[r].re += __shfl_xor(acc[r].re, off, W);
+        acc[r].im += __shfl_xor(acc[r].im, off, W);
+      }
+    if (active && lane == 0)
+      for (int r = 0; r < nr; ++r) f[((size_t)bf[xb] * nrhs + r0 + r) * 12 + e] = acc[r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block modified Gram-Schmidt, FP64.  One workgroup per (coarse site,
+// chirality); thread t owns panel rows t, t+256, ... (at most RPT rows) and
+// only ever touches its own rows, so the only communication is the
+// workgroup-wide reductions of the inner products and norms.
+// ---------------------------------------------------------------------------
+constexpr int MGS_THREADS = 256;
+constexpr int MGS_RPT = 8;
+
+__device__ inline double2 wg_sum2(double a, double b, double* s_buf) {
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    s_buf[2 * w] = a;
+    s_buf[2 * w + 1] = b;
+  }
+  __syncthreads();
+  double ra = 0, rb = 0;
+  for (int k = 0; k < nw; ++k) {
+    ra += s_buf[2 * k];
+    rb += s_buf[2 * k + 1];
+  }
+  __syncthreads();
+  return make_double2(ra, rb);
+}
+
+__global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __restrict__ U,
+                                                                int nb, int Vb, int passes) {
+  __shared__ double s_buf[2 * (MGS_THREADS / 64)];
+  const int xc = blockIdx.x >> 1;
+  const int k = blockIdx.x & 1;
+  const int R6 = Vb * 6;
+  size_t rowoff[MGS_RPT];
+  int nrow = 0;
+#pragma unroll
+  for (int j = 0; j < MGS_RPT; ++j) {
+    const int r = threadIdx.x + j * MGS_THREADS;
+    if (r < R6) nrow = j + 1;
+    const int rr = r < R6 ? r : 0;
+    const int xb = rr / 6, e = 6 * k + rr % 6;
+    rowoff[j] = agg_row(nb, Vb, xc, xb, e);
+  }
+  for (int pass = 0; pass < passes; ++pass) {
+    for (int v = 0; v < nb; ++v) {
+      cplx<double> vr[MGS_RPT];
+#pragma unroll
+      for (int j = 0; j < MGS_RPT; ++j)
+        vr[j] = j < nrow ? U[rowoff[j] + v] : cplx<double>{0.0, 0.0};
+      for (int u = 0; u < v; ++u) {
+        cplx<double> d = {0.0, 0.0};
+        cplx<double> ur[MGS_RPT];
+#pragma unroll
+        for (int j = 0; j < MGS_RPT; ++j) {
+          ur[j] = j < nrow ? U[rowoff[j] + u] : cplx<double>{0.0, 0.0};
+          cmac_conj(d, ur[j], vr[j]);
+        }
+        const double2 a = wg_sum2(d.re, d.im, s_buf);
+        const cplx<double> alpha = {a.x, a.y};
+#pragma unroll
+        for (int j = 0; j < MGS_RPT; ++j) {
+          const cplx<double> t = cmul(alpha, ur[j]);
+          vr[j].re -= t.re;
+          vr[j].im -= t.im;
+        }
+      }
+      double n2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < MGS_RPT; ++j) n2 += vr[j].re * vr[j].re + vr[j].im * vr[j].im;
+      const double2 nn = wg_sum2(n2, 0.0, s_buf);
+      const double sc = 1.0 / sqrt(nn.x);
+#pragma unroll
+      for (int j = 0; j < MGS_RPT; ++j)
+        if (j < nrow) U[rowoff[j] + v] = {vr[j].re * sc, vr[j].im * sc};
+    }
+  }
+}
+
+// errs[i] += sum_{k, j} |delta_ij - <U_j, U_i>_{block,k}|^2 (CheckOrthogonal)
+__global__ __launch_bounds__(256) void k_check_orthogonal(const cplx<double>* __restrict__ U,
+                                                          int nb, int Vb,
+                                                          double* __restrict__ errs) {
+  const int xc = blockIdx.x >> 1;
+  const int k = blockIdx.x & 1;
+  for (int pair = threadIdx.x; pair < nb * nb; pair += blockDim.x) {
+    const int i = pair / nb, j = pair % nb;
+    cplx<double> g = {0.0, 0.0};
+    for (int r = 0; r < Vb * 6; ++r) {
+      const size_t off = agg_row(nb, Vb, xc, r / 6, 6 * k + r % 6);
+      cmac_conj(g, U[off + j], U[off + i]);
+    }
+    const double dr = (i == j ? 1.0 : 0.0) - g.re;
+    atomicAdd(&errs[i], dr * dr + g.im * g.im);
+  }
+}
+
+static void free_agg(mgc_aggregation* a) {
+  if (!a) return;
+  void* bufs[] = {a->basis64, a->basis32, a->blk_fine};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete a;
+}
+
+static void check_agg(const mgc_aggregation* a) {
+  MGC_CHECK(a, MGC_ERR_ARG, "null aggregation handle");
+  MGC_CHECK(a->basis_set, MGC_ERR_STATE, "basis not set");
+}
+
+static int next_pow2(int n) {
+  int w = 1;
+  while (w < n) w <<= 1;
+  return w;
+}
+
+}  // namespace mgc
+
+using namespace mgc;
+
+extern "C" {
+
+int mgc_aggregation_create(mgc_aggregation** out, const int fine_dims[4], const int block[4],
+                           int nb) {
+  mgc_aggregation* a = nullptr;
+  int st = guarded([&] {
+    MGC_CHECK(out && fine_dims && block, MGC_ERR_ARG, "null argument");
+    MGC_CHECK(nb >= 1 && nb <= 64, MGC_ERR_SHAPE, "nb must be in 1..64");
+    a = new mgc_aggregation();
+    for (int d = 0; d < 4; ++d) {
+      MGC_CHECK(fine_dims[d] >= 1 && block[d] >= 1, MGC_ERR_SHAPE, "extent < 1");
+      MGC_CHECK(fine_dims[d] % block[d] == 0, MGC_ERR_SHAPE,
+                "block size must divide the fine lattice (subdivides)");
+      a->fine.L[d] = fine_dims[d];
+      a->blk.L[d] = block[d];
+      a->coarse.L[d] = fine_dims[d] / block[d];
+    }
+    a->Vf = a->fine.volume();
+    a->Vc = a->coarse.volume();
+    a->Vb = a->blk.volume();
+    a->nb = nb;
+    std::vector<int> bf((size_t)a->Vc * a->Vb);
+    for (int xc = 0; xc < a->Vc; ++xc) {
+      int cc[4];
+      a->coarse.coords(xc, cc);
+      for (int xb = 0; xb < a->Vb; ++xb) {
+        int cb[4], cf[4];
+        a->blk.coords(xb, cb);
+        for (int d = 0; d < 4; ++d) cf[d] = cc[d] * block[d] + cb[d];
+        bf[(size_t)xc * a->Vb + xb] = a->fine.index(cf);
+      }
+    }
+    MGC_HIP(hipMalloc(&a->blk_fine, bf.size() * sizeof(int)));
+    MGC_HIP(hipMemcpy(a->blk_fine, bf.data(), bf.size() * sizeof(int), hipMemcpyHostToDevice));
+    MGC_HIP(hipMalloc(&a->basis64, (size_t)a->Vf * 12 * nb * sizeof(cplx<double>)));
+    *out = a;
+  });
+  if (st != MGC_OK) free_agg(a);
+  return st;
+}
+
+int mgc_aggregation_destroy(mgc_aggregation* a) {
+  return guarded([&] { free_agg(a); });
+}
+
+int mgc_aggregation_info(const mgc_aggregation* a, int coarse_dims[4], int* nb, int* fine_volume,
+                         int* coarse_volume) {
+  return guarded([&] {
+    MGC_CHECK(a, MGC_ERR_ARG, "null aggregation handle");
+    if (coarse_dims)
+      for (int d = 0; d < 4; ++d) coarse_dims[d] = a->coarse.L[d];
+    if (nb) *nb = a->nb;
+    if (fine_volume) *fine_volume = a->Vf;
+    if (coarse_volume) *coarse_volume = a->Vc;
+  });
+}
+
+int mgc_aggregation_set_basis(mgc_aggregation* a, const void* basis_f64, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(a && basis_f64, MGC_ERR_ARG, "null argument");
+    const size_t total = (size_t)a->Vf * 12 * a->nb;
+    hipStream_t s = as_stream(stream);
+    k_import_basis<<<grid_for(total, 256, 16384), 256, 0, s>>>(
+        (const cplx<double>*)basis_f64, a->nb, a->Vf, a->Vb, total, a->blk_fine, a->basis64);
+    MGC_LAUNCH_CHECK();
+    a->basis_set = true;
+    a->basis32_valid = false;
+  });
+}
+
+int mgc_aggregation_get_basis(const mgc_aggregation* a, void* basis_f64, void* stream) {
+  return guarded([&] {
+    check_agg(a);
+    MGC_CHECK(basis_f64, MGC_ERR_ARG, "null output");
+    const size_t total = (size_t)a->Vf * 12 * a->nb;
+    k_export_basis<<<grid_for(total, 256, 16384), 256, 0, as_stream(stream)>>>(
+        a->basis64, a->nb, a->Vf, a->Vb, total, a->blk_fine, (cplx<double>*)basis_f64);
+    MGC_LAUNCH_CHECK();
+  });
+}
+
+int mgc_aggregation_orthogonalise(mgc_aggregation* a, int passes, void* stream) {
+  return guarded([&] {
+    check_agg(a);
+    MGC_CHECK(passes >= 0, MGC_ERR_ARG, "passes < 0");
+    MGC_CHECK(a->Vb * 6 <= MGS_THREADS * MGS_RPT, MGC_ERR_SHAPE,
+              "block too large for the in-register Gram-Schmidt (|block| <= 341)");
+    if (passes == 0) return;
+    k_orthogonalise<<<2 * a->Vc, MGS_THREADS, 0, as_stream(stream)>>>(a->basis64, a->nb, a->Vb,
+                                                                     passes);
+    MGC_LAUNCH_CHECK();
+    a->basis32_valid = false;
+  });
+}
+
+int mgc_aggregation_check_orthogonal(mgc_aggregation* a, double* errs_host, void* stream) {
+  return guarded([&] {
+    check_agg(a);
+    MGC_CHECK(errs_host, MGC_ERR_ARG, "null output");
+    hipStream_t s = as_stream(stream);
+    double* d = nullptr;
+    MGC_HIP(hipMalloc(&d, a->nb * sizeof(double)));
+    fill_zero(d, a->nb * sizeof(double), s);
+    k_check_orthogonal<<<2 * a->Vc, 256, 0, s>>>(a->basis64, a->nb, a->Vb, d);
+    hipError_t e1 = hipGetLastError();
+    hipError_t e2 = hipMemcpyAsync(errs_host, d, a->nb * sizeof(double), hipMemcpyDeviceToHost, s);
+    hipError_t e3 = hipStreamSynchronize(s);
+    (void)hipFree(d);
+    MGC_HIP(e1);
+    MGC_HIP(e2);
+    MGC_HIP(e3);
+  });
+}
+
+static void transfer(mgc_aggregation* a, const void* in, void* out, int prec, int nrhs,
+                     bool project, hipStream_t s) {
+  check_agg(a);
+  MGC_CHECK(in && out && in != out, MGC_ERR_ARG, "null or aliased vector");
+  MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
+  const dim3 grid(a->Vc, (nrhs + RC - 1) / RC);
+  const int nb = a->nb;
+  if (project) {
+    const int block = (256 / nb) * nb;
+    const size_t sh = (size_t)(block / nb) * RC * 2 * nb *
+                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
+    if (prec == MGC_F64)
+      k_project<double><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
+                                                (cplx<double>*)out, a->blk_fine, nb, a->Vb, nrhs);
+    else if (prec == MGC_F32)
+      k_project<float><<<grid, block, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
+                                               (cplx<float>*)out, a->blk_fine, nb, a->Vb, nrhs);
+    else
+      throw Error(MGC_ERR_PRECISION, "project: F64 or F32 vectors");
+  } else {
+    const int W = next_pow2(nb);
+    const size_t sh = (size_t)RC * 2 * nb *
+                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
+    if (prec == MGC_F64)
+      k_promote<double><<<grid, 256, sh, s>>>(a->basis64, (const cplx<double>*)in,
+                                              (cplx<double>*)out, a->blk_fine, nb, a->Vb, nrhs, W);
+    else if (prec == MGC_F32)
+      k_promote<float><<<grid, 256, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
+                                             (cplx<float>*)out, a->blk_fine, nb, a->Vb, nrhs, W);
+    else
+      throw Error(MGC_ERR_PRECISION, "promote: F64 or F32 vectors");
+  }
+  MGC_LAUNCH_CHECK();
+}
+
+int mgc_aggregation_project(mgc_aggregation* a, const void* fine, void* coarse, int prec,
+                            int nrhs, void* stream) {
+  return guarded([&] { transfer(a, fine, coarse, prec, nrhs, true, as_stream(stream)); });
+}
+
+int mgc_aggregation_promote(mgc_aggregation* a, const void* coarse, void* fine, int prec,
+                            int nrhs, void* stream) {
+  return guarded([&] { transfer(a, coarse, fine, prec, nrhs, false, as_stream(stream)); });
+}
+
+}  // extern "C"

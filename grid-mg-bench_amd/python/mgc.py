@@ -68,6 +68,18 @@ _SIGS = {
     "mgc_coarse_op_MooeeInv": ([_vp, _vp, _vp, _i, _vp], _i),
     "mgc_coarse_op_MooeeDag": ([_vp, _vp, _vp, _i, _vp], _i),
     "mgc_coarse_op_MooeeInvDag": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_prepare_mrhs": ([_vp, _i, _vp], _i),
+    "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
+    "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
+    "mgc_aggregation_create": ([_c.POINTER(_vp), _ip, _ip, _i], _i),
+    "mgc_aggregation_destroy": ([_vp], _i),
+    "mgc_aggregation_info": ([_vp, _ip, _ip, _ip, _ip], _i),
+    "mgc_aggregation_set_basis": ([_vp, _vp, _vp], _i),
+    "mgc_aggregation_get_basis": ([_vp, _vp, _vp], _i),
+    "mgc_aggregation_orthogonalise": ([_vp, _i, _vp], _i),
+    "mgc_aggregation_check_orthogonal": ([_vp, _c.POINTER(_c.c_double), _vp], _i),
+    "mgc_aggregation_project": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "mgc_aggregation_promote": ([_vp, _vp, _vp, _i, _i, _vp], _i),
 }
 
 
@@ -327,3 +339,74 @@ class CoarseOp:
 
     def MooeeInvDag(self, i, o, cb=FULL, stream=None):
         call("mgc_coarse_op_MooeeInvDag", self.handle, _p(i), _p(o), int(cb), self._s(stream))
+
+    def prepare_mrhs(self, mrhs_prec, stream=None):
+        call("mgc_coarse_op_prepare_mrhs", self.handle, int(mrhs_prec), self._s(stream))
+
+    def M_mrhs(self, i, o, nrhs, mrhs_prec=F32, in_prec=F32, in_scales=None, dag=0,
+               stream=None):
+        call("mgc_coarse_op_M_mrhs", self.handle, _p(i), _p(in_scales), int(in_prec), _p(o),
+             int(nrhs), int(mrhs_prec), int(dag), self._s(stream))
+
+
+def quantize(inp, rows, n, prec, out, scales, stream=None):
+    call("mgc_quantize", _p(inp), _sz(int(rows)), int(n), int(prec), _p(out), _p(scales),
+         stream.handle if stream else None)
+
+
+# ----------------------------------------------------------------------------
+# aggregation
+# ----------------------------------------------------------------------------
+class Aggregation:
+    """Handle of a block-transfer operator (the reference's Aggregation)."""
+
+    def __init__(self, fine_dims: Sequence[int], block: Sequence[int], nb: int):
+        h = _vp()
+        fd = (_c.c_int * 4)(*[int(v) for v in fine_dims])
+        bd = (_c.c_int * 4)(*[int(v) for v in block])
+        call("mgc_aggregation_create", _c.byref(h), fd, bd, int(nb))
+        self.handle = h.value
+        self.fine_dims = tuple(int(v) for v in fine_dims)
+        self.block = tuple(int(v) for v in block)
+        self.coarse_dims = tuple(f // b for f, b in zip(self.fine_dims, self.block))
+        self.nb = int(nb)
+        self.N = 2 * self.nb
+        self.Vf = int(np.prod(self.fine_dims))
+        self.Vc = int(np.prod(self.coarse_dims))
+
+    def close(self) -> None:
+        if self.handle:
+            call("mgc_aggregation_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _s(self, stream):
+        return stream.handle if stream else None
+
+    def set_basis(self, basis, stream=None) -> None:
+        call("mgc_aggregation_set_basis", self.handle, _p(basis), self._s(stream))
+
+    def get_basis(self, out, stream=None) -> None:
+        call("mgc_aggregation_get_basis", self.handle, _p(out), self._s(stream))
+
+    def orthogonalise(self, passes: int = 2, stream=None) -> None:
+        call("mgc_aggregation_orthogonalise", self.handle, int(passes), self._s(stream))
+
+    def check_orthogonal(self, stream=None) -> np.ndarray:
+        errs = np.zeros(self.nb, dtype=np.float64)
+        call("mgc_aggregation_check_orthogonal", self.handle,
+             errs.ctypes.data_as(_c.POINTER(_c.c_double)), self._s(stream))
+        return errs
+
+    def project(self, fine, coarse, prec=F64, nrhs=1, stream=None) -> None:
+        call("mgc_aggregation_project", self.handle, _p(fine), _p(coarse), int(prec), int(nrhs),
+             self._s(stream))
+
+    def promote(self, coarse, fine, prec=F64, nrhs=1, stream=None) -> None:
+        call("mgc_aggregation_promote", self.handle, _p(coarse), _p(fine), int(prec), int(nrhs),
+             self._s(stream))

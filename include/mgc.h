@@ -162,6 +162,64 @@ int mgc_coarse_op_MooeeDag(mgc_coarse_op* op, const void* in, void* out, int cb,
 int mgc_coarse_op_MooeeInvDag(mgc_coarse_op* op, const void* in, void* out, int cb,
                               void* stream);
 
+/* ------------------------------------------------------------------ */
+/* multi-rhs coarse stencil on MFMA                                    */
+/* ------------------------------------------------------------------ */
+/* out[site][nrhs][N] (complex FP32) = M in (dag: G5C M G5C), all rhs in one
+ * launch, links read once for every rhs.  Replaces the 5d-grid M of
+ * include/CoarsenedMatrixUpstreamImprovedDirsaveLutMRHS.h:736-804 (rhs index
+ * between site and basis index, links shared over rhs).
+ *   mrhs_prec = MGC_F32  : FP32 MFMA, in must be complex FP32.
+ *   mrhs_prec = MGC_F16 / MGC_BF16 : half-precision MFMA with FP32
+ *     accumulation; links carry one FP32 scale per (site, point); in is
+ *     either complex FP32 (quantised on the fly) or already half precision
+ *     (in_prec == mrhs_prec) with in_scales[site][nrhs] from mgc_quantize.
+ * The operator keeps a swizzled copy of its links for the last mrhs_prec
+ * used (built from the F64/F32 links on first use or by prepare_mrhs). */
+int mgc_coarse_op_prepare_mrhs(mgc_coarse_op* op, int mrhs_prec, void* stream);
+int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scales, int in_prec,
+                         void* out, int nrhs, int mrhs_prec, int dag, void* stream);
+/* rows x n complex FP32 -> rows x n complex F16/BF16 with scales[row] =
+ * max |component| of the row (values divided by it). */
+int mgc_quantize(const void* in_f32, size_t rows, int n, int prec, void* out, float* scales,
+                 void* stream);
+
+/* ------------------------------------------------------------------ */
+/* block transfer (Aggregation), chirality-split form: nb basis        */
+/* vectors, coarse vectors of N = 2*nb with n = k*nb + i, k = spin/2    */
+/* ------------------------------------------------------------------ */
+typedef struct mgc_aggregation mgc_aggregation;
+
+/* Aggregation(GridBase* coarse, GridBase* fine, int cb, int fast)
+ * (include/AggregationRework.h:64-73); block = fine/coarse extents, which
+ * must divide the fine lattice (subdivides, :71). */
+int mgc_aggregation_create(mgc_aggregation** agg, const int fine_dims[4], const int block[4],
+                           int nb);
+int mgc_aggregation_destroy(mgc_aggregation* agg);
+int mgc_aggregation_info(const mgc_aggregation* agg, int coarse_dims[4], int* nb,
+                         int* fine_volume, int* coarse_volume);
+/* Copy nb fine fields [nb][fine site][12] (complex double) into the subspace:
+ * the copy step of Create (include/AggregationRework.h:133-146) and the
+ * writable Subspace() (:75). */
+int mgc_aggregation_set_basis(mgc_aggregation* agg, const void* basis_f64, void* stream);
+int mgc_aggregation_get_basis(const mgc_aggregation* agg, void* basis_f64, void* stream);
+/* Block modified Gram-Schmidt per (block, chirality), FP64, `passes` sweeps.
+ * Orthogonalise (include/AggregationRework.h:80-89) -> aggregateOrthogonalise
+ * (include/AggregationKernelsRework.h:266-341). */
+int mgc_aggregation_orthogonalise(mgc_aggregation* agg, int passes, void* stream);
+/* CheckOrthogonal (include/AggregationRework.h:91-108): errs_host[i] =
+ * norm2(e_i - R v_i) summed over coarse sites (host array of nb doubles). */
+int mgc_aggregation_check_orthogonal(mgc_aggregation* agg, double* errs_host, void* stream);
+/* ProjectToSubspace: coarse[Vc][nrhs][N] = R fine[Vf][nrhs][12]
+ * (include/AggregationRework.h:110-124, include/AggregationKernelsRework.h:96-157).
+ * prec = MGC_F64 or MGC_F32 vectors (FP32 uses an FP32 copy of the basis). */
+int mgc_aggregation_project(mgc_aggregation* agg, const void* fine, void* coarse, int prec,
+                            int nrhs, void* stream);
+/* PromoteFromSubspace: fine = P coarse
+ * (include/AggregationRework.h:126-131, include/AggregationKernelsRework.h:207-264). */
+int mgc_aggregation_promote(mgc_aggregation* agg, const void* coarse, void* fine, int prec,
+                            int nrhs, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,23 @@ void orc_coarse_mrhs_M(const int dims[4], int N, int nrhs, const double* links, 
  * disp_have = +1 the converse (include/CoarsenedMatrixKernelsRework.h:141-167) */
 void orc_construct_remaining_links(const int dims[4], int N, double* links, int disp_have);
 
+/* ---- block transfer (aggregation.c) ---- */
+/* LUT: lut[sc][0..sizes[sc]) = fine sites of block sc with mask != 0
+ * (kind 0 = all, 1 = inner of point, 2 = outer of point); reverse[sf] = sc */
+int orc_lut_populate(const int fine_dims[4], const int block[4], int kind, int point,
+                     int64_t* lut, int64_t* sizes, int64_t* reverse);
+void orc_project(const int fine_dims[4], const int block[4], int nb, const double* basis,
+                 const double* fine, double* coarse, int kind, int point);
+void orc_promote(const int fine_dims[4], const int block[4], int nb, const double* basis,
+                 const double* coarse, double* fine);
+void orc_orthogonalise(const int fine_dims[4], const int block[4], int nb, double* basis,
+                       int passes);
+void orc_chiral_double(int Vf, int nb, const double* in, double* out);
+void orc_project_doubled(const int fine_dims[4], const int block[4], int N, const double* basis,
+                         const double* fine, double* coarse);
+void orc_orthogonalise_doubled(const int fine_dims[4], const int block[4], int N, double* basis,
+                               int passes);
+
 #ifdef __cplusplus
 }
 #endif

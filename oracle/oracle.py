@@ -166,3 +166,85 @@ def relative_error(ref: np.ndarray, res: np.ndarray) -> float:
     res = np.asarray(res).ravel()
     n = np.linalg.norm(ref)
     return float(np.linalg.norm(ref - res) / (n if n > 0 else 1.0))
+
+
+# ----------------------------------------------------------------------------
+# block transfer (aggregation.c)
+# ----------------------------------------------------------------------------
+MASK_ALL, MASK_INNER, MASK_OUTER = 0, 1, 2
+
+
+def coarse_dims(fine_dims, block):
+    return tuple(int(f) // int(b) for f, b in zip(fine_dims, block))
+
+
+def lut_populate(fine_dims, block, kind=MASK_ALL, point=8):
+    """include/CoarseningLookupTableRework.h:121-179: (lut lists, reverse table)."""
+    Vf = volume(fine_dims)
+    Vc = volume(coarse_dims(fine_dims, block))
+    bv = volume(block)
+    lut = np.full(Vc * bv, -1, dtype=np.int64)
+    sizes = np.zeros(Vc, dtype=np.int64)
+    rev = np.zeros(Vf, dtype=np.int64)
+    ip = ctypes.POINTER(ctypes.c_int64)
+    rc = lib().orc_lut_populate(_dims(fine_dims), _dims(block), kind, point,
+                                lut.ctypes.data_as(ip), sizes.ctypes.data_as(ip),
+                                rev.ctypes.data_as(ip))
+    if rc != 0:
+        raise ValueError("block does not divide the lattice")
+    lut = lut.reshape(Vc, bv)
+    return [lut[s, :sizes[s]].copy() for s in range(Vc)], rev
+
+
+def project(fine_dims, block, nb, basis, fine, kind=MASK_ALL, point=8):
+    """aggregateProject, include/AggregationKernelsRework.h:49-94 (split chirality)."""
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    fine = np.ascontiguousarray(fine, dtype=np.complex128)
+    Vc = volume(coarse_dims(fine_dims, block))
+    out = np.zeros((Vc, 2 * nb), dtype=np.complex128)
+    lib().orc_project(_dims(fine_dims), _dims(block), nb, _d(basis.view(np.float64)),
+                      _d(fine.view(np.float64)), _d(out.view(np.float64)), kind, point)
+    return out
+
+
+def promote(fine_dims, block, nb, basis, coarse):
+    """aggregatePromote, include/AggregationKernelsRework.h:159-205."""
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    coarse = np.ascontiguousarray(coarse, dtype=np.complex128)
+    out = np.zeros((volume(fine_dims), 12), dtype=np.complex128)
+    lib().orc_promote(_dims(fine_dims), _dims(block), nb, _d(basis.view(np.float64)),
+                      _d(coarse.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def orthogonalise(fine_dims, block, nb, basis, passes=2):
+    """aggregateOrthogonalise, include/AggregationKernelsRework.h:266-341."""
+    out = np.array(basis, dtype=np.complex128, copy=True, order="C")
+    lib().orc_orthogonalise(_dims(fine_dims), _dims(block), nb, _d(out.view(np.float64)), passes)
+    return out
+
+
+def chiral_double(Vf, nb, basis):
+    """performChiralDoublingG5C, include/Benchmark_helpers.h:156-181."""
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    out = np.zeros((2 * nb,) + basis.shape[1:], dtype=np.complex128)
+    lib().orc_chiral_double(Vf, nb, _d(basis.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def project_doubled(fine_dims, block, N, basis, fine):
+    """Grid blockProject over N chirally doubled vectors (include/CoarsenedMatrixUpstream.h:172)."""
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    fine = np.ascontiguousarray(fine, dtype=np.complex128)
+    out = np.zeros((volume(coarse_dims(fine_dims, block)), N), dtype=np.complex128)
+    lib().orc_project_doubled(_dims(fine_dims), _dims(block), N, _d(basis.view(np.float64)),
+                              _d(fine.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def orthogonalise_doubled(fine_dims, block, N, basis, passes=1):
+    """blockLutedOrthonormalise, include/CoarsenedMatrixUpstreamImprovedDirsaveLut.h:176-239."""
+    out = np.array(basis, dtype=np.complex128, copy=True, order="C")
+    lib().orc_orthogonalise_doubled(_dims(fine_dims), _dims(block), N, _d(out.view(np.float64)),
+                                    passes)
+    return out

@@ -1,0 +1,98 @@
+"""GPU parity of the block transfer operators against the FP64 oracle.
+
+Tolerances (relative L2): FP64 1e-13 (project/promote) and 1e-12 (Gram-Schmidt,
+which compounds nb^2 rounding steps); FP32 vectors 2e-6.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import gauss, rel
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((8, 8, 8, 8), (4, 4, 4, 4), 12),   # fine 8^4 -> 2^4, N = 24
+         ((4, 4, 4, 8), (2, 2, 2, 4), 3),
+         ((4, 8, 4, 4), (4, 4, 2, 2), 16),
+         ((8, 4, 4, 4), (2, 2, 2, 2), 32)]
+
+
+def make(gpu, fd, blk, nb, stream=501, ortho=True):
+    Vf = int(np.prod(fd))
+    B = gauss(nb * Vf * 12, stream).reshape(nb, Vf, 12)
+    agg = gpu.Aggregation(fd, blk, nb)
+    agg.set_basis(gpu.DeviceBuffer.from_numpy(B))
+    return agg, B
+
+
+@pytest.mark.parametrize("fd,blk,nb", CASES)
+def test_orthogonalise_matches_oracle(gpu, fd, blk, nb):
+    print(f"[case] ortho fd={fd} blk={blk} nb={nb}", flush=True)
+    agg, B = make(gpu, fd, blk, nb)
+    for passes in (1, 2):
+        agg.set_basis(gpu.DeviceBuffer.from_numpy(B))
+        agg.orthogonalise(passes)
+        d = gpu.DeviceBuffer(B.nbytes)
+        agg.get_basis(d)
+        gpu.synchronize()
+        got = d.to_numpy(np.complex128, B.shape)
+        ref = oracle.orthogonalise(fd, blk, nb, B, passes)
+        assert rel(ref, got) < 1e-12, passes
+        errs = agg.check_orthogonal()  # squared deviations summed over coarse sites
+        assert errs.max() < 1e-20, errs.max()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("fd,blk,nb", CASES)
+@pytest.mark.parametrize("nrhs", [1, 5])
+def test_project_promote_match_oracle(gpu, fd, blk, nb, prec, nrhs):
+    print(f"[case] transfer fd={fd} blk={blk} nb={nb} prec={prec} nrhs={nrhs}", flush=True)
+    agg, B = make(gpu, fd, blk, nb, 511)
+    Vf = int(np.prod(fd))
+    Vc = int(np.prod(oracle.coarse_dims(fd, blk)))
+    dt = gpu.complex_dtype(prec)
+    tol = 1e-13 if prec == 0 else 2e-6
+    f = gauss(Vf * nrhs * 12, 512).reshape(Vf, nrhs, 12)
+    fq = f.astype(dt).astype(np.complex128)
+    dc = gpu.DeviceBuffer(Vc * nrhs * 2 * nb * np.dtype(dt).itemsize)
+    agg.project(gpu.DeviceBuffer.from_numpy(f.astype(dt)), dc, prec, nrhs)
+    gpu.synchronize()
+    got = dc.to_numpy(dt, (Vc, nrhs, 2 * nb)).astype(np.complex128)
+    for r in range(nrhs):
+        ref = oracle.project(fd, blk, nb, B, np.ascontiguousarray(fq[:, r]))
+        assert rel(ref, got[:, r]) < tol, f"project rhs={r}"
+    c = gauss(Vc * nrhs * 2 * nb, 513).reshape(Vc, nrhs, 2 * nb)
+    cq = c.astype(dt).astype(np.complex128)
+    df = gpu.DeviceBuffer(Vf * nrhs * 12 * np.dtype(dt).itemsize)
+    agg.promote(gpu.DeviceBuffer.from_numpy(c.astype(dt)), df, prec, nrhs)
+    gpu.synchronize()
+    gotf = df.to_numpy(dt, (Vf, nrhs, 12)).astype(np.complex128)
+    for r in range(nrhs):
+        ref = oracle.promote(fd, blk, nb, B, np.ascontiguousarray(cq[:, r]))
+        assert rel(ref, gotf[:, r]) < tol, f"promote rhs={r}"
+
+
+def test_full_size_transfer_identities(gpu):
+    """Fine 32^4, 4^4 blocks, nb = 16 (coarse 8^4, N = 32): R P v_c = v_c after
+    orthonormalisation, and R of a basis vector is e_i (CheckOrthogonal), with
+    the basis generated on the device."""
+    fd, blk, nb = (32, 32, 32, 32), (4, 4, 4, 4), 16
+    print(f"[case] full-size transfer fd={fd} nb={nb}", flush=True)
+    Vf, Vc, N = 32 ** 4, 8 ** 4, 32
+    agg = gpu.Aggregation(fd, blk, nb)
+    basis = gpu.DeviceBuffer(nb * Vf * 12 * 16)
+    gpu.random_gaussian(basis, nb * Vf * 12, gpu.F64, 99, 1)
+    agg.set_basis(basis)
+    agg.orthogonalise(1)
+    errs = agg.check_orthogonal()
+    assert errs.max() < 1e-20 * Vc * N
+    vc = gpu.DeviceBuffer(Vc * N * 16)
+    gpu.random_gaussian(vc, Vc * N, gpu.F64, 99, 2)
+    vf = gpu.DeviceBuffer(Vf * 12 * 16)
+    back = gpu.DeviceBuffer(Vc * N * 16)
+    agg.promote(vc, vf, gpu.F64)
+    agg.project(vf, back, gpu.F64)
+    gpu.synchronize()
+    a = vc.to_numpy(np.complex128, (Vc, N))
+    b = back.to_numpy(np.complex128, (Vc, N))
+    assert rel(a, b) < 1e-13

@@ -1,0 +1,107 @@
+"""GPU parity of the multi-rhs MFMA coarse stencil against the FP64 oracle.
+
+Per-rhs relative L2 bounds (stated against the FP64 oracle on the same
+inputs, with the inputs rounded to FP32 for the FP32 path):
+  FP32 MFMA                : 2e-6 * sqrt(N/8)
+  FP16 MFMA, FP32 accumulate: 4e-3   (u = 2^-11, block scales set each
+                                      link/vector block's max to 1)
+  BF16 MFMA, FP32 accumulate: 3e-2   (u = 2^-8)
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from helpers import g5_links, rand_links, rand_vec, rel
+
+pytestmark = pytest.mark.gpu
+
+F32, F16, BF16 = 1, 2, 3
+TOL = {F32: lambda N: 2e-6 * math.sqrt(N / 8), F16: lambda N: 4e-3, BF16: lambda N: 3e-2}
+
+
+def run_mrhs(gpu, op, v, nrhs, mp, dag=0, half_input=False):
+    V, N = op.V, op.N
+    din = gpu.DeviceBuffer.from_numpy(v.astype(np.complex64))
+    dout = gpu.DeviceBuffer(V * nrhs * N * 8)
+    if half_input:
+        q = gpu.DeviceBuffer(V * nrhs * N * 4)
+        sc = gpu.DeviceBuffer(V * nrhs * 4)
+        gpu.quantize(din, V * nrhs, N, mp, q, sc)
+        op.M_mrhs(q, dout, nrhs, mrhs_prec=mp, in_prec=mp, in_scales=sc, dag=dag)
+    else:
+        op.M_mrhs(din, dout, nrhs, mrhs_prec=mp, in_prec=F32, dag=dag)
+    gpu.synchronize()
+    return dout.to_numpy(np.complex64, (V, nrhs, N)).astype(np.complex128)
+
+
+@pytest.mark.parametrize("mp", [F32, F16, BF16])
+@pytest.mark.parametrize("dims,N,nrhs", [((2, 2, 2, 2), 32, 12), ((4, 2, 2, 2), 64, 5),
+                                         ((2, 4, 2, 2), 64, 24), ((2, 2, 2, 2), 32, 70)])
+def test_mrhs_matches_oracle(gpu, dims, N, nrhs, mp):
+    print(f"[case] mrhs dims={dims} N={N} nrhs={nrhs} prec={mp}", flush=True)
+    links = rand_links(dims, N, 601)
+    op = gpu.CoarseOp(dims, N, gpu.F64)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 602, nrhs=nrhs)
+    vq = v.astype(np.complex64).astype(np.complex128)
+    ref = oracle.coarse_mrhs_M(dims, N, nrhs, links, vq)
+    got = run_mrhs(gpu, op, v, nrhs, mp)
+    errs = [rel(ref[:, k], got[:, k]) for k in range(nrhs)]
+    print(f"  max per-rhs rel err {max(errs):.3e}", flush=True)
+    assert max(errs) < TOL[mp](N)
+    if mp != F32:
+        got_h = run_mrhs(gpu, op, v, nrhs, mp, half_input=True)
+        assert max(rel(ref[:, k], got_h[:, k]) for k in range(nrhs)) < TOL[mp](N)
+
+
+@pytest.mark.parametrize("mp", [F32, F16])
+def test_mrhs_dagger(gpu, mp):
+    dims, N, nrhs = (2, 2, 2, 2), 32, 6
+    links = rand_links(dims, N, 611)
+    op = gpu.CoarseOp(dims, N, gpu.F64)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 612, nrhs=nrhs).astype(np.complex64).astype(np.complex128)
+    got = run_mrhs(gpu, op, v, nrhs, mp, dag=1)
+    for k in range(nrhs):
+        ref = oracle.coarse_apply(dims, N, links, np.ascontiguousarray(v[:, k]), dag=1)
+        assert rel(ref, got[:, k]) < TOL[mp](N)
+
+
+def test_mrhs_equals_single_rhs_loop_fp32(gpu):
+    """The batched apply equals nrhs single-rhs FP32 applies (src/Benchmark_mrhs_coarsening.cc:227-242)."""
+    dims, N, nrhs = (4, 2, 2, 2), 32, 8
+    links = g5_links(dims, N, 621)
+    op = gpu.CoarseOp(dims, N, gpu.F32)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 622, nrhs=nrhs)
+    got = run_mrhs(gpu, op, v, nrhs, F32)
+    V = int(np.prod(dims))
+    for k in range(nrhs):
+        vk = np.ascontiguousarray(v[:, k]).astype(np.complex64)
+        di = gpu.DeviceBuffer.from_numpy(vk)
+        do = gpu.DeviceBuffer(V * N * 8)
+        op.M(di, do)
+        gpu.synchronize()
+        single = do.to_numpy(np.complex64, (V, N)).astype(np.complex128)
+        assert rel(single, got[:, k]) < 2e-6
+
+
+@pytest.mark.parametrize("nrhs", [12, 24, 48])
+def test_mrhs_full_size_cfg3(gpu, nrhs):
+    """8^4 coarse lattice, nbasis 64 (the multi-rhs precision workload): every
+    precision against the FP64 oracle on gamma5-hermitian synthetic links."""
+    dims, N = (8, 8, 8, 8), 64
+    print(f"[case] mrhs full size nrhs={nrhs}", flush=True)
+    links = g5_links(dims, N, 631, herm_self=False)
+    op = gpu.CoarseOp(dims, N, gpu.F64)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 632, nrhs=nrhs)
+    vq = v.astype(np.complex64).astype(np.complex128)
+    ref = oracle.coarse_mrhs_M(dims, N, nrhs, links, vq)
+    for mp in (F32, F16, BF16):
+        got = run_mrhs(gpu, op, v, nrhs, mp)
+        e = max(rel(ref[:, k], got[:, k]) for k in range(nrhs))
+        print(f"  prec={mp} max per-rhs rel err {e:.3e}", flush=True)
+        assert e < TOL[mp](N)

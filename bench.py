@@ -99,6 +99,76 @@ def single_rhs_leg(mgc, stream, dims, N, prec, steps, warmup, check=True):
     return res
 
 
+def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, check=True):
+    """Multi-rhs coarse M on MFMA: one launch applies M to nrhs right-hand sides.
+    half_input: the FP16/BF16 inputs are already quantised (+ per (site, rhs)
+    FP32 scales); otherwise FP32 inputs are quantised inside the step."""
+    V = int(np.prod(dims))
+    names = {mgc.F32: "f32", mgc.F16: "f16", mgc.BF16: "bf16"}
+    links = mgc.DeviceBuffer(9 * V * N * N * 16)
+    mgc.random_gaussian(links, 9 * V * N * N, mgc.F64, SEED, 11, stream)
+    op = mgc.CoarseOp(dims, N, mgc.F64)
+    op.set_links(links, mgc.LAYOUT_FLAT, invert=False, stream=stream)
+    op.prepare_mrhs(mp, stream)
+    vin = mgc.DeviceBuffer(V * nrhs * N * 8)
+    mgc.random_gaussian(vin, V * nrhs * N, mgc.F32, SEED, 12, stream)
+    vout = mgc.DeviceBuffer(V * nrhs * N * 8)
+    wl = 8 if mp == mgc.F32 else 4
+    use_half = half_input and mp != mgc.F32
+    if use_half:
+        q = mgc.DeviceBuffer(V * nrhs * N * 4)
+        qs = mgc.DeviceBuffer(V * nrhs * 4)
+        mgc.quantize(vin, V * nrhs, N, mp, q, qs, stream)
+
+        def step():
+            op.M_mrhs(q, vout, nrhs, mrhs_prec=mp, in_prec=mp, in_scales=qs, stream=stream)
+        bytes_in = V * nrhs * N * 4 + V * nrhs * 4
+    else:
+        def step():
+            op.M_mrhs(vin, vout, nrhs, mrhs_prec=mp, in_prec=mgc.F32, stream=stream)
+        bytes_in = V * nrhs * N * 8
+    stream.sync()
+    t_host, t_dev = Timer(mgc, stream).run(step, steps, warmup)
+    bytes_alg = 9 * V * N * N * wl + (0 if mp == mgc.F32 else 9 * V * 4) + bytes_in + \
+        V * nrhs * N * 8
+    flops_alg = 8 * 9 * V * N * N * nrhs
+    peak_tf = PEAK_TFLOPS[names[mp]]
+    t_hbm = bytes_alg / (HBM_PEAK_GBS * 1e9)
+    t_mfma = flops_alg / (peak_tf * 1e12)
+    res = {
+        "workload": f"coarse M multi-rhs {'x'.join(map(str, dims))} nbasis={N} nrhs={nrhs} "
+                    f"{names[mp]} MFMA (fp32 accumulate), "
+                    f"{'pre-quantised half' if use_half else 'fp32'} input, fp32 output",
+        "rhs_sites_per_s": V * nrhs / t_dev,
+        "ms_per_step": t_host * 1e3,
+        "kernel_us": t_dev * 1e6,
+        "GBps": bytes_alg / t_dev / 1e9,
+        "TFLOPs": flops_alg / t_dev / 1e12,
+        "hbm_frac": bytes_alg / t_dev / 1e9 / HBM_PEAK_GBS,
+        "mfma_frac": flops_alg / t_dev / 1e12 / peak_tf,
+        "bound": "hbm" if t_hbm >= t_mfma else "mfma",
+        "roofline_frac": max(t_hbm, t_mfma) / t_dev,
+        "bytes_per_launch": bytes_alg,
+        "flops_per_launch": flops_alg,
+        "dtype": names[mp],
+    }
+    if check:
+        import oracle
+        h_links = links.to_numpy(np.complex128, (9, V, N, N))
+        h_in = vin.to_numpy(np.complex64, (V, nrhs, N)).astype(np.complex128)
+        got = vout.to_numpy(np.complex64, (V, nrhs, N)).astype(np.complex128)
+        ks = range(0, nrhs, max(1, nrhs // 4))
+        errs = []
+        for k in ks:  # a sample of rhs against the FP64 oracle
+            ref = oracle.coarse_apply(dims, N, h_links, np.ascontiguousarray(h_in[:, k]))
+            errs.append(oracle.relative_error(ref, got[:, k]))
+        res["rel_err_vs_oracle_max"] = max(errs)
+    for b in (links, vin, vout):
+        b.close()
+    op.close()
+    return res
+
+
 def cpu_baseline(dims, N):
     """FP64 oracle coarse M on the host cores, timed with the reference's
     protocol (include/Benchmark_helpers.h:36-59: 10 calibration calls, then
@@ -153,6 +223,14 @@ def main() -> int:
     log("leg: single-rhs fp64 nbasis=64")
     legs["M_1rhs_f64_n64"] = single_rhs_leg(mgc, stream, dims8, 64, mgc.F64, args.steps,
                                             args.warmup)
+    for nrhs in (12, 24, 48):
+        for mp, nm in ((mgc.F32, "f32"), (mgc.F16, "f16"), (mgc.BF16, "bf16")):
+            log(f"leg: multi-rhs {nm} nbasis=64 nrhs={nrhs}")
+            legs[f"M_mrhs{nrhs}_{nm}_n64"] = mrhs_leg(mgc, stream, dims8, 64, nrhs, mp,
+                                                      args.steps, args.warmup)
+    log("leg: multi-rhs f16 nbasis=64 nrhs=48, fp32 input quantised in the step")
+    legs["M_mrhs48_f16_n64_f32in"] = mrhs_leg(mgc, stream, dims8, 64, 48, mgc.F16, args.steps,
+                                              args.warmup, half_input=False)
 
     head_key = "M_1rhs_f64_n32"
     head = legs[head_key]

@@ -72,6 +72,18 @@ void orc_project_doubled(const int fine_dims[4], const int block[4], int N, cons
 void orc_orthogonalise_doubled(const int fine_dims[4], const int block[4], int N, double* basis,
                                int passes);
 
+/* ---- fine Wilson-clover operator and coarse operator construction (wilson.c) ---- */
+void orc_gamma(int mu, double* out4x4);
+/* (4+m) + clover term per site, [V][12][12] */
+void orc_wilson_diag(const int dims[4], const double* gauge, double mass, double csw, double* diag);
+void orc_wilson_dir(const int dims[4], const double* gauge, const double* in, double* out, int mu,
+                    int disp);
+void orc_wilson_diag_apply(int V, const double* diag, const double* in, double* out);
+void orc_wilson_M(const int dims[4], const double* gauge, const double* diag, const double* in,
+                  double* out);
+void orc_coarsen_operator(const int fine_dims[4], const int block[4], int nb, const double* gauge,
+                          const double* diag, const double* basis, double* links);
+
 #ifdef __cplusplus
 }
 #endif

@@ -248,3 +248,78 @@ def orthogonalise_doubled(fine_dims, block, N, basis, passes=1):
     lib().orc_orthogonalise_doubled(_dims(fine_dims), _dims(block), N, _d(out.view(np.float64)),
                                     passes)
     return out
+
+
+# ----------------------------------------------------------------------------
+# fine Wilson-clover operator, gauge field, coarse operator construction
+# ----------------------------------------------------------------------------
+def gamma(mu: int) -> np.ndarray:
+    out = np.zeros((4, 4), dtype=np.complex128)
+    lib().orc_gamma(mu, _d(out.view(np.float64)))
+    return out
+
+
+def su3_gauge(dims, eps=0.4, seed=1, stream=0) -> np.ndarray:
+    """U_mu(x) = exp(i eps H), H = traceless Hermitian part of a Gaussian 3x3
+    (a_ij from the counter-based generator, index (mu*V + x)*9 + 3i + j);
+    exponential through numpy's Hermitian eigensolver.  [4][V][3][3]."""
+    V = volume(dims)
+    a = gaussian(4 * V * 9, seed, stream).reshape(4 * V, 3, 3)
+    H = 0.5 * (a + a.conj().transpose(0, 2, 1))
+    tr = np.trace(H, axis1=1, axis2=2).real / 3.0
+    H = H - tr[:, None, None] * np.eye(3)[None]
+    w, Q = np.linalg.eigh(H)
+    U = np.einsum("nij,nj,nkj->nik", Q, np.exp(1j * eps * w), Q.conj())
+    return np.ascontiguousarray(U.reshape(4, V, 3, 3))
+
+
+def wilson_diag(dims, gauge, mass, csw) -> np.ndarray:
+    """(4+m) + clover term per site, [V][12][12]."""
+    gauge = np.ascontiguousarray(gauge, dtype=np.complex128)
+    out = np.zeros((volume(dims), 12, 12), dtype=np.complex128)
+    lib().orc_wilson_diag(_dims(dims), _d(gauge.view(np.float64)), ctypes.c_double(mass),
+                          ctypes.c_double(csw), _d(out.view(np.float64)))
+    return out
+
+
+def wilson_dir(dims, gauge, psi, mu, disp) -> np.ndarray:
+    gauge = np.ascontiguousarray(gauge, dtype=np.complex128)
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros_like(psi)
+    lib().orc_wilson_dir(_dims(dims), _d(gauge.view(np.float64)), _d(psi.view(np.float64)),
+                         _d(out.view(np.float64)), mu, disp)
+    return out
+
+
+def wilson_diag_apply(diag, psi) -> np.ndarray:
+    diag = np.ascontiguousarray(diag, dtype=np.complex128)
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros_like(psi)
+    lib().orc_wilson_diag_apply(diag.shape[0], _d(diag.view(np.float64)),
+                                _d(psi.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def wilson_M(dims, gauge, diag, psi) -> np.ndarray:
+    gauge = np.ascontiguousarray(gauge, dtype=np.complex128)
+    diag = np.ascontiguousarray(diag, dtype=np.complex128)
+    psi = np.ascontiguousarray(psi, dtype=np.complex128)
+    out = np.zeros_like(psi)
+    lib().orc_wilson_M(_dims(dims), _d(gauge.view(np.float64)), _d(diag.view(np.float64)),
+                       _d(psi.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def coarsen_operator(fine_dims, block, nb, gauge, diag, basis) -> np.ndarray:
+    """CoarsenOperator, include/CoarsenedMatrixBaseline.h:575-705 (mask form).
+    Returns FLAT links [9][Vc][N][N], N = 2*nb."""
+    gauge = np.ascontiguousarray(gauge, dtype=np.complex128)
+    diag = np.ascontiguousarray(diag, dtype=np.complex128)
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    Vc = volume(coarse_dims(fine_dims, block))
+    N = 2 * nb
+    out = np.zeros((9, Vc, N, N), dtype=np.complex128)
+    lib().orc_coarsen_operator(_dims(fine_dims), _dims(block), nb, _d(gauge.view(np.float64)),
+                               _d(diag.view(np.float64)), _d(basis.view(np.float64)),
+                               _d(out.view(np.float64)))
+    return out

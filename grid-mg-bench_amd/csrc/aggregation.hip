@@ -272,7 +272,7 @@ __global__ __launch_bounds__(256) void k_check_orthogonal(const cplx<double>* __
 
 static void free_agg(mgc_aggregation* a) {
   if (!a) return;
-  void* bufs[] = {a->basis64, a->basis32, a->blk_fine};
+  void* bufs[] = {a->basis64, a->basis32, a->blk_fine, a->fine_row};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;
@@ -325,8 +325,12 @@ int mgc_aggregation_create(mgc_aggregation** out, const int fine_dims[4], const 
         bf[(size_t)xc * a->Vb + xb] = a->fine.index(cf);
       }
     }
+    std::vector<int> fr((size_t)a->Vf);
+    for (size_t r = 0; r < bf.size(); ++r) fr[bf[r]] = (int)r;
     MGC_HIP(hipMalloc(&a->blk_fine, bf.size() * sizeof(int)));
     MGC_HIP(hipMemcpy(a->blk_fine, bf.data(), bf.size() * sizeof(int), hipMemcpyHostToDevice));
+    MGC_HIP(hipMalloc(&a->fine_row, fr.size() * sizeof(int)));
+    MGC_HIP(hipMemcpy(a->fine_row, fr.data(), fr.size() * sizeof(int), hipMemcpyHostToDevice));
     MGC_HIP(hipMalloc(&a->basis64, (size_t)a->Vf * 12 * nb * sizeof(cplx<double>)));
     *out = a;
   });

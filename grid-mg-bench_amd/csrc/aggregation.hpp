@@ -16,6 +16,7 @@ struct mgc_aggregation {
   bool basis_set = false;
   bool basis32_valid = false;
   int* blk_fine = nullptr;  // [Vc][Vb] fine site of (block, block site)
+  int* fine_row = nullptr;  // [Vf] panel row (xc*Vb + xb) of a fine site
 };
 
 namespace mgc {

@@ -164,6 +164,14 @@ __global__ __launch_bounds__(256) void k_quantize(const float2* __restrict__ in,
 // ---------------------------------------------------------------------------
 // the multi-rhs stencil
 // ---------------------------------------------------------------------------
+// Slot s of the staged B' operand (kc = (s/64)/NT, nt = (s/64)%NT, lane = s%64)
+// holds B'[k][n] for k = kc*KC + (lane/16)*KCL + e (e < KCL), n = nt*16 + lane%16,
+// i.e. the KCL/2 complex entries c0.. of input column j = n/2, permuted into
+// (Re, -Im) for n even and (Im, Re) for n odd.  For the half precisions every
+// point's column is pre-multiplied by f(p, j) = s_link(x,p) s_in(y_p, j) / R_j
+// with R_j = max_p s_link s_in (so |f| <= 1 and nothing overflows), and the
+// accumulated tile is multiplied by R_j once at the end: all nine points then
+// accumulate into a single FP32 tile.
 template <int KIND, int N, int NT>
 __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
     const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
@@ -175,82 +183,109 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
   constexpr int NKC = 2 * N / M::KC;
   constexpr int THREADS = NMT * 64;
   constexpr int NSLOT = NKC * NT * 64;
+  constexpr int SPT = NSLOT / THREADS;  // staged slots per thread
+  constexpr int NCOL = NT * 8;          // rhs columns covered by the tiles
   constexpr bool SCALED = KIND != MGC_F32;
   __shared__ uint4 sB[NSLOT];
+  __shared__ float s_fac[SCALED ? 9 : 1][NCOL];
+  __shared__ float s_R[NCOL];
 
   const int x = blockIdx.x;
   const int w = threadIdx.x >> 6;
   const int l = threadIdx.x & 63;
   const T* in = reinterpret_cast<const T*>(in_);  // [site][nrhs][N] complex (T pairs)
 
+  int ys[9];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) ys[p] = nbr[(size_t)x * 8 + p];
+  ys[8] = x;
+
+  if constexpr (SCALED) {
+    for (int j = threadIdx.x; j < NCOL; j += THREADS) {
+      float v[9], R = 0.f;
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        v[p] = j < nr ? ascale[(size_t)x * 9 + p] * in_scale[(size_t)ys[p] * nrhs + rhs0 + j] : 0.f;
+        R = fmaxf(R, v[p]);
+      }
+      const float inv = R > 0.f ? 1.f / R : 0.f;
+#pragma unroll
+      for (int p = 0; p < 9; ++p) s_fac[p][j] = v[p] * inv;
+      s_R[j] = R;
+    }
+  }
+
+  // per-thread staging geometry (independent of the point)
+  int soff[SPT];  // element offset of the 16-byte source chunk within a site, -1 if padding
+  int scol[SPT];
+#pragma unroll
+  for (int q = 0; q < SPT; ++q) {
+    const int s = threadIdx.x + q * THREADS;
+    const int lane = s & 63, nt = (s >> 6) % NT, kc = (s >> 6) / NT;
+    const int n = nt * 16 + (lane & 15);
+    const int j = n >> 1;
+    const int c0 = (kc * M::KC + (lane >> 4) * M::KCL) >> 1;
+    soff[q] = j < nr ? ((rhs0 + j) * N + c0) * 2 : -1;
+    scol[q] = n;
+  }
+
+  uint4 a_n[NKC], b_n[SPT];
+  auto issue = [&](int p) {
+    const uint4* Ap = A + ((((size_t)x * 9 + p) * NMT + w) * NKC) * 64 + l;
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) a_n[kc] = Ap[(size_t)kc * 64];
+    const T* base = in + (size_t)ys[p] * nrhs * N * 2;
+#pragma unroll
+    for (int q = 0; q < SPT; ++q)
+      if (soff[q] >= 0) b_n[q] = *reinterpret_cast<const uint4*>(base + soff[q]);
+      else b_n[q] = make_uint4(0u, 0u, 0u, 0u);
+  };
+
   floatx4 C[NT];
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) C[nt] = {0.f, 0.f, 0.f, 0.f};
 
+  issue(0);
 #pragma unroll 1
   for (int p = 0; p < 9; ++p) {
-    const int y = (p == 8) ? x : nbr[(size_t)x * 8 + p];
-    // issue this point's link loads first (independent of the staging below)
-    uint4 a[NKC];
-    const uint4* Ap = A + ((((size_t)x * 9 + p) * NMT + w) * NKC) * 64 + l;
+    uint4 a[NKC], b[SPT];
 #pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) a[kc] = Ap[(size_t)kc * 64];
+    for (int kc = 0; kc < NKC; ++kc) a[kc] = a_n[kc];
+#pragma unroll
+    for (int q = 0; q < SPT; ++q) b[q] = b_n[q];
 
-    __syncthreads();  // the previous point's readers are done with sB
-    // stage B' fragments: slot (kc, nt, lane) holds B'[k][n] for
-    // k = kc*KC + (lane/16)*KCL + e, n = nt*16 + lane%16
-    for (int s = threadIdx.x; s < NSLOT; s += THREADS) {
-      const int lane = s & 63;
-      const int nt = (s >> 6) % NT;
-      const int kc = (s >> 6) / NT;
-      const int n = nt * 16 + (lane & 15);
-      const int j = n >> 1, u = n & 1;
-      const int c0 = (kc * M::KC + (lane >> 4) * M::KCL) >> 1;
-      T v[M::KCL];
-      if (j < nr) {
-        const T* src = in + (((size_t)y * nrhs + rhs0 + j) * N + c0) * 2;
-        T raw[M::KCL];
-        __builtin_memcpy(raw, src, 16);
+    __syncthreads();  // previous point's MFMA readers are done with sB
 #pragma unroll
-        for (int h = 0; h < M::KCL / 2; ++h) {
-          T re = raw[2 * h], im = raw[2 * h + 1];
-          if (dag && (c0 + h) >= N / 2) { re = -re; im = -im; }
-          v[2 * h] = u ? im : re;
-          v[2 * h + 1] = u ? re : (T)(-im);
-        }
-      } else {
+    for (int q = 0; q < SPT; ++q) {
+      const int s = threadIdx.x + q * THREADS;
+      const int n = scol[q];
+      const int u = n & 1;
+      const int c0 = (soff[q] >= 0) ? ((soff[q] >> 1) % N) : 0;
+      float f = 1.f;
+      if constexpr (SCALED) f = s_fac[p][n >> 1];
+      T raw[M::KCL], v[M::KCL];
+      __builtin_memcpy(raw, &b[q], 16);
 #pragma unroll
-        for (int e = 0; e < M::KCL; ++e) v[e] = (T)0.f;
+      for (int h = 0; h < M::KCL / 2; ++h) {
+        float re = (float)raw[2 * h], im = (float)raw[2 * h + 1];
+        const float sg = (dag && (c0 + h) >= N / 2) ? -f : f;
+        re *= sg;
+        im *= sg;
+        v[2 * h] = (T)(u ? im : re);
+        v[2 * h + 1] = (T)(u ? re : -im);
       }
       uint4 wv;
       __builtin_memcpy(&wv, v, 16);
       sB[s] = wv;
     }
     __syncthreads();
+    if (p < 8) issue(p + 1);  // next point's loads overlap this point's MFMAs
 
-    if constexpr (SCALED) {
-      floatx4 T_[NT];
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) T_[nt] = {0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < NKC; ++kc)
 #pragma unroll
-      for (int kc = 0; kc < NKC; ++kc)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          T_[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], T_[nt]);
-      const float sl = ascale[(size_t)x * 9 + p];
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const int j = (nt * 16 + (l & 15)) >> 1;
-        const float f = j < nr ? sl * in_scale[(size_t)y * nrhs + rhs0 + j] : 0.f;
-        C[nt] += f * T_[nt];
-      }
-    } else {
-#pragma unroll
-      for (int kc = 0; kc < NKC; ++kc)
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
-    }
+      for (int nt = 0; nt < NT; ++nt)
+        C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
   }
   // D[m][n]: lane l holds rows 4*(l/16)+i, column l%16
 #pragma unroll
@@ -258,10 +293,12 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
     const int n = nt * 16 + (l & 15);
     const int j = n >> 1, u = n & 1;
     if (j < nr) {
+      float R = 1.f;
+      if constexpr (SCALED) R = s_R[j];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = w * 16 + 4 * (l >> 4) + i;
-        float v = C[nt][i];
+        float v = C[nt][i] * R;
         if (dag && r >= N / 2) v = -v;
         out[(((size_t)x * nrhs + rhs0 + j) * N + r) * 2 + u] = v;
       }

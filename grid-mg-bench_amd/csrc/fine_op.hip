@@ -1,0 +1,426 @@
+// Fine Wilson-clover operator on the GPU (the caller-side input of
+// CoarsenOperator and the smoother's operator), and the SU(3) gauge
+// generator for synthetic inputs.
+//
+// Reference use: the fine LinearOperator's OpDir / OpDiag / Op called by
+// CoarsenOperator (include/CoarsenedMatrixBaseline.h:628-633,
+// include/CoarsenedMatrixUpstream.h:843-844, 890-891) and by the multigrid
+// smoother (include/MultigridPreconditioner.h:295-304); Wilson-clover with
+// m = -0.1, csw = 1.0 as in src/Test_coarse_red_black.cc:99-100.
+#include "fine_op.hpp"
+
+#include <vector>
+
+namespace mgc {
+
+// ---------------------------------------------------------------------------
+// gauge generator: U = exp(i eps H), H = traceless Hermitian part of a 3x3
+// complex Gaussian (entries from the counter-based generator, element index
+// (mu*V + x)*9 + 3i + j).  Exponential by scaling and squaring of a degree-18
+// Taylor polynomial in FP64.
+// ---------------------------------------------------------------------------
+__host__ __device__ uint64_t splitmix64_g(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ inline cplx<double> gauss_at(uint64_t key, uint64_t i) {
+  auto unif = [&](uint64_t ctr) {
+    uint64_t b = splitmix64_g(key ^ splitmix64_g(ctr));
+    return ((double)(b >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+  };
+  const double u1 = unif(2 * i), u2 = unif(2 * i + 1);
+  const double r = sqrt(-log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  return {r * c, r * s};
+}
+
+__device__ inline void m3mul(const cplx<double>* a, const cplx<double>* b, cplx<double>* c) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      cplx<double> s = {0.0, 0.0};
+      for (int k = 0; k < 3; ++k) cmac(s, a[i * 3 + k], b[k * 3 + j]);
+      c[i * 3 + j] = s;
+    }
+}
+
+__global__ void k_random_gauge(cplx<double>* __restrict__ U, size_t nmat, double eps,
+                               uint64_t key) {
+  size_t m = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (m >= nmat) return;
+  cplx<double> a[9], X[9], E[9], T[9];
+  for (int k = 0; k < 9; ++k) a[k] = gauss_at(key, m * 9 + k);
+  // H = (a + a^dag)/2 - tr/3 ; X = i eps H
+  double tr = 0.0;
+  for (int i = 0; i < 3; ++i) tr += a[i * 4].re;
+  tr /= 3.0;
+  double nrm = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      cplx<double> h = {0.5 * (a[i * 3 + j].re + a[j * 3 + i].re),
+                        0.5 * (a[i * 3 + j].im - a[j * 3 + i].im)};
+      if (i == j) h.re -= tr;
+      X[i * 3 + j] = {-eps * h.im, eps * h.re};
+      nrm += X[i * 3 + j].re * X[i * 3 + j].re + X[i * 3 + j].im * X[i * 3 + j].im;
+    }
+  nrm = sqrt(nrm);
+  int sq = 0;
+  while (nrm > 0.25) { nrm *= 0.5; ++sq; }
+  const double f = ldexp(1.0, -sq);
+  for (int k = 0; k < 9; ++k) { X[k].re *= f; X[k].im *= f; }
+  // Horner: E = 1 + X(1 + X/2 (1 + X/3 (...)))
+  for (int k = 0; k < 9; ++k) E[k] = {(k % 4 == 0) ? 1.0 : 0.0, 0.0};
+  for (int n = 18; n >= 1; --n) {
+    m3mul(X, E, T);
+    for (int k = 0; k < 9; ++k) {
+      E[k] = {T[k].re / n, T[k].im / n};
+      if (k % 4 == 0) E[k].re += 1.0;
+    }
+  }
+  for (int s = 0; s < sq; ++s) {
+    m3mul(E, E, T);
+    for (int k = 0; k < 9; ++k) E[k] = T[k];
+  }
+  for (int k = 0; k < 9; ++k) U[m * 9 + k] = E[k];
+}
+
+// ---------------------------------------------------------------------------
+// clover term: (4+m) + C(x) as two 6x6 chirality blocks per site (FP64)
+// ---------------------------------------------------------------------------
+__device__ inline void m3dag(const cplx<double>* a, cplx<double>* c) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) c[i * 3 + j] = {a[j * 3 + i].re, -a[j * 3 + i].im};
+}
+__device__ inline void m3load(const cplx<double>* g, int V, int mu, int x, cplx<double>* o) {
+  const cplx<double>* s = g + ((size_t)mu * V + x) * 9;
+  for (int k = 0; k < 9; ++k) o[k] = s[k];
+}
+
+__global__ void k_clover(const cplx<double>* __restrict__ U, Lat4 lat, double mass, double csw,
+                         cplx<double>* __restrict__ diag) {
+  const int V = lat.volume();
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= V) return;
+  const GammaTab G = gamma_table();
+  cplx<double> D[144];
+  for (int i = 0; i < 144; ++i) D[i] = {0.0, 0.0};
+  for (int i = 0; i < 12; ++i) D[i * 13].re = 4.0 + mass;
+  if (csw != 0.0) {
+    for (int mu = 0; mu < 4; ++mu)
+      for (int nu = mu + 1; nu < 4; ++nu) {
+        const int xpm = lat.shift(x, mu, +1), xpn = lat.shift(x, nu, +1);
+        const int xmm = lat.shift(x, mu, -1), xmn = lat.shift(x, nu, -1);
+        cplx<double> Q[9], A[9], B[9], T1[9], T2[9];
+        for (int k = 0; k < 9; ++k) Q[k] = {0.0, 0.0};
+        // leaf 1
+        m3load(U, V, mu, x, A); m3load(U, V, nu, xpm, B); m3mul(A, B, T1);
+        m3load(U, V, mu, xpn, A); m3dag(A, B); m3mul(T1, B, T2);
+        m3load(U, V, nu, x, A); m3dag(A, B); m3mul(T2, B, T1);
+        for (int k = 0; k < 9; ++k) Q[k] = cadd(Q[k], T1[k]);
+        // leaf 2
+        m3load(U, V, mu, lat.shift(xmm, nu, +1), A); m3dag(A, B);
+        m3load(U, V, nu, x, A); m3mul(A, B, T1);
+        m3load(U, V, nu, xmm, A); m3dag(A, B); m3mul(T1, B, T2);
+        m3load(U, V, mu, xmm, A); m3mul(T2, A, T1);
+        for (int k = 0; k < 9; ++k) Q[k] = cadd(Q[k], T1[k]);
+        // leaf 3
+        const int xmmmn = lat.shift(xmm, nu, -1);
+        m3load(U, V, mu, xmm, A); m3dag(A, T1);
+        m3load(U, V, nu, xmmmn, A); m3dag(A, B); m3mul(T1, B, T2);
+        m3load(U, V, mu, xmmmn, A); m3mul(T2, A, T1);
+        m3load(U, V, nu, xmn, A); m3mul(T1, A, T2);
+        for (int k = 0; k < 9; ++k) Q[k] = cadd(Q[k], T2[k]);
+        // leaf 4
+        m3load(U, V, nu, xmn, A); m3dag(A, T1);
+        m3load(U, V, mu, xmn, A); m3mul(T1, A, T2);
+        m3load(U, V, nu, lat.shift(xmn, mu, +1), A); m3mul(T2, A, T1);
+        m3load(U, V, mu, x, A); m3dag(A, B); m3mul(T1, B, T2);
+        for (int k = 0; k < 9; ++k) Q[k] = cadd(Q[k], T2[k]);
+        // F = (Q - Q^dag)/(8i)
+        cplx<double> F[9];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) {
+            const double zr = Q[i * 3 + j].re - Q[j * 3 + i].re;
+            const double zi = Q[i * 3 + j].im + Q[j * 3 + i].im;
+            F[i * 3 + j] = {zi / 8.0, -zr / 8.0};
+          }
+        // sigma_{mu nu}[r][c] = (i/2)(g_mu g_nu - g_nu g_mu)[r][c]
+        for (int r = 0; r < 4; ++r) {
+          // (g_mu g_nu)[r][c] nonzero at c = col_nu[col_mu[r]]
+          const int k1 = G.col[mu][r], c1 = G.col[nu][k1];
+          const cplx<double> p1 = cmul(cplx<double>{G.re[mu][r], G.im[mu][r]},
+                                       cplx<double>{G.re[nu][k1], G.im[nu][k1]});
+          const int k2 = G.col[nu][r], c2 = G.col[mu][k2];
+          const cplx<double> p2 = cmul(cplx<double>{G.re[nu][r], G.im[nu][r]},
+                                       cplx<double>{G.re[mu][k2], G.im[mu][k2]});
+          // both products land on the same column for mu != nu
+          const cplx<double> s = {0.5 * -(p1.im - p2.im), 0.5 * (p1.re - p2.re)};  // (i/2)(p1-p2)
+          const int c = c1;
+          (void)c2;
+          for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+              const cplx<double> v = cmul(s, F[i * 3 + j]);
+              D[(r * 3 + i) * 12 + c * 3 + j].re += -0.5 * csw * v.re;
+              D[(r * 3 + i) * 12 + c * 3 + j].im += -0.5 * csw * v.im;
+            }
+        }
+      }
+  }
+  // store the two chirality blocks
+  for (int l = 0; l < 2; ++l)
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b)
+        diag[(((size_t)x * 2 + l) * 6 + a) * 6 + b] = D[(6 * l + a) * 12 + 6 * l + b];
+}
+
+template <typename R>
+__global__ void k_convert(const cplx<double>* __restrict__ s, cplx<R>* __restrict__ d, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] = {(R)s[i].re, (R)s[i].im};
+}
+
+__global__ void k_antiperiodic(cplx<double>* __restrict__ U, Lat4 lat) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= lat.volume()) return;
+  int c[4];
+  lat.coords(x, c);
+  if (c[3] != lat.L[3] - 1) return;
+  cplx<double>* u = U + ((size_t)3 * lat.volume() + x) * 9;
+  for (int k = 0; k < 9; ++k) { u[k].re = -u[k].re; u[k].im = -u[k].im; }
+}
+
+// ---------------------------------------------------------------------------
+// Wilson-clover apply: thread per (site, rhs); vectors [V][nrhs][12]
+// mask bit p < 8 selects hopping point p, bit 8 the diagonal; dag applies
+// g5 D g5 (= D^dag)
+// ---------------------------------------------------------------------------
+template <typename R>
+__global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
+                                                const cplx<R>* __restrict__ diag,
+                                                const cplx<R>* __restrict__ in,
+                                                cplx<R>* __restrict__ out,
+                                                const int* __restrict__ nbr, int V, int nrhs,
+                                                unsigned mask, int dag) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= (size_t)V * nrhs) return;
+  const int x = (int)(t / nrhs), rr = (int)(t % nrhs);
+  const GammaTab G = gamma_table();
+  cplx<R> acc[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) acc[e] = {R(0), R(0)};
+  auto ld = [&](int site, int e) {
+    cplx<R> v = in[((size_t)site * nrhs + rr) * 12 + e];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    return v;
+  };
+  if (mask & 0x100u) {
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+      const cplx<R>* D = diag + ((size_t)x * 2 + l) * 36;
+      cplx<R> v[6];
+#pragma unroll
+      for (int b = 0; b < 6; ++b) v[b] = ld(x, 6 * l + b);
+#pragma unroll
+      for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int b = 0; b < 6; ++b) cmac(acc[6 * l + a], D[a * 6 + b], v[b]);
+    }
+  }
+#pragma unroll 1
+  for (int p = 0; p < 8; ++p) {
+    if (!((mask >> p) & 1u)) continue;
+    const int mu = p & 3;
+    const R sg = p < 4 ? R(1) : R(-1);
+    const int y = nbr[(size_t)x * 8 + p];
+    cplx<R> L[9];
+    if (p < 4) {
+      const cplx<R>* u = U + ((size_t)mu * V + x) * 9;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) L[k] = u[k];
+    } else {
+      const cplx<R>* u = U + ((size_t)mu * V + y) * 9;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) L[i * 3 + j] = {u[j * 3 + i].re, -u[j * 3 + i].im};
+    }
+    cplx<R> chi[12];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      cplx<R> w[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) w[j] = ld(y, s * 3 + j);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cplx<R> a = {R(0), R(0)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) cmac(a, L[i * 3 + j], w[j]);
+        chi[s * 3 + i] = a;
+      }
+    }
+    // acc += -1/2 (chi - sg * g_mu chi)
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int cs = G.col[mu][s];
+      const cplx<R> ph = {(R)G.re[mu][s], (R)G.im[mu][s]};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const cplx<R> g = cmul(ph, chi[cs * 3 + i]);
+        acc[s * 3 + i].re += R(-0.5) * (chi[s * 3 + i].re - sg * g.re);
+        acc[s * 3 + i].im += R(-0.5) * (chi[s * 3 + i].im - sg * g.im);
+      }
+    }
+  }
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    cplx<R> v = acc[e];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    out[((size_t)x * nrhs + rr) * 12 + e] = v;
+  }
+}
+
+static void free_fine(mgc_fine_op* op) {
+  if (!op) return;
+  void* bufs[] = {op->gauge64, op->gauge32, op->diag64, op->diag32, op->nbr};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete op;
+}
+
+static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
+                       unsigned mask, int dag, hipStream_t s) {
+  MGC_CHECK(op, MGC_ERR_ARG, "null fine operator");
+  MGC_CHECK(op->gauge_set, MGC_ERR_STATE, "gauge field not set");
+  MGC_CHECK(in && out && in != out, MGC_ERR_ARG, "null or aliased vector");
+  MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
+  const size_t n = (size_t)op->V * nrhs;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  if (prec == MGC_F64) {
+    k_wilson<double><<<g, 256, 0, s>>>(op->gauge64, op->diag64, (const cplx<double>*)in,
+                                       (cplx<double>*)out, op->nbr, op->V, nrhs, mask, dag);
+  } else if (prec == MGC_F32) {
+    if (!op->gauge32) {
+      MGC_HIP(hipMalloc(&op->gauge32, (size_t)4 * op->V * 9 * sizeof(cplx<float>)));
+      MGC_HIP(hipMalloc(&op->diag32, (size_t)op->V * 72 * sizeof(cplx<float>)));
+      k_convert<float><<<grid_for((size_t)4 * op->V * 9, 256, 16384), 256, 0, s>>>(
+          op->gauge64, op->gauge32, (size_t)4 * op->V * 9);
+      k_convert<float><<<grid_for((size_t)op->V * 72, 256, 16384), 256, 0, s>>>(
+          op->diag64, op->diag32, (size_t)op->V * 72);
+    }
+    k_wilson<float><<<g, 256, 0, s>>>(op->gauge32, op->diag32, (const cplx<float>*)in,
+                                      (cplx<float>*)out, op->nbr, op->V, nrhs, mask, dag);
+  } else {
+    throw Error(MGC_ERR_PRECISION, "fine operator: F64 or F32");
+  }
+  MGC_LAUNCH_CHECK();
+}
+
+}  // namespace mgc
+
+using namespace mgc;
+
+extern "C" {
+
+int mgc_random_gauge(void* gauge_f64, const int dims[4], double eps, uint64_t seed,
+                     uint64_t stream_id, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(gauge_f64 && dims, MGC_ERR_ARG, "null argument");
+    size_t V = 1;
+    for (int d = 0; d < 4; ++d) V *= (size_t)dims[d];
+    const size_t nmat = 4 * V;
+    const uint64_t key = splitmix64_g(splitmix64_g(seed) ^ (stream_id * 0xD1B54A32D192ED03ull));
+    k_random_gauge<<<(unsigned)((nmat + 127) / 128), 128, 0, as_stream(stream)>>>(
+        (cplx<double>*)gauge_f64, nmat, eps, key);
+    MGC_LAUNCH_CHECK();
+  });
+}
+
+int mgc_fine_op_create(mgc_fine_op** out, const int dims[4], double mass, double csw,
+                       int antiperiodic_t) {
+  mgc_fine_op* op = nullptr;
+  int st = guarded([&] {
+    MGC_CHECK(out && dims, MGC_ERR_ARG, "null argument");
+    op = new mgc_fine_op();
+    for (int d = 0; d < 4; ++d) {
+      MGC_CHECK(dims[d] >= 1, MGC_ERR_SHAPE, "extent < 1");
+      op->lat.L[d] = dims[d];
+    }
+    op->V = op->lat.volume();
+    op->mass = mass;
+    op->csw = csw;
+    op->antiperiodic_t = antiperiodic_t ? 1 : 0;
+    std::vector<int> nb((size_t)op->V * 8);
+    for (int x = 0; x < op->V; ++x)
+      for (int p = 0; p < 8; ++p) nb[(size_t)x * 8 + p] = op->lat.shift(x, p & 3, p < 4 ? 1 : -1);
+    MGC_HIP(hipMalloc(&op->nbr, nb.size() * sizeof(int)));
+    MGC_HIP(hipMemcpy(op->nbr, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
+    MGC_HIP(hipMalloc(&op->gauge64, (size_t)4 * op->V * 9 * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&op->diag64, (size_t)op->V * 72 * sizeof(cplx<double>)));
+    *out = op;
+  });
+  if (st != MGC_OK) free_fine(op);
+  return st;
+}
+
+int mgc_fine_op_destroy(mgc_fine_op* op) {
+  return guarded([&] { free_fine(op); });
+}
+
+int mgc_fine_op_set_gauge(mgc_fine_op* op, const void* gauge_f64, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(op && gauge_f64, MGC_ERR_ARG, "null argument");
+    hipStream_t s = as_stream(stream);
+    MGC_HIP(hipMemcpyAsync(op->gauge64, gauge_f64, (size_t)4 * op->V * 9 * sizeof(cplx<double>),
+                           hipMemcpyDeviceToDevice, s));
+    if (op->antiperiodic_t) {
+      k_antiperiodic<<<(op->V + 255) / 256, 256, 0, s>>>(op->gauge64, op->lat);
+      MGC_LAUNCH_CHECK();
+    }
+    k_clover<<<(op->V + 63) / 64, 64, 0, s>>>(op->gauge64, op->lat, op->mass, op->csw,
+                                              op->diag64);
+    MGC_LAUNCH_CHECK();
+    if (op->gauge32) {  // refresh the FP32 copies
+      k_convert<float><<<grid_for((size_t)4 * op->V * 9, 256, 16384), 256, 0, s>>>(
+          op->gauge64, op->gauge32, (size_t)4 * op->V * 9);
+      k_convert<float><<<grid_for((size_t)op->V * 72, 256, 16384), 256, 0, s>>>(
+          op->diag64, op->diag32, (size_t)op->V * 72);
+      MGC_LAUNCH_CHECK();
+    }
+    op->gauge_set = true;
+  });
+}
+
+int mgc_fine_op_get_diag(const mgc_fine_op* op, void* out_f64, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(op && out_f64, MGC_ERR_ARG, "null argument");
+    MGC_CHECK(op->gauge_set, MGC_ERR_STATE, "gauge field not set");
+    MGC_HIP(hipMemcpyAsync(out_f64, op->diag64, (size_t)op->V * 72 * sizeof(cplx<double>),
+                           hipMemcpyDeviceToDevice, as_stream(stream)));
+  });
+}
+
+int mgc_fine_op_M(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs, void* stream) {
+  return guarded([&] { fine_apply(op, in, out, prec, nrhs, 0x1FFu, 0, as_stream(stream)); });
+}
+int mgc_fine_op_Mdag(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
+                     void* stream) {
+  return guarded([&] { fine_apply(op, in, out, prec, nrhs, 0x1FFu, 1, as_stream(stream)); });
+}
+int mgc_fine_op_Mdiag(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
+                      void* stream) {
+  return guarded([&] { fine_apply(op, in, out, prec, nrhs, 0x100u, 0, as_stream(stream)); });
+}
+int mgc_fine_op_Mdir(mgc_fine_op* op, const void* in, void* out, int dir, int disp, int prec,
+                     int nrhs, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(disp == 1 || disp == -1, MGC_ERR_ARG, "Mdir needs disp = +1 or -1");
+    MGC_CHECK(dir >= 0 && dir < 4, MGC_ERR_ARG, "dir in 0..3");
+    const int p = disp == 1 ? dir : dir + 4;
+    fine_apply(op, in, out, prec, nrhs, 1u << p, 0, as_stream(stream));
+  });
+}
+
+}  // extern "C"

@@ -71,6 +71,17 @@ _SIGS = {
     "mgc_coarse_op_prepare_mrhs": ([_vp, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
+    "mgc_fine_op_create": ([_c.POINTER(_vp), _ip, _c.c_double, _c.c_double, _i], _i),
+    "mgc_fine_op_destroy": ([_vp], _i),
+    "mgc_fine_op_set_gauge": ([_vp, _vp, _vp], _i),
+    "mgc_fine_op_get_diag": ([_vp, _vp, _vp], _i),
+    "mgc_fine_op_M": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "mgc_fine_op_Mdag": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "mgc_fine_op_Mdiag": ([_vp, _vp, _vp, _i, _i, _vp], _i),
+    "mgc_fine_op_Mdir": ([_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i),
+    "mgc_random_gauge": ([_vp, _ip, _c.c_double, _u64, _u64, _vp], _i),
+    "mgc_coarsen_operator": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarse_op_construct_remaining_links": ([_vp, _i, _vp], _i),
     "mgc_aggregation_create": ([_c.POINTER(_vp), _ip, _ip, _i], _i),
     "mgc_aggregation_destroy": ([_vp], _i),
     "mgc_aggregation_info": ([_vp, _ip, _ip, _ip, _ip], _i),
@@ -410,3 +421,76 @@ class Aggregation:
     def promote(self, coarse, fine, prec=F64, nrhs=1, stream=None) -> None:
         call("mgc_aggregation_promote", self.handle, _p(coarse), _p(fine), int(prec), int(nrhs),
              self._s(stream))
+
+
+# ----------------------------------------------------------------------------
+# fine Wilson-clover operator
+# ----------------------------------------------------------------------------
+class FineOp:
+    """Wilson-clover fine operator (the fine LinearOperator of CoarsenOperator)."""
+
+    def __init__(self, dims: Sequence[int], mass: float = -0.1, csw: float = 1.0,
+                 antiperiodic_t: bool = False):
+        h = _vp()
+        d = (_c.c_int * 4)(*[int(v) for v in dims])
+        call("mgc_fine_op_create", _c.byref(h), d, float(mass), float(csw),
+             int(bool(antiperiodic_t)))
+        self.handle = h.value
+        self.dims = tuple(int(v) for v in dims)
+        self.V = int(np.prod(self.dims))
+
+    def close(self) -> None:
+        if self.handle:
+            call("mgc_fine_op_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _s(self, stream):
+        return stream.handle if stream else None
+
+    def set_gauge(self, gauge, stream=None):
+        call("mgc_fine_op_set_gauge", self.handle, _p(gauge), self._s(stream))
+
+    def get_diag(self, out, stream=None):
+        call("mgc_fine_op_get_diag", self.handle, _p(out), self._s(stream))
+
+    def M(self, i, o, prec=F64, nrhs=1, stream=None):
+        call("mgc_fine_op_M", self.handle, _p(i), _p(o), int(prec), int(nrhs), self._s(stream))
+
+    def Mdag(self, i, o, prec=F64, nrhs=1, stream=None):
+        call("mgc_fine_op_Mdag", self.handle, _p(i), _p(o), int(prec), int(nrhs),
+             self._s(stream))
+
+    def Mdiag(self, i, o, prec=F64, nrhs=1, stream=None):
+        call("mgc_fine_op_Mdiag", self.handle, _p(i), _p(o), int(prec), int(nrhs),
+             self._s(stream))
+
+    def Mdir(self, i, o, dir_, disp, prec=F64, nrhs=1, stream=None):
+        call("mgc_fine_op_Mdir", self.handle, _p(i), _p(o), int(dir_), int(disp), int(prec),
+             int(nrhs), self._s(stream))
+
+
+def random_gauge(buf: DeviceBuffer, dims, eps=0.4, seed=1, stream_id=0, stream=None):
+    d = (_c.c_int * 4)(*[int(v) for v in dims])
+    call("mgc_random_gauge", buf.ptr, d, float(eps), _u64(seed), _u64(stream_id),
+         stream.handle if stream else None)
+
+
+COARSEN_FULL, COARSEN_DIRSAVE = 0, 1
+
+
+def coarsen_operator(cop: CoarseOp, fop: FineOp, agg: "Aggregation", mode=COARSEN_DIRSAVE,
+                     stream=None):
+    """CoarsenOperator: build cop's links from the fine operator and the basis."""
+    call("mgc_coarsen_operator", cop.handle, fop.handle, agg.handle, int(mode),
+         stream.handle if stream else None)
+
+
+def construct_remaining_links(cop: CoarseOp, disp_have=-1, stream=None):
+    call("mgc_coarse_op_construct_remaining_links", cop.handle, int(disp_have),
+         stream.handle if stream else None)

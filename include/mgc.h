@@ -220,6 +220,60 @@ int mgc_aggregation_project(mgc_aggregation* agg, const void* fine, void* coarse
 int mgc_aggregation_promote(mgc_aggregation* agg, const void* coarse, void* fine, int prec,
                             int nrhs, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* fine Wilson-clover operator (input of CoarsenOperator, smoother)    */
+/* ------------------------------------------------------------------ */
+/* Convention (chiral basis, g5 = g_x g_y g_z g_t = diag(1,1,-1,-1)):
+ *   D psi(x) = [(4+m) + C(x)] psi(x) - 1/2 sum_mu [(1-g_mu) U_mu(x) psi(x+mu)
+ *                                                 + (1+g_mu) U_mu(x-mu)^dag psi(x-mu)]
+ *   C = -(csw/2) sum_{mu<nu} sigma_{mu nu} F_{mu nu} (clover leaves).
+ * Plays the role of the fine LinearOperatorBase the reference's
+ * CoarsenOperator takes (OpDir / OpDiag / Op, include/CoarsenedMatrixBaseline.h:628-633)
+ * with the Wilson-clover parameters of src/Test_coarse_red_black.cc:99-100.
+ * Vectors: [site][nrhs][12] complex in precision prec (F64 or F32). */
+typedef struct mgc_fine_op mgc_fine_op;
+
+int mgc_fine_op_create(mgc_fine_op** op, const int dims[4], double mass, double csw,
+                       int antiperiodic_t);
+int mgc_fine_op_destroy(mgc_fine_op* op);
+/* gauge field [mu][site][3][3] complex double (device); builds the clover term */
+int mgc_fine_op_set_gauge(mgc_fine_op* op, const void* gauge_f64, void* stream);
+/* (4+m) + C(x) as [site][2 chiralities][6][6] complex double (device) */
+int mgc_fine_op_get_diag(const mgc_fine_op* op, void* out_f64, void* stream);
+int mgc_fine_op_M(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs, void* stream);
+int mgc_fine_op_Mdag(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
+                     void* stream);
+int mgc_fine_op_Mdiag(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
+                      void* stream);
+int mgc_fine_op_Mdir(mgc_fine_op* op, const void* in, void* out, int dir, int disp, int prec,
+                     int nrhs, void* stream);
+/* SU(3) gauge field U = exp(i eps H), H the traceless Hermitian part of a
+ * Gaussian 3x3 (counter-based generator, element (mu*V + x)*9 + 3i + j);
+ * the synthetic-gauge recipe of the multigrid configurations (eps = 0.4). */
+int mgc_random_gauge(void* gauge_f64, const int dims[4], double eps, uint64_t seed,
+                     uint64_t stream_id, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* coarse operator construction                                        */
+/* ------------------------------------------------------------------ */
+enum { MGC_COARSEN_FULL = 0, MGC_COARSEN_DIRSAVE = 1 };
+
+/* CoarsenOperator(FineGrid, linop, Aggregation): the Galerkin links
+ * A_p = R_outer D_p P, A_8 = R D_BB P of the fine operator on the
+ * aggregation's (chirally doubled) basis, in FP64, then installed in `cop`
+ * with its self-link inverses.  FULL computes all nine points
+ * (include/CoarsenedMatrixBaseline.h:575-705); DIRSAVE computes the backward
+ * points and the self link and rebuilds the forward ones by
+ * gamma5-hermiticity (include/CoarsenedMatrixUpstreamImprovedDirsave.h:835-932,
+ * 999-1032).  cop must have the aggregation's coarse lattice and N = 2*nb. */
+int mgc_coarsen_operator(mgc_coarse_op* cop, mgc_fine_op* fop, mgc_aggregation* agg, int mode,
+                         void* stream);
+/* ConstructRemainingLinks (include/CoarsenedMatrixUpstreamImprovedDirsave.h:999-1032,
+ * disp_have = -1) and shiftLinks (include/CoarsenedMatrixKernelsRework.h:141-167,
+ * disp_have = +1): A_want(x) = sigma o adj(A_have(x + disp_want mu)), sigma = -1
+ * on the off-diagonal chirality blocks. */
+int mgc_coarse_op_construct_remaining_links(mgc_coarse_op* op, int disp_have, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,164 @@
+"""GPU parity of the fine Wilson-clover operator and of CoarsenOperator.
+
+Bounds (relative L2 against the FP64 oracle): fine operator FP64 1e-13,
+FP32 2e-6; coarse links FP64 1e-12; the Galerkin identity D_c = R D P through
+the library's own kernels 1e-12.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import gauss, rel
+
+pytestmark = pytest.mark.gpu
+MASS, CSW = -0.1, 1.0
+
+
+def fine_setup(gpu, dims, seed=31):
+    U = oracle.su3_gauge(dims, 0.4, seed, 1)
+    fop = gpu.FineOp(dims, MASS, CSW)
+    fop.set_gauge(gpu.DeviceBuffer.from_numpy(U))
+    return fop, U
+
+
+def test_random_gauge_matches_oracle(gpu):
+    dims = (4, 2, 2, 4)
+    V = int(np.prod(dims))
+    d = gpu.DeviceBuffer(4 * V * 9 * 16)
+    gpu.random_gauge(d, dims, 0.4, 77, 3)
+    got = d.to_numpy(np.complex128, (4, V, 3, 3))
+    ref = oracle.su3_gauge(dims, 0.4, 77, 3)
+    assert np.abs(got - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("dims", [(4, 4, 2, 2), (2, 2, 4, 6)])
+def test_clover_term_matches_oracle(gpu, dims):
+    fop, U = fine_setup(gpu, dims)
+    V = int(np.prod(dims))
+    d = gpu.DeviceBuffer(V * 72 * 16)
+    fop.get_diag(d)
+    gpu.synchronize()
+    blocks = d.to_numpy(np.complex128, (V, 2, 6, 6))
+    ref = oracle.wilson_diag(dims, U, MASS, CSW)
+    for l in range(2):
+        assert np.abs(blocks[:, l] - ref[:, 6 * l:6 * l + 6, 6 * l:6 * l + 6]).max() < 1e-13
+    assert np.abs(ref[:, :6, 6:]).max() < 1e-14  # off-chirality blocks vanish
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_fine_operator_matches_oracle(gpu, prec, nrhs):
+    dims = (4, 4, 2, 6)
+    V = int(np.prod(dims))
+    fop, U = fine_setup(gpu, dims, 41)
+    diag = oracle.wilson_diag(dims, U, MASS, CSW)
+    dt = gpu.complex_dtype(prec)
+    psi = gauss(V * nrhs * 12, 42).reshape(V, nrhs, 12)
+    pq = psi.astype(dt).astype(np.complex128)
+    tol = 1e-13 if prec == 0 else 2e-6
+    din = gpu.DeviceBuffer.from_numpy(psi.astype(dt))
+    dout = gpu.DeviceBuffer(psi.size * np.dtype(dt).itemsize)
+
+    def got():
+        gpu.synchronize()
+        return dout.to_numpy(dt, (V, nrhs, 12)).astype(np.complex128)
+
+    g5 = np.repeat([1.0, 1.0, -1.0, -1.0], 3)
+    for r in range(nrhs):
+        pr = np.ascontiguousarray(pq[:, r])
+        fop.M(din, dout, prec, nrhs)
+        assert rel(oracle.wilson_M(dims, U, diag, pr), got()[:, r]) < tol
+        fop.Mdag(din, dout, prec, nrhs)
+        ref = g5 * oracle.wilson_M(dims, U, diag, pr * g5)
+        assert rel(ref, got()[:, r]) < tol
+        fop.Mdiag(din, dout, prec, nrhs)
+        assert rel(oracle.wilson_diag_apply(diag, pr), got()[:, r]) < tol
+        for mu in range(4):
+            for disp in (1, -1):
+                fop.Mdir(din, dout, mu, disp, prec, nrhs)
+                assert rel(oracle.wilson_dir(dims, U, pr, mu, disp), got()[:, r]) < tol
+
+
+CO_CASES = [((8, 8, 8, 8), (4, 4, 4, 4), 12),   # the reference's CPU case: 8^4 -> 2^4, N = 24
+            ((6, 4, 2, 2), (2, 2, 2, 2), 4),    # odd coarse extent 3
+            ((4, 4, 2, 2), (4, 2, 2, 2), 8),    # coarse extents 1 and 2
+            ((4, 4, 4, 4), (2, 2, 2, 2), 16)]
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("fd,blk,nb", CO_CASES)
+def test_coarsen_operator_matches_oracle(gpu, fd, blk, nb, mode):
+    print(f"[case] coarsen fd={fd} blk={blk} nb={nb} mode={mode}", flush=True)
+    Vf = int(np.prod(fd))
+    fop, U = fine_setup(gpu, fd, 51)
+    diag = oracle.wilson_diag(fd, U, MASS, CSW)
+    B = oracle.orthogonalise(fd, blk, nb, gauss(nb * Vf * 12, 52).reshape(nb, Vf, 12), 2)
+    agg = gpu.Aggregation(fd, blk, nb)
+    agg.set_basis(gpu.DeviceBuffer.from_numpy(B))
+    cdims = oracle.coarse_dims(fd, blk)
+    Vc, N = int(np.prod(cdims)), 2 * nb
+    cop = gpu.CoarseOp(cdims, N, gpu.F64)
+    gpu.coarsen_operator(cop, fop, agg, mode)
+    d = gpu.DeviceBuffer(9 * Vc * N * N * 16)
+    cop.get_links(d, gpu.LAYOUT_FLAT)
+    gpu.synchronize()
+    got = d.to_numpy(np.complex128, (9, Vc, N, N))
+    ref = oracle.coarsen_operator(fd, blk, nb, U, diag, B)
+    for p in range(9):
+        assert rel(ref[p], got[p]) < 1e-12, f"point {p}"
+
+
+def test_construct_remaining_links_matches_oracle(gpu):
+    dims, N = (4, 2, 2, 2), 8
+    V = int(np.prod(dims))
+    L = gauss(9 * V * N * N, 61).reshape(9, V, N, N)
+    for disp_have in (-1, 1):
+        cop = gpu.CoarseOp(dims, N, gpu.F64)
+        cop.set_links(gpu.DeviceBuffer.from_numpy(L), gpu.LAYOUT_FLAT, invert=False)
+        gpu.construct_remaining_links(cop, disp_have)
+        d = gpu.DeviceBuffer(9 * V * N * N * 16)
+        cop.get_links(d, gpu.LAYOUT_FLAT)
+        gpu.synchronize()
+        got = d.to_numpy(np.complex128, (9, V, N, N))
+        ref = oracle.construct_remaining_links(dims, N, L, disp_have).reshape(9, V, N, N)
+        assert np.abs(got - ref).max() < 1e-15
+
+
+def test_full_size_galerkin_identity_cfg4(gpu):
+    """Fine 32^4, 4^4 blocks, nb = 32 (coarse 8^4, N = 64): D_c v = R D P v on
+    random coarse vectors, every operator the library's own, gauge field and
+    basis generated on the device."""
+    fd, blk, nb = (32, 32, 32, 32), (4, 4, 4, 4), 32
+    Vf, Vc, N = 32 ** 4, 8 ** 4, 64
+    print(f"[case] full-size coarsen fd={fd} nb={nb}", flush=True)
+    s = gpu.Stream()
+    g = gpu.DeviceBuffer(4 * Vf * 9 * 16)
+    gpu.random_gauge(g, fd, 0.4, 5, 1, s)
+    fop = gpu.FineOp(fd, MASS, CSW)
+    fop.set_gauge(g, s)
+    agg = gpu.Aggregation(fd, blk, nb)
+    basis = gpu.DeviceBuffer(nb * Vf * 12 * 16)
+    gpu.random_gaussian(basis, nb * Vf * 12, gpu.F64, 5, 2, s)
+    agg.set_basis(basis, s)
+    agg.orthogonalise(2, s)
+    cop = gpu.CoarseOp((8, 8, 8, 8), N, gpu.F64)
+    e0, e1 = gpu.Event(), gpu.Event()
+    e0.record(s)
+    gpu.coarsen_operator(cop, fop, agg, gpu.COARSEN_DIRSAVE, s)
+    e1.record(s)
+    s.sync()
+    print(f"  CoarsenOperator (dirsave, FP64) {e0.elapsed_ms(e1):.1f} ms", flush=True)
+    vc = gpu.DeviceBuffer(Vc * N * 16)
+    gpu.random_gaussian(vc, Vc * N, gpu.F64, 5, 3, s)
+    lhs = gpu.DeviceBuffer(Vc * N * 16)
+    cop.M(vc, lhs, s)
+    f1 = gpu.DeviceBuffer(Vf * 12 * 16)
+    f2 = gpu.DeviceBuffer(Vf * 12 * 16)
+    rhs = gpu.DeviceBuffer(Vc * N * 16)
+    agg.promote(vc, f1, gpu.F64, 1, s)
+    fop.M(f1, f2, gpu.F64, 1, s)
+    agg.project(f2, rhs, gpu.F64, 1, s)
+    s.sync()
+    a = lhs.to_numpy(np.complex128, (Vc, N))
+    b = rhs.to_numpy(np.complex128, (Vc, N))
+    assert rel(b, a) < 1e-12

@@ -1,13 +1,15 @@
 // Block transfer operators (Aggregation): basis import/export, block
 // projection (restriction R), promotion (prolongation P), block modified
-// Gram-Schmidt in FP64, and the orthogonality check.
+// Gram-Schmidt in FP64, the orthogonality check, and the masked projections
+// of the coarse-operator build.
 //
-// Reference semantics (chirality-split form, Ns_c = 2, Ns_b = 2):
+// Reference semantics (chirality-split form, Ns_c = 2):
 //   include/AggregationKernelsRework.h:96-157   aggregateProjectFast
 //   include/AggregationKernelsRework.h:207-264  aggregatePromoteFast
 //   include/AggregationKernelsRework.h:266-341  aggregateOrthogonalise
 //   include/AggregationRework.h:64-146          class Aggregation
-//   include/CoarseningLookupTableRework.h:121-179 (block <-> fine site tables)
+//   include/CoarseningLookupTableRework.h:121-179 (block <-> fine site tables,
+//                                               inner/outer masks of a point)
 #include "aggregation.hpp"
 
 #include <vector>
@@ -17,7 +19,7 @@ namespace mgc {
 // ---------------------------------------------------------------------------
 // layout conversion
 // ---------------------------------------------------------------------------
-__global__ void k_import_basis(const cplx<double>* __restrict__ src, int nb, int Vf, int Vb,
+__global__ void k_import_basis(const cplx<double>* __restrict__ src, int nb, int Vf, int Ef,
                                size_t total, const int* __restrict__ blk_fine,
                                cplx<double>* __restrict__ dst) {
   size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -25,14 +27,14 @@ __global__ void k_import_basis(const cplx<double>* __restrict__ src, int nb, int
   for (; t < total; t += stride) {
     const int i = (int)(t % nb);
     size_t r = t / nb;
-    const int e = (int)(r % 12);
-    r /= 12;  // r = xc*Vb + xb
+    const int e = (int)(r % Ef);
+    r /= Ef;  // r = xc*Vb + xb
     const int sf = blk_fine[r];
-    dst[t] = src[((size_t)i * Vf + sf) * 12 + e];
+    dst[t] = src[((size_t)i * Vf + sf) * Ef + e];
   }
 }
 
-__global__ void k_export_basis(const cplx<double>* __restrict__ src, int nb, int Vf, int Vb,
+__global__ void k_export_basis(const cplx<double>* __restrict__ src, int nb, int Vf, int Ef,
                                size_t total, const int* __restrict__ blk_fine,
                                cplx<double>* __restrict__ dst) {
   size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -40,10 +42,26 @@ __global__ void k_export_basis(const cplx<double>* __restrict__ src, int nb, int
   for (; t < total; t += stride) {
     const int i = (int)(t % nb);
     size_t r = t / nb;
-    const int e = (int)(r % 12);
-    r /= 12;
+    const int e = (int)(r % Ef);
+    r /= Ef;
     const int sf = blk_fine[r];
-    dst[((size_t)i * Vf + sf) * 12 + e] = src[t];
+    dst[((size_t)i * Vf + sf) * Ef + e] = src[t];
+  }
+}
+
+// fine field [Vf][Ef] = chirality-l part of basis vector i (l = -1: both)
+template <typename R>
+__global__ void k_extract_basis(const cplx<double>* __restrict__ U, int nb, int Ef, int i, int l,
+                                size_t total, const int* __restrict__ blk_fine,
+                                cplx<R>* __restrict__ out) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < total; t += stride) {  // t = (xc*Vb + xb)*Ef + e
+    const int e = (int)(t % Ef);
+    const int sf = blk_fine[t / Ef];
+    cplx<double> v = U[t * nb + i];
+    if (l >= 0 && (2 * e >= Ef) != (l == 1)) v = {0.0, 0.0};
+    out[(size_t)sf * Ef + e] = {(R)v.re, (R)v.im};
   }
 }
 
@@ -56,7 +74,7 @@ __global__ void k_basis_to_f32(const cplx<double>* __restrict__ src, cplx<float>
 
 const cplx<float>* agg_basis32(mgc_aggregation* a, hipStream_t s) {
   if (!a->basis32_valid) {
-    const size_t n = (size_t)a->Vc * a->Vb * 12 * a->nb;
+    const size_t n = (size_t)a->Vc * a->Vb * a->Ef * a->nb;
     if (!a->basis32) MGC_HIP(hipMalloc(&a->basis32, n * sizeof(cplx<float>)));
     k_basis_to_f32<<<grid_for(n, 256, 16384), 256, 0, s>>>(a->basis64, a->basis32, n);
     MGC_LAUNCH_CHECK();
@@ -65,8 +83,19 @@ const cplx<float>* agg_basis32(mgc_aggregation* a, hipStream_t s) {
   return a->basis32;
 }
 
+// Site mask of a block site: kind 0 = all, 1 = inner of point p, 2 = outer of
+// point p (outer for +mu: x_mu mod b = b-1; for -mu: x_mu mod b = 0).
+__device__ inline bool site_in_mask(const Lat4& blk, int xb, int kind, int p) {
+  if (kind == 0 || p == 8) return kind != 2;
+  int c[4];
+  blk.coords(xb, c);
+  const int mu = p & 3;
+  const bool outer = c[mu] == (p < 4 ? blk.L[mu] - 1 : 0);
+  return kind == 2 ? outer : !outer;
+}
+
 // ---------------------------------------------------------------------------
-// project: c(xc)(rhs)(k, i) = sum_{xb, e in k} conj(U(xc,xb,e,i)) f(fine(xc,xb))(rhs)(e)
+// project: c(xc)(rhs)(k, i) = sum_{xb in mask, e in k} conj(U(xc,xb,e,i)) f(fine(xc,xb))(rhs)(e)
 // One workgroup per (coarse site, chunk of RC right-hand sides).  Thread
 // (i = t % nb, g = t / nb) walks rows q = g, g+G, ... of the block panel:
 // the nb threads of a group read one row's nb basis entries (contiguous) and
@@ -80,7 +109,7 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ f,
                                                  cplx<R>* __restrict__ c,
                                                  const int* __restrict__ blk_fine, int nb, int Vb,
-                                                 int nrhs) {
+                                                 int Ef, int nrhs, Lat4 blk, int kind, int point) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* s_red = reinterpret_cast<cplx<R>*>(smem_raw);  // [G][RC][2][nb]
   const int xc = blockIdx.x;
@@ -89,22 +118,24 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
   const int tid = threadIdx.x;
   const int G = blockDim.x / nb;
   const int i = tid % nb, g = tid / nb;
+  const int half = Ef / 2;
   cplx<R> acc[RC][2];
 #pragma unroll
   for (int r = 0; r < RC; ++r) acc[r][0] = acc[r][1] = {R(0), R(0)};
   if (g < G) {
-    const int rows = Vb * 12;
+    const int rows = Vb * Ef;
     const cplx<R>* Ub = U + (size_t)xc * rows * nb;
     const int* bf = blk_fine + (size_t)xc * Vb;
     for (int q = g; q < rows; q += G) {
-      const int xb = q / 12, e = q - xb * 12;
-      const int k = e / 6;
+      const int xb = q / Ef, e = q - xb * Ef;
+      if (kind != 0 && !site_in_mask(blk, xb, kind, point)) continue;
+      const int k = e >= half;
       const cplx<R> u = Ub[(size_t)q * nb + i];
-      const cplx<R>* fr = f + ((size_t)bf[xb] * nrhs + r0) * 12 + e;
+      const cplx<R>* fr = f + ((size_t)bf[xb] * nrhs + r0) * Ef + e;
 #pragma unroll
       for (int r = 0; r < RC; ++r)
         if (r < nr) {
-          const cplx<R> x = fr[(size_t)r * 12];
+          const cplx<R> x = fr[(size_t)r * Ef];
           if (k == 0) cmac_conj(acc[r][0], u, x);
           else cmac_conj(acc[r][1], u, x);
         }
@@ -122,7 +153,7 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
 }
 
 // ---------------------------------------------------------------------------
-// promote: f(fine(xc,xb))(rhs)(e) = sum_i c(xc)(rhs)(e/6, i) U(xc,xb,e,i)
+// promote: f(fine(xc,xb))(rhs)(e) = sum_i c(xc)(rhs)(chirality(e), i) U(xc,xb,e,i)
 // W lanes (W = next power of two >= nb, W <= 64) per row: each lane loads one
 // basis entry, multiplies by the LDS-resident coarse coefficient, and the W
 // lanes reduce with cross-lane shuffles.
@@ -132,27 +163,28 @@ __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ c,
                                                  cplx<R>* __restrict__ f,
                                                  const int* __restrict__ blk_fine, int nb, int Vb,
-                                                 int nrhs, int W) {
+                                                 int Ef, int nrhs, int W) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* s_c = reinterpret_cast<cplx<R>*>(smem_raw);  // [RC][2][nb]
   const int xc = blockIdx.x;
   const int r0 = blockIdx.y * RC;
   const int nr = min(RC, nrhs - r0);
   const int tid = threadIdx.x;
+  const int half = Ef / 2;
   for (int o = tid; o < nr * 2 * nb; o += blockDim.x)
     s_c[o] = c[((size_t)xc * nrhs + r0) * 2 * nb + o];
   __syncthreads();
   const int lane = tid % W;
   const int rpp = blockDim.x / W;  // rows per pass
-  const int rows = Vb * 12;
+  const int rows = Vb * Ef;
   const cplx<R>* Ub = U + (size_t)xc * rows * nb;
   const int* bf = blk_fine + (size_t)xc * Vb;
   for (int q0 = 0; q0 < rows; q0 += rpp) {
     const int q = q0 + tid / W;
     const bool active = q < rows;
     const int qq = active ? q : 0;
-    const int xb = qq / 12, e = qq - xb * 12;
-    const int k = e / 6;
+    const int xb = qq / Ef, e = qq - xb * Ef;
+    const int k = e >= half;
     cplx<R> u = {R(0), R(0)};
     if (active && lane < nb) u = Ub[(size_t)qq * nb + lane];
     cplx<R> acc[RC];
@@ -168,17 +200,17 @@ __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
         acc[r].im += __shfl_xor(acc[r].im, off, W);
       }
     if (active && lane == 0)
-      for (int r = 0; r < nr; ++r) f[((size_t)bf[xb] * nrhs + r0 + r) * 12 + e] = acc[r];
+      for (int r = 0; r < nr; ++r) f[((size_t)bf[xb] * nrhs + r0 + r) * Ef + e] = acc[r];
   }
 }
 
 // ---------------------------------------------------------------------------
 // block modified Gram-Schmidt, FP64.  One workgroup per (coarse site,
-// chirality); thread t owns panel rows t, t+256, ... (at most RPT rows) and
+// chirality); thread t owns panel rows t, t+T, ... (at most RPT rows) and
 // only ever touches its own rows, so the only communication is the
 // workgroup-wide reductions of the inner products and norms.
 // ---------------------------------------------------------------------------
-constexpr int MGS_THREADS = 256;
+constexpr int MGS_THREADS = 512;
 constexpr int MGS_RPT = 8;
 
 __device__ inline double2 wg_sum2(double a, double b, double* s_buf) {
@@ -202,20 +234,22 @@ __device__ inline double2 wg_sum2(double a, double b, double* s_buf) {
 }
 
 __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __restrict__ U,
-                                                                int nb, int Vb, int passes) {
+                                                                int nb, int Vb, int Ef,
+                                                                int passes) {
   __shared__ double s_buf[2 * (MGS_THREADS / 64)];
   const int xc = blockIdx.x >> 1;
   const int k = blockIdx.x & 1;
-  const int R6 = Vb * 6;
+  const int half = Ef / 2;
+  const int Rk = Vb * half;
   size_t rowoff[MGS_RPT];
   int nrow = 0;
 #pragma unroll
   for (int j = 0; j < MGS_RPT; ++j) {
     const int r = threadIdx.x + j * MGS_THREADS;
-    if (r < R6) nrow = j + 1;
-    const int rr = r < R6 ? r : 0;
-    const int xb = rr / 6, e = 6 * k + rr % 6;
-    rowoff[j] = agg_row(nb, Vb, xc, xb, e);
+    if (r < Rk) nrow = j + 1;
+    const int rr = r < Rk ? r : 0;
+    const int xb = rr / half, e = half * k + rr % half;
+    rowoff[j] = agg_row(nb, Vb, Ef, xc, xb, e);
   }
   for (int pass = 0; pass < passes; ++pass) {
     for (int v = 0; v < nb; ++v) {
@@ -254,15 +288,16 @@ __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __r
 
 // errs[i] += sum_{k, j} |delta_ij - <U_j, U_i>_{block,k}|^2 (CheckOrthogonal)
 __global__ __launch_bounds__(256) void k_check_orthogonal(const cplx<double>* __restrict__ U,
-                                                          int nb, int Vb,
+                                                          int nb, int Vb, int Ef,
                                                           double* __restrict__ errs) {
   const int xc = blockIdx.x >> 1;
   const int k = blockIdx.x & 1;
+  const int half = Ef / 2;
   for (int pair = threadIdx.x; pair < nb * nb; pair += blockDim.x) {
     const int i = pair / nb, j = pair % nb;
     cplx<double> g = {0.0, 0.0};
-    for (int r = 0; r < Vb * 6; ++r) {
-      const size_t off = agg_row(nb, Vb, xc, r / 6, 6 * k + r % 6);
+    for (int r = 0; r < Vb * half; ++r) {
+      const size_t off = agg_row(nb, Vb, Ef, xc, r / half, half * k + r % half);
       cmac_conj(g, U[off + j], U[off + i]);
     }
     const double dr = (i == j ? 1.0 : 0.0) - g.re;
@@ -289,19 +324,74 @@ static int next_pow2(int n) {
   return w;
 }
 
+void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int nrhs,
+                  bool project, int kind, int point, hipStream_t s) {
+  check_agg(a);
+  MGC_CHECK(in && out && in != out, MGC_ERR_ARG, "null or aliased vector");
+  MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
+  const dim3 grid(a->Vc, (nrhs + RC - 1) / RC);
+  const int nb = a->nb;
+  if (project) {
+    const int block = (256 / nb) * nb;
+    const size_t sh = (size_t)(block / nb) * RC * 2 * nb *
+                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
+    if (prec == MGC_F64)
+      k_project<double><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
+                                                (cplx<double>*)out, a->blk_fine, nb, a->Vb, a->Ef,
+                                                nrhs, a->blk, kind, point);
+    else if (prec == MGC_F32)
+      k_project<float><<<grid, block, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
+                                               (cplx<float>*)out, a->blk_fine, nb, a->Vb, a->Ef,
+                                               nrhs, a->blk, kind, point);
+    else
+      throw Error(MGC_ERR_PRECISION, "project: F64 or F32 vectors");
+  } else {
+    const int W = next_pow2(nb);
+    const size_t sh = (size_t)RC * 2 * nb *
+                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
+    if (prec == MGC_F64)
+      k_promote<double><<<grid, 256, sh, s>>>(a->basis64, (const cplx<double>*)in,
+                                              (cplx<double>*)out, a->blk_fine, nb, a->Vb, a->Ef,
+                                              nrhs, W);
+    else if (prec == MGC_F32)
+      k_promote<float><<<grid, 256, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
+                                             (cplx<float>*)out, a->blk_fine, nb, a->Vb, a->Ef,
+                                             nrhs, W);
+    else
+      throw Error(MGC_ERR_PRECISION, "promote: F64 or F32 vectors");
+  }
+  MGC_LAUNCH_CHECK();
+}
+
+void agg_extract_basis(mgc_aggregation* a, int i, int l, void* out, int prec, hipStream_t s) {
+  check_agg(a);
+  const size_t total = (size_t)a->Vc * a->Vb * a->Ef;
+  const unsigned g = grid_for(total, 256, 16384);
+  if (prec == MGC_F64)
+    k_extract_basis<double><<<g, 256, 0, s>>>(a->basis64, a->nb, a->Ef, i, l, total, a->blk_fine,
+                                              (cplx<double>*)out);
+  else
+    k_extract_basis<float><<<g, 256, 0, s>>>(a->basis64, a->nb, a->Ef, i, l, total, a->blk_fine,
+                                             (cplx<float>*)out);
+  MGC_LAUNCH_CHECK();
+}
+
 }  // namespace mgc
 
 using namespace mgc;
 
 extern "C" {
 
-int mgc_aggregation_create(mgc_aggregation** out, const int fine_dims[4], const int block[4],
-                           int nb) {
+int mgc_aggregation_create_ex(mgc_aggregation** out, const int fine_dims[4], const int block[4],
+                              int nb, int site_elems) {
   mgc_aggregation* a = nullptr;
   int st = guarded([&] {
     MGC_CHECK(out && fine_dims && block, MGC_ERR_ARG, "null argument");
     MGC_CHECK(nb >= 1 && nb <= 64, MGC_ERR_SHAPE, "nb must be in 1..64");
+    MGC_CHECK(site_elems >= 2 && site_elems % 2 == 0 && site_elems <= 256, MGC_ERR_SHAPE,
+              "site elements must be even (two chiralities) and <= 256");
     a = new mgc_aggregation();
+    a->Ef = site_elems;
     for (int d = 0; d < 4; ++d) {
       MGC_CHECK(fine_dims[d] >= 1 && block[d] >= 1, MGC_ERR_SHAPE, "extent < 1");
       MGC_CHECK(fine_dims[d] % block[d] == 0, MGC_ERR_SHAPE,
@@ -331,11 +421,16 @@ int mgc_aggregation_create(mgc_aggregation** out, const int fine_dims[4], const 
     MGC_HIP(hipMemcpy(a->blk_fine, bf.data(), bf.size() * sizeof(int), hipMemcpyHostToDevice));
     MGC_HIP(hipMalloc(&a->fine_row, fr.size() * sizeof(int)));
     MGC_HIP(hipMemcpy(a->fine_row, fr.data(), fr.size() * sizeof(int), hipMemcpyHostToDevice));
-    MGC_HIP(hipMalloc(&a->basis64, (size_t)a->Vf * 12 * nb * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&a->basis64, (size_t)a->Vf * a->Ef * nb * sizeof(cplx<double>)));
     *out = a;
   });
   if (st != MGC_OK) free_agg(a);
   return st;
+}
+
+int mgc_aggregation_create(mgc_aggregation** out, const int fine_dims[4], const int block[4],
+                           int nb) {
+  return mgc_aggregation_create_ex(out, fine_dims, block, nb, 12);
 }
 
 int mgc_aggregation_destroy(mgc_aggregation* a) {
@@ -357,10 +452,10 @@ int mgc_aggregation_info(const mgc_aggregation* a, int coarse_dims[4], int* nb, 
 int mgc_aggregation_set_basis(mgc_aggregation* a, const void* basis_f64, void* stream) {
   return guarded([&] {
     MGC_CHECK(a && basis_f64, MGC_ERR_ARG, "null argument");
-    const size_t total = (size_t)a->Vf * 12 * a->nb;
+    const size_t total = (size_t)a->Vf * a->Ef * a->nb;
     hipStream_t s = as_stream(stream);
     k_import_basis<<<grid_for(total, 256, 16384), 256, 0, s>>>(
-        (const cplx<double>*)basis_f64, a->nb, a->Vf, a->Vb, total, a->blk_fine, a->basis64);
+        (const cplx<double>*)basis_f64, a->nb, a->Vf, a->Ef, total, a->blk_fine, a->basis64);
     MGC_LAUNCH_CHECK();
     a->basis_set = true;
     a->basis32_valid = false;
@@ -371,9 +466,9 @@ int mgc_aggregation_get_basis(const mgc_aggregation* a, void* basis_f64, void* s
   return guarded([&] {
     check_agg(a);
     MGC_CHECK(basis_f64, MGC_ERR_ARG, "null output");
-    const size_t total = (size_t)a->Vf * 12 * a->nb;
+    const size_t total = (size_t)a->Vf * a->Ef * a->nb;
     k_export_basis<<<grid_for(total, 256, 16384), 256, 0, as_stream(stream)>>>(
-        a->basis64, a->nb, a->Vf, a->Vb, total, a->blk_fine, (cplx<double>*)basis_f64);
+        a->basis64, a->nb, a->Vf, a->Ef, total, a->blk_fine, (cplx<double>*)basis_f64);
     MGC_LAUNCH_CHECK();
   });
 }
@@ -382,11 +477,12 @@ int mgc_aggregation_orthogonalise(mgc_aggregation* a, int passes, void* stream) 
   return guarded([&] {
     check_agg(a);
     MGC_CHECK(passes >= 0, MGC_ERR_ARG, "passes < 0");
-    MGC_CHECK(a->Vb * 6 <= MGS_THREADS * MGS_RPT, MGC_ERR_SHAPE,
-              "block too large for the in-register Gram-Schmidt (|block| <= 341)");
+    MGC_CHECK(a->Vb * a->Ef / 2 <= MGS_THREADS * MGS_RPT, MGC_ERR_SHAPE,
+              "block too large for the in-register Gram-Schmidt (|block| * site elements / 2 "
+              "<= 4096)");
     if (passes == 0) return;
     k_orthogonalise<<<2 * a->Vc, MGS_THREADS, 0, as_stream(stream)>>>(a->basis64, a->nb, a->Vb,
-                                                                     passes);
+                                                                     a->Ef, passes);
     MGC_LAUNCH_CHECK();
     a->basis32_valid = false;
   });
@@ -400,7 +496,7 @@ int mgc_aggregation_check_orthogonal(mgc_aggregation* a, double* errs_host, void
     double* d = nullptr;
     MGC_HIP(hipMalloc(&d, a->nb * sizeof(double)));
     fill_zero(d, a->nb * sizeof(double), s);
-    k_check_orthogonal<<<2 * a->Vc, 256, 0, s>>>(a->basis64, a->nb, a->Vb, d);
+    k_check_orthogonal<<<2 * a->Vc, 256, 0, s>>>(a->basis64, a->nb, a->Vb, a->Ef, d);
     hipError_t e1 = hipGetLastError();
     hipError_t e2 = hipMemcpyAsync(errs_host, d, a->nb * sizeof(double), hipMemcpyDeviceToHost, s);
     hipError_t e3 = hipStreamSynchronize(s);
@@ -411,49 +507,25 @@ int mgc_aggregation_check_orthogonal(mgc_aggregation* a, double* errs_host, void
   });
 }
 
-static void transfer(mgc_aggregation* a, const void* in, void* out, int prec, int nrhs,
-                     bool project, hipStream_t s) {
-  check_agg(a);
-  MGC_CHECK(in && out && in != out, MGC_ERR_ARG, "null or aliased vector");
-  MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
-  const dim3 grid(a->Vc, (nrhs + RC - 1) / RC);
-  const int nb = a->nb;
-  if (project) {
-    const int block = (256 / nb) * nb;
-    const size_t sh = (size_t)(block / nb) * RC * 2 * nb *
-                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
-    if (prec == MGC_F64)
-      k_project<double><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
-                                                (cplx<double>*)out, a->blk_fine, nb, a->Vb, nrhs);
-    else if (prec == MGC_F32)
-      k_project<float><<<grid, block, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
-                                               (cplx<float>*)out, a->blk_fine, nb, a->Vb, nrhs);
-    else
-      throw Error(MGC_ERR_PRECISION, "project: F64 or F32 vectors");
-  } else {
-    const int W = next_pow2(nb);
-    const size_t sh = (size_t)RC * 2 * nb *
-                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
-    if (prec == MGC_F64)
-      k_promote<double><<<grid, 256, sh, s>>>(a->basis64, (const cplx<double>*)in,
-                                              (cplx<double>*)out, a->blk_fine, nb, a->Vb, nrhs, W);
-    else if (prec == MGC_F32)
-      k_promote<float><<<grid, 256, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
-                                             (cplx<float>*)out, a->blk_fine, nb, a->Vb, nrhs, W);
-    else
-      throw Error(MGC_ERR_PRECISION, "promote: F64 or F32 vectors");
-  }
-  MGC_LAUNCH_CHECK();
-}
-
 int mgc_aggregation_project(mgc_aggregation* a, const void* fine, void* coarse, int prec,
                             int nrhs, void* stream) {
-  return guarded([&] { transfer(a, fine, coarse, prec, nrhs, true, as_stream(stream)); });
+  return guarded([&] { agg_transfer(a, fine, coarse, prec, nrhs, true, 0, 8, as_stream(stream)); });
+}
+
+int mgc_aggregation_project_masked(mgc_aggregation* a, const void* fine, void* coarse, int prec,
+                                   int nrhs, int mask_kind, int point, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(mask_kind >= 0 && mask_kind <= 2, MGC_ERR_ARG, "mask kind must be 0, 1 or 2");
+    MGC_CHECK(point >= 0 && point <= 8, MGC_ERR_ARG, "point must be in 0..8");
+    agg_transfer(a, fine, coarse, prec, nrhs, true, mask_kind, point, as_stream(stream));
+  });
 }
 
 int mgc_aggregation_promote(mgc_aggregation* a, const void* coarse, void* fine, int prec,
                             int nrhs, void* stream) {
-  return guarded([&] { transfer(a, coarse, fine, prec, nrhs, false, as_stream(stream)); });
+  return guarded([&] {
+    agg_transfer(a, coarse, fine, prec, nrhs, false, 0, 8, as_stream(stream));
+  });
 }
 
 }  // extern "C"

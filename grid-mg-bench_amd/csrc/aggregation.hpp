@@ -3,14 +3,17 @@
 #include "common.hpp"
 
 // Basis storage (FP64 master copy, FP32 copy refreshed on demand):
-//   [coarse site xc][block site xb][spin-colour e 12][basis i nb]
-// i.e. for each block a (|B|*12) x nb row-major panel, so that a wavefront
+//   [coarse site xc][block site xb][site element e Ef][basis i nb]
+// i.e. for each block a (|B|*Ef) x nb row-major panel, so that a wavefront
 // reading one row's nb entries and the next rows is one contiguous stream
 // (project, promote, Gram-Schmidt and the coarse-operator build all walk
-// these panels).  Chirality of row e is e / 6 (spin / 2).
+// these panels).  Ef = 12 (spin-colour) on a fine Dirac field, Ef = N on a
+// coarse field; the chirality of element e is e / (Ef/2) (spin / 2 for a
+// Dirac field, the coarse chirality n / nb for a coarse one).
 struct mgc_aggregation {
   mgc::Lat4 fine, coarse, blk;
   int Vf = 0, Vc = 0, Vb = 0, nb = 0;
+  int Ef = 12;
   mgc::cplx<double>* basis64 = nullptr;
   mgc::cplx<float>* basis32 = nullptr;
   bool basis_set = false;
@@ -22,7 +25,12 @@ struct mgc_aggregation {
 namespace mgc {
 // FP32 copy of the basis, refreshed when stale
 const cplx<float>* agg_basis32(mgc_aggregation* a, hipStream_t s);
-__host__ __device__ inline size_t agg_row(int nb, int Vb, int xc, int xb, int e) {
-  return (((size_t)xc * Vb + xb) * 12 + e) * nb;
+// project (mask kind 0 all / 1 inner of point / 2 outer of point) or promote
+void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int nrhs,
+                  bool project, int kind, int point, hipStream_t s);
+// fine field [Vf][Ef] holding the chirality-l part of basis vector i (l = -1: all)
+void agg_extract_basis(mgc_aggregation* a, int i, int l, void* out, int prec, hipStream_t s);
+__host__ __device__ inline size_t agg_row(int nb, int Vb, int Ef, int xc, int xb, int e) {
+  return (((size_t)xc * Vb + xb) * Ef + e) * nb;
 }
 }  // namespace mgc

@@ -242,6 +242,73 @@ static void set_links_or_throw(mgc_coarse_op* cop, const void* links, int invert
   if (st != MGC_OK) throw Error(st, mgc_last_error());
 }
 
+// links[p][xc][j][i] (=, or += when accumulate) proj[xc][j]
+__global__ void k_scatter_column(cplx<double>* __restrict__ links, const cplx<double>* __restrict__ proj,
+                                 int Vc, int N, int p, int i, int accumulate) {
+  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (t >= (size_t)Vc * N) return;
+  const size_t xc = t / N, j = t % N;
+  cplx<double>& dst = links[(((size_t)p * Vc + xc) * N + j) * N + i];
+  const cplx<double> v = proj[t];
+  if (accumulate) dst = cadd(dst, v);
+  else dst = v;
+}
+
+static void check_call(int st) {
+  if (st != MGC_OK) throw Error(st, mgc_last_error());
+}
+
+// Galerkin product of a coarse operator on the next level's basis, in the
+// inner/outer-mask form of include/CoarsenedMatrixBaseline.h:609-675 (one
+// doubled basis vector at a time: Mdir_p v_i, masked projections, links column
+// i), FP64.
+static void coarsen_coarse(mgc_coarse_op* cop2, mgc_coarse_op* cop1, mgc_aggregation* agg,
+                           int mode, hipStream_t s) {
+  const int N2 = cop2->N, Vc2 = cop2->V, Vf = agg->Vf, Ef = agg->Ef, nb = agg->nb;
+  cplx<double>* links = nullptr;
+  cplx<double>* phi = nullptr;
+  cplx<double>* mphi = nullptr;
+  cplx<double>* proj = nullptr;
+  auto cleanup = [&] {
+    (void)hipStreamSynchronize(s);
+    void* bufs[] = {links, phi, mphi, proj};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+  };
+  try {
+    MGC_HIP(hipMalloc(&links, (size_t)9 * Vc2 * N2 * N2 * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&phi, (size_t)Vf * Ef * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&mphi, (size_t)Vf * Ef * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&proj, (size_t)Vc2 * N2 * sizeof(cplx<double>)));
+    fill_zero(links, (size_t)9 * Vc2 * N2 * N2 * sizeof(cplx<double>), s);
+    const unsigned g = (unsigned)(((size_t)Vc2 * N2 + 255) / 256);
+    for (int i = 0; i < N2; ++i) {
+      const int l = i / nb, c = i % nb;
+      agg_extract_basis(agg, c, l, phi, MGC_F64, s);
+      for (int p = 0; p < 9; ++p) {
+        if (p == 8) check_call(mgc_coarse_op_Mdiag(cop1, phi, mphi, (void*)s));
+        else check_call(mgc_coarse_op_Mdir(cop1, phi, mphi, p & 3, p < 4 ? 1 : -1, (void*)s));
+        // every point's inner part feeds the self link; DIRSAVE rebuilds the
+        // forward links instead of projecting them
+        if (p < 8 && !(mode == MGC_COARSEN_DIRSAVE && p < 4)) {
+          agg_transfer(agg, mphi, proj, MGC_F64, 1, true, 2, p, s);
+          k_scatter_column<<<g, 256, 0, s>>>(links, proj, Vc2, N2, p, i, 0);
+          MGC_LAUNCH_CHECK();
+        }
+        agg_transfer(agg, mphi, proj, MGC_F64, 1, true, 1, p, s);
+        k_scatter_column<<<g, 256, 0, s>>>(links, proj, Vc2, N2, 8, i, 1);
+        MGC_LAUNCH_CHECK();
+      }
+    }
+    if (mode == MGC_COARSEN_DIRSAVE) reconstruct(links, cop2, -1, s);
+    set_links_or_throw(cop2, links, 1, s);
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+  cleanup();
+}
+
 }  // namespace mgc
 
 using namespace mgc;
@@ -255,6 +322,7 @@ int mgc_coarsen_operator(mgc_coarse_op* cop, mgc_fine_op* fop, mgc_aggregation* 
     MGC_CHECK(mode == MGC_COARSEN_FULL || mode == MGC_COARSEN_DIRSAVE, MGC_ERR_ARG, "bad mode");
     MGC_CHECK(agg->basis_set, MGC_ERR_STATE, "aggregation basis not set");
     MGC_CHECK(fop->gauge_set, MGC_ERR_STATE, "fine gauge field not set");
+    MGC_CHECK(agg->Ef == 12, MGC_ERR_SHAPE, "the aggregation must act on Dirac (12-element) sites");
     for (int d = 0; d < 4; ++d) {
       MGC_CHECK(fop->lat.L[d] == agg->fine.L[d], MGC_ERR_SHAPE, "fine lattices differ");
       MGC_CHECK(cop->lat.L[d] == agg->coarse.L[d], MGC_ERR_SHAPE, "coarse lattices differ");
@@ -284,6 +352,25 @@ int mgc_coarsen_operator(mgc_coarse_op* cop, mgc_fine_op* fop, mgc_aggregation* 
     MGC_HIP(hipStreamSynchronize(s));
     MGC_HIP(hipFree(links));
     MGC_HIP(hipFree(plist));
+  });
+}
+
+int mgc_coarsen_coarse_operator(mgc_coarse_op* cop2, mgc_coarse_op* cop1, mgc_aggregation* agg,
+                                int mode, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(cop2 && cop1 && agg, MGC_ERR_ARG, "null handle");
+    MGC_CHECK(mode == MGC_COARSEN_FULL || mode == MGC_COARSEN_DIRSAVE, MGC_ERR_ARG, "bad mode");
+    MGC_CHECK(agg->basis_set, MGC_ERR_STATE, "aggregation basis not set");
+    MGC_CHECK(cop1->links_set, MGC_ERR_STATE, "level operator links not set");
+    MGC_CHECK(cop1->prec == MGC_F64, MGC_ERR_PRECISION,
+              "the operator being coarsened must be an F64 operator");
+    MGC_CHECK(agg->Ef == cop1->N, MGC_ERR_SHAPE, "aggregation site elements must equal nbasis");
+    for (int d = 0; d < 4; ++d) {
+      MGC_CHECK(cop1->lat.L[d] == agg->fine.L[d], MGC_ERR_SHAPE, "fine lattices differ");
+      MGC_CHECK(cop2->lat.L[d] == agg->coarse.L[d], MGC_ERR_SHAPE, "coarse lattices differ");
+    }
+    MGC_CHECK(cop2->N == 2 * agg->nb, MGC_ERR_SHAPE, "coarse nbasis must be 2 * nb");
+    coarsen_coarse(cop2, cop1, agg, mode, as_stream(stream));
   });
 }
 

@@ -81,6 +81,9 @@ _SIGS = {
     "mgc_fine_op_Mdir": ([_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i),
     "mgc_random_gauge": ([_vp, _ip, _c.c_double, _u64, _u64, _vp], _i),
     "mgc_coarsen_operator": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarsen_coarse_operator": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_aggregation_create_ex": ([_c.POINTER(_vp), _ip, _ip, _i, _i], _i),
+    "mgc_aggregation_project_masked": ([_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i),
     "mgc_coarse_op_construct_remaining_links": ([_vp, _i, _vp], _i),
     "mgc_aggregation_create": ([_c.POINTER(_vp), _ip, _ip, _i], _i),
     "mgc_aggregation_destroy": ([_vp], _i),
@@ -371,12 +374,14 @@ def quantize(inp, rows, n, prec, out, scales, stream=None):
 class Aggregation:
     """Handle of a block-transfer operator (the reference's Aggregation)."""
 
-    def __init__(self, fine_dims: Sequence[int], block: Sequence[int], nb: int):
+    def __init__(self, fine_dims: Sequence[int], block: Sequence[int], nb: int,
+                 site_elems: int = 12):
         h = _vp()
         fd = (_c.c_int * 4)(*[int(v) for v in fine_dims])
         bd = (_c.c_int * 4)(*[int(v) for v in block])
-        call("mgc_aggregation_create", _c.byref(h), fd, bd, int(nb))
+        call("mgc_aggregation_create_ex", _c.byref(h), fd, bd, int(nb), int(site_elems))
         self.handle = h.value
+        self.Ef = int(site_elems)
         self.fine_dims = tuple(int(v) for v in fine_dims)
         self.block = tuple(int(v) for v in block)
         self.coarse_dims = tuple(f // b for f, b in zip(self.fine_dims, self.block))
@@ -417,6 +422,10 @@ class Aggregation:
     def project(self, fine, coarse, prec=F64, nrhs=1, stream=None) -> None:
         call("mgc_aggregation_project", self.handle, _p(fine), _p(coarse), int(prec), int(nrhs),
              self._s(stream))
+
+    def project_masked(self, fine, coarse, kind, point, prec=F64, nrhs=1, stream=None) -> None:
+        call("mgc_aggregation_project_masked", self.handle, _p(fine), _p(coarse), int(prec),
+             int(nrhs), int(kind), int(point), self._s(stream))
 
     def promote(self, coarse, fine, prec=F64, nrhs=1, stream=None) -> None:
         call("mgc_aggregation_promote", self.handle, _p(coarse), _p(fine), int(prec), int(nrhs),
@@ -488,6 +497,13 @@ def coarsen_operator(cop: CoarseOp, fop: FineOp, agg: "Aggregation", mode=COARSE
                      stream=None):
     """CoarsenOperator: build cop's links from the fine operator and the basis."""
     call("mgc_coarsen_operator", cop.handle, fop.handle, agg.handle, int(mode),
+         stream.handle if stream else None)
+
+
+def coarsen_coarse_operator(cop2: CoarseOp, cop1: CoarseOp, agg: "Aggregation",
+                            mode=COARSEN_DIRSAVE, stream=None):
+    """CoarsenOperator of a coarse operator onto the next level."""
+    call("mgc_coarsen_coarse_operator", cop2.handle, cop1.handle, agg.handle, int(mode),
          stream.handle if stream else None)
 
 

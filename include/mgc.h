@@ -195,6 +195,12 @@ typedef struct mgc_aggregation mgc_aggregation;
  * must divide the fine lattice (subdivides, :71). */
 int mgc_aggregation_create(mgc_aggregation** agg, const int fine_dims[4], const int block[4],
                            int nb);
+/* Same for a "fine" level whose sites carry site_elems (even) components with
+ * chirality = element / (site_elems/2): site_elems = 12 for a Dirac field,
+ * = N for a coarse field being coarsened again (coarse-of-coarse aggregation,
+ * src/Benchmark_create_coarse_coarse_operator.cc:240-244). */
+int mgc_aggregation_create_ex(mgc_aggregation** agg, const int fine_dims[4], const int block[4],
+                              int nb, int site_elems);
 int mgc_aggregation_destroy(mgc_aggregation* agg);
 int mgc_aggregation_info(const mgc_aggregation* agg, int coarse_dims[4], int* nb,
                          int* fine_volume, int* coarse_volume);
@@ -215,6 +221,11 @@ int mgc_aggregation_check_orthogonal(mgc_aggregation* agg, double* errs_host, vo
  * prec = MGC_F64 or MGC_F32 vectors (FP32 uses an FP32 copy of the basis). */
 int mgc_aggregation_project(mgc_aggregation* agg, const void* fine, void* coarse, int prec,
                             int nrhs, void* stream);
+/* ProjectToSubspace with a lookup table of masked sites
+ * (include/AggregationRework.h:117-124, include/CoarsenedMatrixRework.h:437-453):
+ * mask_kind 0 = all sites, 1 = inner sites of `point`, 2 = outer sites. */
+int mgc_aggregation_project_masked(mgc_aggregation* agg, const void* fine, void* coarse,
+                                   int prec, int nrhs, int mask_kind, int point, void* stream);
 /* PromoteFromSubspace: fine = P coarse
  * (include/AggregationRework.h:126-131, include/AggregationKernelsRework.h:207-264). */
 int mgc_aggregation_promote(mgc_aggregation* agg, const void* coarse, void* fine, int prec,
@@ -268,6 +279,11 @@ enum { MGC_COARSEN_FULL = 0, MGC_COARSEN_DIRSAVE = 1 };
  * 999-1032).  cop must have the aggregation's coarse lattice and N = 2*nb. */
 int mgc_coarsen_operator(mgc_coarse_op* cop, mgc_fine_op* fop, mgc_aggregation* agg, int mode,
                          void* stream);
+/* CoarsenOperator of a coarse operator (the next level of the hierarchy,
+ * src/Benchmark_create_coarse_coarse_operator.cc:250-381): cop1 (F64) on the
+ * fine side, agg created with site_elems = cop1's nbasis, result in cop2. */
+int mgc_coarsen_coarse_operator(mgc_coarse_op* cop2, mgc_coarse_op* cop1, mgc_aggregation* agg,
+                                int mode, void* stream);
 /* ConstructRemainingLinks (include/CoarsenedMatrixUpstreamImprovedDirsave.h:999-1032,
  * disp_have = -1) and shiftLinks (include/CoarsenedMatrixKernelsRework.h:141-167,
  * disp_have = +1): A_want(x) = sigma o adj(A_have(x + disp_want mu)), sigma = -1

@@ -11,11 +11,15 @@
  *                                                    on the chirally doubled basis
  *   include/Benchmark_helpers.h:156-181, 203-213     chiral doubling and its undo
  *   include/AggregationRework.h:91-108               CheckOrthogonal
+ *   include/CoarsenedMatrixBaseline.h:609-675        CoarsenOperator (mask form), here on a
+ *                                                    coarse operator (coarse-of-coarse level)
  *
- * Fine fields are [site][spin 4][colour 3]; chirality k = spin / 2
- * (gamma5 = diag(1,1,-1,-1) in the chiral basis).  Split basis: nb fine
- * fields; coarse vectors [coarse site][2*nb] with n = k*nb + i.  Doubled
- * basis: 2*nb fine fields of definite chirality; coarse vector index n.
+ * Fine fields are [site][Ef]: Ef = 12 for a Dirac field ([spin 4][colour 3]),
+ * Ef = N for a coarse field.  The chirality of element e is e / (Ef/2)
+ * (spin / 2 for gamma5 = diag(1,1,-1,-1); the first / second half of a
+ * coarse vector).  Split basis: nb fine fields; coarse vectors
+ * [coarse site][2*nb] with n = k*nb + i.  Doubled basis: 2*nb fine fields of
+ * definite chirality; coarse vector index n.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -81,10 +85,10 @@ static int coarse_of(const int fd[4], const int blk[4], int sf) {
   return lex(cd_, c);
 }
 
-/* c(sc)(k, i) = sum_{sf in lut(sc)} sum_{s: s/2 = k} <basis_i(sf)(s), f(sf)(s)> */
-void orc_project(const int fd[4], const int blk[4], int nb, const double* basis, const double* fine,
-                 double* coarse, int kind, int point) {
-  const int Vf = vol(fd);
+/* c(sc)(k, i) = sum_{sf in lut(sc)} sum_{e: chirality k} <basis_i(sf)(e), f(sf)(e)> */
+void orc_project_ef(const int fd[4], const int blk[4], int nb, int Ef, const double* basis,
+                    const double* fine, double* coarse, int kind, int point) {
+  const int Vf = vol(fd), half = Ef / 2;
   int cd_[4];
   for (int d = 0; d < 4; ++d) cd_[d] = fd[d] / blk[d];
   const int Vc = vol(cd_);
@@ -96,12 +100,11 @@ void orc_project(const int fd[4], const int blk[4], int nb, const double* basis,
     if (!mask_value(fd, blk, sf, kind, point)) continue;
     const int sc = coarse_of(fd, blk, sf);
     for (int i = 0; i < nb; ++i) {
-      const cd* u = B + ((size_t)i * Vf + sf) * 12;
-      for (int s = 0; s < 4; ++s) {
-        const int k = s / 2;
+      const cd* u = B + ((size_t)i * Vf + sf) * Ef;
+      for (int k = 0; k < 2; ++k) {
         cd acc = {0, 0};
-        for (int col = 0; col < 3; ++col) {
-          cd a = u[s * 3 + col], b = f[(size_t)sf * 12 + s * 3 + col];
+        for (int e = k * half; e < (k + 1) * half; ++e) {
+          cd a = u[e], b = f[(size_t)sf * Ef + e];
           acc.re += a.re * b.re + a.im * b.im;
           acc.im += a.re * b.im - a.im * b.re;
         }
@@ -113,32 +116,43 @@ void orc_project(const int fd[4], const int blk[4], int nb, const double* basis,
   }
 }
 
-/* f(sf)(s) = sum_i c(sc)(s/2, i) basis_i(sf)(s) */
-void orc_promote(const int fd[4], const int blk[4], int nb, const double* basis,
-                 const double* coarse, double* fine) {
-  const int Vf = vol(fd);
+void orc_project(const int fd[4], const int blk[4], int nb, const double* basis, const double* fine,
+                 double* coarse, int kind, int point) {
+  orc_project_ef(fd, blk, nb, 12, basis, fine, coarse, kind, point);
+}
+
+/* f(sf)(e) = sum_i c(sc)(chirality(e), i) basis_i(sf)(e) */
+void orc_promote_ef(const int fd[4], const int blk[4], int nb, int Ef, const double* basis,
+                    const double* coarse, double* fine) {
+  const int Vf = vol(fd), half = Ef / 2;
   const cd* B = (const cd*)basis;
   const cd* c = (const cd*)coarse;
   cd* f = (cd*)fine;
   for (int sf = 0; sf < Vf; ++sf) {
     const int sc = coarse_of(fd, blk, sf);
-    for (int sc12 = 0; sc12 < 12; ++sc12) {
-      const int k = sc12 / 6;
+    for (int e = 0; e < Ef; ++e) {
+      const int k = e / half;
       cd acc = {0, 0};
       for (int i = 0; i < nb; ++i) {
-        cd a = c[(size_t)sc * 2 * nb + k * nb + i], b = B[((size_t)i * Vf + sf) * 12 + sc12];
+        cd a = c[(size_t)sc * 2 * nb + k * nb + i], b = B[((size_t)i * Vf + sf) * Ef + e];
         acc.re += a.re * b.re - a.im * b.im;
         acc.im += a.re * b.im + a.im * b.re;
       }
-      f[(size_t)sf * 12 + sc12] = acc;
+      f[(size_t)sf * Ef + e] = acc;
     }
   }
 }
 
+void orc_promote(const int fd[4], const int blk[4], int nb, const double* basis,
+                 const double* coarse, double* fine) {
+  orc_promote_ef(fd, blk, nb, 12, basis, coarse, fine);
+}
+
 /* per block and chirality: modified Gram-Schmidt, v -= <u,v> u for u < v with
  * the updated v, then v *= <v,v>^{-1/2}; `passes` sweeps */
-void orc_orthogonalise(const int fd[4], const int blk[4], int nb, double* basis, int passes) {
-  const int Vf = vol(fd);
+void orc_orthogonalise_ef(const int fd[4], const int blk[4], int nb, int Ef, double* basis,
+                          int passes) {
+  const int Vf = vol(fd), half = Ef / 2;
   int cd_[4], bv = 1;
   for (int d = 0; d < 4; ++d) { cd_[d] = fd[d] / blk[d]; bv *= blk[d]; }
   const int Vc = vol(cd_);
@@ -155,19 +169,19 @@ void orc_orthogonalise(const int fd[4], const int blk[4], int nb, double* basis,
         for (int u = 0; u < v; ++u) {
           cd alpha[2] = {{0, 0}, {0, 0}};
           for (int q = 0; q < sizes[sc]; ++q) {
-            const cd* pu = B + ((size_t)u * Vf + L[q]) * 12;
-            const cd* pv = B + ((size_t)v * Vf + L[q]) * 12;
-            for (int e = 0; e < 12; ++e) {
-              const int k = e / 6;
+            const cd* pu = B + ((size_t)u * Vf + L[q]) * Ef;
+            const cd* pv = B + ((size_t)v * Vf + L[q]) * Ef;
+            for (int e = 0; e < Ef; ++e) {
+              const int k = e / half;
               alpha[k].re += pu[e].re * pv[e].re + pu[e].im * pv[e].im;
               alpha[k].im += pu[e].re * pv[e].im - pu[e].im * pv[e].re;
             }
           }
           for (int q = 0; q < sizes[sc]; ++q) {
-            const cd* pu = B + ((size_t)u * Vf + L[q]) * 12;
-            cd* pv = B + ((size_t)v * Vf + L[q]) * 12;
-            for (int e = 0; e < 12; ++e) {
-              const cd a = alpha[e / 6];
+            const cd* pu = B + ((size_t)u * Vf + L[q]) * Ef;
+            cd* pv = B + ((size_t)v * Vf + L[q]) * Ef;
+            for (int e = 0; e < Ef; ++e) {
+              const cd a = alpha[e / half];
               pv[e].re -= a.re * pu[e].re - a.im * pu[e].im;
               pv[e].im -= a.re * pu[e].im + a.im * pu[e].re;
             }
@@ -175,16 +189,16 @@ void orc_orthogonalise(const int fd[4], const int blk[4], int nb, double* basis,
         }
         double nrm[2] = {0, 0};
         for (int q = 0; q < sizes[sc]; ++q) {
-          const cd* pv = B + ((size_t)v * Vf + L[q]) * 12;
-          for (int e = 0; e < 12; ++e) nrm[e / 6] += pv[e].re * pv[e].re + pv[e].im * pv[e].im;
+          const cd* pv = B + ((size_t)v * Vf + L[q]) * Ef;
+          for (int e = 0; e < Ef; ++e) nrm[e / half] += pv[e].re * pv[e].re + pv[e].im * pv[e].im;
         }
         nrm[0] = pow(nrm[0], -0.5);
         nrm[1] = pow(nrm[1], -0.5);
         for (int q = 0; q < sizes[sc]; ++q) {
-          cd* pv = B + ((size_t)v * Vf + L[q]) * 12;
-          for (int e = 0; e < 12; ++e) {
-            pv[e].re *= nrm[e / 6];
-            pv[e].im *= nrm[e / 6];
+          cd* pv = B + ((size_t)v * Vf + L[q]) * Ef;
+          for (int e = 0; e < Ef; ++e) {
+            pv[e].re *= nrm[e / half];
+            pv[e].im *= nrm[e / half];
           }
         }
       }
@@ -193,6 +207,10 @@ void orc_orthogonalise(const int fd[4], const int blk[4], int nb, double* basis,
   free(lut);
   free(sizes);
   free(rev);
+}
+
+void orc_orthogonalise(const int fd[4], const int blk[4], int nb, double* basis, int passes) {
+  orc_orthogonalise_ef(fd, blk, nb, 12, basis, passes);
 }
 
 /* ---------- doubled (Upstream) forms: 2*nb full vectors ---------- */
@@ -283,4 +301,50 @@ void orc_orthogonalise_doubled(const int fd[4], const int blk[4], int N, double*
           }
         }
       }
+}
+
+/* CoarsenOperator of a coarse operator (links1 FLAT [9][V1][N1][N1]) on the
+ * split basis of the next level (nb2 vectors of N1 elements per site):
+ * for each doubled vector i and point p, Mphi = A_p v_i, outer projection to
+ * A2_p(:, i), inner projection added to A2_8(:, i).  links2: [9][V2][N2][N2]. */
+void orc_coarsen_coarse_operator(const int d1[4], const int blk[4], int nb2, int N1,
+                                 const double* links1, const double* basis, double* links2) {
+  int d2[4];
+  for (int k = 0; k < 4; ++k) d2[k] = d1[k] / blk[k];
+  const int V1 = vol(d1), V2 = vol(d2), N2 = 2 * nb2, half = N1 / 2;
+  cd* A = (cd*)links2;
+  memset(A, 0, sizeof(cd) * 9 * (size_t)V2 * N2 * N2);
+  double* phi = (double*)malloc(sizeof(double) * 2 * (size_t)V1 * N1);
+  double* Mphi = (double*)malloc(sizeof(double) * 2 * (size_t)V1 * N1);
+  double* proj = (double*)malloc(sizeof(double) * 2 * (size_t)V2 * N2);
+  for (int i = 0; i < N2; ++i) {
+    const int l = i / nb2, c = i % nb2;
+    const double* u = basis + 2 * (size_t)c * V1 * N1;
+    for (size_t x = 0; x < (size_t)V1; ++x)
+      for (int e = 0; e < N1; ++e) {
+        const int keep = (e / half) == l;
+        phi[2 * (x * N1 + e)] = keep ? u[2 * (x * N1 + e)] : 0.0;
+        phi[2 * (x * N1 + e) + 1] = keep ? u[2 * (x * N1 + e) + 1] : 0.0;
+      }
+    for (int p = 0; p < 9; ++p) {
+      orc_coarse_apply(d1, N1, links1, phi, Mphi, 1u << p, 0);
+      if (p < 8) {
+        orc_project_ef(d1, blk, nb2, N1, basis, Mphi, proj, 2, p);
+        const cd* pr = (const cd*)proj;
+        for (int xc = 0; xc < V2; ++xc)
+          for (int j = 0; j < N2; ++j) A[(((size_t)p * V2 + xc) * N2 + j) * N2 + i] = pr[(size_t)xc * N2 + j];
+      }
+      orc_project_ef(d1, blk, nb2, N1, basis, Mphi, proj, 1, p);
+      const cd* pr = (const cd*)proj;
+      for (int xc = 0; xc < V2; ++xc)
+        for (int j = 0; j < N2; ++j) {
+          cd* a = &A[(((size_t)8 * V2 + xc) * N2 + j) * N2 + i];
+          a->re += pr[(size_t)xc * N2 + j].re;
+          a->im += pr[(size_t)xc * N2 + j].im;
+        }
+    }
+  }
+  free(phi);
+  free(Mphi);
+  free(proj);
 }

@@ -66,6 +66,15 @@ void orc_promote(const int fine_dims[4], const int block[4], int nb, const doubl
                  const double* coarse, double* fine);
 void orc_orthogonalise(const int fine_dims[4], const int block[4], int nb, double* basis,
                        int passes);
+/* the same on fields of Ef elements per site (chirality = element / (Ef/2)) */
+void orc_project_ef(const int fine_dims[4], const int block[4], int nb, int Ef,
+                    const double* basis, const double* fine, double* coarse, int kind, int point);
+void orc_promote_ef(const int fine_dims[4], const int block[4], int nb, int Ef,
+                    const double* basis, const double* coarse, double* fine);
+void orc_orthogonalise_ef(const int fine_dims[4], const int block[4], int nb, int Ef,
+                          double* basis, int passes);
+void orc_coarsen_coarse_operator(const int dims1[4], const int block[4], int nb2, int N1,
+                                 const double* links1, const double* basis, double* links2);
 void orc_chiral_double(int Vf, int nb, const double* in, double* out);
 void orc_project_doubled(const int fine_dims[4], const int block[4], int N, const double* basis,
                          const double* fine, double* coarse);

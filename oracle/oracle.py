@@ -224,6 +224,47 @@ def orthogonalise(fine_dims, block, nb, basis, passes=2):
     return out
 
 
+def project_ef(fine_dims, block, nb, Ef, basis, fine, kind=MASK_ALL, point=8):
+    """aggregateProject on fields of Ef elements per site."""
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    fine = np.ascontiguousarray(fine, dtype=np.complex128)
+    Vc = volume(coarse_dims(fine_dims, block))
+    out = np.zeros((Vc, 2 * nb), dtype=np.complex128)
+    lib().orc_project_ef(_dims(fine_dims), _dims(block), nb, Ef, _d(basis.view(np.float64)),
+                         _d(fine.view(np.float64)), _d(out.view(np.float64)), kind, point)
+    return out
+
+
+def promote_ef(fine_dims, block, nb, Ef, basis, coarse):
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    coarse = np.ascontiguousarray(coarse, dtype=np.complex128)
+    out = np.zeros((volume(fine_dims), Ef), dtype=np.complex128)
+    lib().orc_promote_ef(_dims(fine_dims), _dims(block), nb, Ef, _d(basis.view(np.float64)),
+                         _d(coarse.view(np.float64)), _d(out.view(np.float64)))
+    return out
+
+
+def orthogonalise_ef(fine_dims, block, nb, Ef, basis, passes=2):
+    out = np.array(basis, dtype=np.complex128, copy=True, order="C")
+    lib().orc_orthogonalise_ef(_dims(fine_dims), _dims(block), nb, Ef, _d(out.view(np.float64)),
+                               passes)
+    return out
+
+
+def coarsen_coarse_operator(dims1, block, nb2, N1, links1, basis):
+    """CoarsenOperator (mask form, include/CoarsenedMatrixBaseline.h:609-675) of a
+    coarse operator onto the next level.  Returns FLAT links [9][V2][N2][N2]."""
+    links1 = np.ascontiguousarray(links1, dtype=np.complex128)
+    basis = np.ascontiguousarray(basis, dtype=np.complex128)
+    V2 = volume(coarse_dims(dims1, block))
+    N2 = 2 * nb2
+    out = np.zeros((9, V2, N2, N2), dtype=np.complex128)
+    lib().orc_coarsen_coarse_operator(_dims(dims1), _dims(block), nb2, N1,
+                                      _d(links1.view(np.float64)), _d(basis.view(np.float64)),
+                                      _d(out.view(np.float64)))
+    return out
+
+
 def chiral_double(Vf, nb, basis):
     """performChiralDoublingG5C, include/Benchmark_helpers.h:156-181."""
     basis = np.ascontiguousarray(basis, dtype=np.complex128)

@@ -124,6 +124,40 @@ def test_construct_remaining_links_matches_oracle(gpu):
         assert np.abs(got - ref).max() < 1e-15
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("d1,blk,N1,nb2", [((4, 4, 4, 4), (2, 2, 2, 2), 8, 3),
+                                           ((6, 4, 2, 2), (2, 2, 2, 2), 16, 4),
+                                           ((12, 12, 12, 12), (4, 4, 4, 4), 32, 16)])
+def test_coarse_of_coarse_matches_oracle(gpu, d1, blk, N1, nb2, mode):
+    """Second-level CoarsenOperator (the 12^4 -> 3^4 level of the solve included)."""
+    print(f"[case] coarse-of-coarse d1={d1} N1={N1} nb2={nb2} mode={mode}", flush=True)
+    from helpers import g5_links
+    V1 = int(np.prod(d1))
+    L1 = g5_links(d1, N1, 81)  # gamma5-hermitian, so the rebuilt forward links agree
+    B = gauss(nb2 * V1 * N1, 82).reshape(nb2, V1, N1)
+    cop1 = gpu.CoarseOp(d1, N1, gpu.F64)
+    cop1.set_links(gpu.DeviceBuffer.from_numpy(L1), gpu.LAYOUT_FLAT, invert=False)
+    agg = gpu.Aggregation(d1, blk, nb2, site_elems=N1)
+    agg.set_basis(gpu.DeviceBuffer.from_numpy(B))
+    agg.orthogonalise(2)
+    db = gpu.DeviceBuffer(B.nbytes)
+    agg.get_basis(db)
+    gpu.synchronize()
+    Bo = db.to_numpy(np.complex128, B.shape)
+    assert rel(oracle.orthogonalise_ef(d1, blk, nb2, N1, B, 2), Bo) < 1e-12
+    d2 = oracle.coarse_dims(d1, blk)
+    V2, N2 = int(np.prod(d2)), 2 * nb2
+    cop2 = gpu.CoarseOp(d2, N2, gpu.F64)
+    gpu.coarsen_coarse_operator(cop2, cop1, agg, mode)
+    d = gpu.DeviceBuffer(9 * V2 * N2 * N2 * 16)
+    cop2.get_links(d, gpu.LAYOUT_FLAT)
+    gpu.synchronize()
+    got = d.to_numpy(np.complex128, (9, V2, N2, N2))
+    ref = oracle.coarsen_coarse_operator(d1, blk, nb2, N1, L1, Bo)
+    for p in range(9):
+        assert rel(ref[p], got[p]) < 1e-12, f"point {p}"
+
+
 def test_full_size_galerkin_identity_cfg4(gpu):
     """Fine 32^4, 4^4 blocks, nb = 32 (coarse 8^4, N = 64): D_c v = R D P v on
     random coarse vectors, every operator the library's own, gauge field and

@@ -134,3 +134,22 @@ def test_coarsen_operator_is_galerkin_product(fd, blk, nb):
     # gamma5-hermitian coarse operator: the forward links follow from the backward ones
     rebuilt = oracle.construct_remaining_links(cdims, N, links, -1).reshape(9, Vc, N, N)
     assert rel(links[0:4], rebuilt[0:4]) < 1e-13
+
+
+@pytest.mark.parametrize("d1,blk,N1,nb2", [((4, 4, 4, 4), (2, 2, 2, 2), 8, 3),
+                                           ((6, 2, 2, 2), (2, 2, 2, 2), 6, 2)])
+def test_coarse_of_coarse_is_galerkin_product(d1, blk, N1, nb2):
+    """Second-level CoarsenOperator: D_2 = R D_1 P on random vectors (odd extent included)."""
+    from helpers import rand_links
+    V1 = int(np.prod(d1))
+    L1 = rand_links(d1, N1, 71)
+    B = oracle.orthogonalise_ef(d1, blk, nb2, N1, gauss(nb2 * V1 * N1, 72).reshape(nb2, V1, N1), 2)
+    L2 = oracle.coarsen_coarse_operator(d1, blk, nb2, N1, L1, B)
+    d2 = oracle.coarse_dims(d1, blk)
+    V2, N2 = int(np.prod(d2)), 2 * nb2
+    v2 = gauss(V2 * N2, 73).reshape(V2, N2)
+    lhs = oracle.coarse_apply(d2, N2, L2, v2)
+    rhs = oracle.project_ef(d1, blk, nb2, N1, B,
+                            oracle.coarse_apply(d1, N1, L1,
+                                                oracle.promote_ef(d1, blk, nb2, N1, B, v2)))
+    assert rel(rhs, lhs) < 1e-13

@@ -1,0 +1,248 @@
+// Complex BLAS-1 kernels with FP64 reductions (grid-stride elementwise ops,
+// two-stage block reductions: NPART partial sums per result, then one block).
+#include "blas.hpp"
+
+namespace mgc {
+
+Blas::Blas(hipStream_t st) : s(st) {
+  MGC_HIP(hipMalloc(&d_part, (size_t)KMAX * NPART * 2 * sizeof(double)));
+  MGC_HIP(hipMalloc(&d_res, (size_t)KMAX * 2 * sizeof(double)));
+  MGC_HIP(hipMalloc(&d_coef, (size_t)KMAX * sizeof(cplx<double>)));
+  MGC_HIP(hipHostMalloc(&h_res, (size_t)KMAX * 2 * sizeof(double), hipHostMallocDefault));
+}
+Blas::~Blas() {
+  if (d_part) (void)hipFree(d_part);
+  if (d_res) (void)hipFree(d_res);
+  if (d_coef) (void)hipFree(d_coef);
+  if (h_res) (void)hipHostFree(h_res);
+}
+
+static unsigned ew_grid(size_t n) { return grid_for(n, 256, 8192); }
+
+template <typename R>
+__global__ void k_vzero(cplx<R>* x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = {R(0), R(0)};
+}
+template <typename R>
+__global__ void k_vcopy(const cplx<R>* x, cplx<R>* y, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = x[i];
+}
+template <typename Rs, typename Rd>
+__global__ void k_vconvert(const cplx<Rs>* x, cplx<Rd>* y, size_t n, double a) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = {(Rd)((double)x[i].re * a), (Rd)((double)x[i].im * a)};
+}
+template <typename R>
+__global__ void k_vaxpy(double ar, double ai, const cplx<R>* x, cplx<R>* y, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const cplx<R> v = x[i];
+    y[i].re += (R)(ar * v.re - ai * v.im);
+    y[i].im += (R)(ar * v.im + ai * v.re);
+  }
+}
+template <typename R>
+__global__ void k_vxpay(const cplx<R>* x, double ar, double ai, cplx<R>* y, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const cplx<R> v = y[i];
+    y[i] = {(R)(x[i].re + ar * v.re - ai * v.im), (R)(x[i].im + ar * v.im + ai * v.re)};
+  }
+}
+template <typename R>
+__global__ void k_vscale(double a, cplx<R>* x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    x[i] = {(R)(a * x[i].re), (R)(a * x[i].im)};
+}
+template <typename R>
+__global__ void k_vsub(const cplx<R>* x, const cplx<R>* y, cplx<R>* z, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    z[i] = {x[i].re - y[i].re, x[i].im - y[i].im};
+}
+
+__device__ inline double2 block_sum2(double a, double b) {
+  __shared__ double sh[2 * 16];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off);
+    b += __shfl_xor(b, off);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sh[2 * w] = a;
+    sh[2 * w + 1] = b;
+  }
+  __syncthreads();
+  double ra = 0, rb = 0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      ra += sh[2 * k];
+      rb += sh[2 * k + 1];
+    }
+  return make_double2(ra, rb);
+}
+
+// partial[j][blockIdx.x] = sum over the block's share of conj(V_j) w (or |x|^2)
+template <typename R, bool NORM>
+__global__ __launch_bounds__(256) void k_dot_part(const cplx<R>* V, size_t ldv,
+                                                  const cplx<R>* w, size_t n, double* part) {
+  const int j = blockIdx.y;
+  const cplx<R>* v = V + (size_t)j * ldv;
+  double re = 0.0, im = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double ar = v[i].re, ai = v[i].im;
+    if (NORM) {
+      re += ar * ar + ai * ai;
+    } else {
+      const double br = w[i].re, bi = w[i].im;
+      re += ar * br + ai * bi;
+      im += ar * bi - ai * br;
+    }
+  }
+  const double2 t = block_sum2(re, im);
+  if (threadIdx.x == 0) {
+    part[((size_t)j * gridDim.x + blockIdx.x) * 2] = t.x;
+    part[((size_t)j * gridDim.x + blockIdx.x) * 2 + 1] = t.y;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_dot_final(const double* part, int nparts, double* res) {
+  const int j = blockIdx.x;
+  double re = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    re += part[((size_t)j * nparts + i) * 2];
+    im += part[((size_t)j * nparts + i) * 2 + 1];
+  }
+  const double2 t = block_sum2(re, im);
+  if (threadIdx.x == 0) {
+    res[2 * j] = t.x;
+    res[2 * j + 1] = t.y;
+  }
+}
+
+template <typename R>
+__global__ void k_vmultiaxpy(const cplx<double>* c, const cplx<R>* V, size_t ldv, int k,
+                             cplx<R>* w, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double re = w[i].re, im = w[i].im;
+    for (int j = 0; j < k; ++j) {
+      const cplx<R> v = V[(size_t)j * ldv + i];
+      re += c[j].re * v.re - c[j].im * v.im;
+      im += c[j].re * v.im + c[j].im * v.re;
+    }
+    w[i] = {(R)re, (R)im};
+  }
+}
+
+template <typename R> void vzero(cplx<R>* x, size_t n, hipStream_t s) {
+  k_vzero<R><<<ew_grid(n), 256, 0, s>>>(x, n);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vcopy(const cplx<R>* x, cplx<R>* y, size_t n, hipStream_t s) {
+  k_vcopy<R><<<ew_grid(n), 256, 0, s>>>(x, y, n);
+  MGC_LAUNCH_CHECK();
+}
+template <typename Rs, typename Rd>
+void vconvert(const cplx<Rs>* x, cplx<Rd>* y, size_t n, double scale, hipStream_t s) {
+  k_vconvert<Rs, Rd><<<ew_grid(n), 256, 0, s>>>(x, y, n, scale);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vaxpy(cplx<double> a, const cplx<R>* x, cplx<R>* y, size_t n,
+                                 hipStream_t s) {
+  k_vaxpy<R><<<ew_grid(n), 256, 0, s>>>(a.re, a.im, x, y, n);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vxpay(const cplx<R>* x, cplx<double> a, cplx<R>* y, size_t n,
+                                 hipStream_t s) {
+  k_vxpay<R><<<ew_grid(n), 256, 0, s>>>(x, a.re, a.im, y, n);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vscale(double a, cplx<R>* x, size_t n, hipStream_t s) {
+  k_vscale<R><<<ew_grid(n), 256, 0, s>>>(a, x, n);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vsub(const cplx<R>* x, const cplx<R>* y, cplx<R>* z, size_t n,
+                                hipStream_t s) {
+  k_vsub<R><<<ew_grid(n), 256, 0, s>>>(x, y, z, n);
+  MGC_LAUNCH_CHECK();
+}
+
+static int nparts_for(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > (size_t)Blas::NPART) g = Blas::NPART;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+template <typename R, bool NORM>
+static void reduce(Blas& b, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w, size_t n) {
+  MGC_CHECK(k >= 1 && k <= Blas::KMAX, MGC_ERR_ARG, "too many simultaneous reductions");
+  const int np = nparts_for(n);
+  k_dot_part<R, NORM><<<dim3(np, k), 256, 0, b.s>>>(V, ldv, w, n, b.d_part);
+  MGC_LAUNCH_CHECK();
+  k_dot_final<<<k, 256, 0, b.s>>>(b.d_part, np, b.d_res);
+  MGC_LAUNCH_CHECK();
+  MGC_HIP(hipMemcpyAsync(b.h_res, b.d_res, (size_t)k * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                         b.s));
+  MGC_HIP(hipStreamSynchronize(b.s));
+}
+
+template <typename R> cplx<double> vdot(Blas& b, const cplx<R>* x, const cplx<R>* y, size_t n) {
+  reduce<R, false>(b, x, 0, 1, y, n);
+  return {b.h_res[0], b.h_res[1]};
+}
+template <typename R> double vnorm2(Blas& b, const cplx<R>* x, size_t n) {
+  reduce<R, true>(b, x, 0, 1, x, n);
+  return b.h_res[0];
+}
+template <typename R> void vmultidot(Blas& b, const cplx<R>* V, size_t ldv, int k,
+                                     const cplx<R>* w, size_t n, cplx<double>* h) {
+  reduce<R, false>(b, V, ldv, k, w, n);
+  for (int j = 0; j < k; ++j) h[j] = {b.h_res[2 * j], b.h_res[2 * j + 1]};
+}
+template <typename R> void vmultiaxpy(Blas& b, const cplx<double>* c, const cplx<R>* V, size_t ldv,
+                                      int k, cplx<R>* w, size_t n) {
+  MGC_CHECK(k >= 0 && k <= Blas::KMAX, MGC_ERR_ARG, "too many vectors");
+  if (k == 0) return;
+  MGC_HIP(hipMemcpyAsync(b.d_coef, c, (size_t)k * sizeof(cplx<double>), hipMemcpyHostToDevice,
+                         b.s));
+  // the host array may be reused right after: wait for the copy
+  MGC_HIP(hipStreamSynchronize(b.s));
+  k_vmultiaxpy<R><<<ew_grid(n), 256, 0, b.s>>>(b.d_coef, V, ldv, k, w, n);
+  MGC_LAUNCH_CHECK();
+}
+
+#define INST(R)                                                                              \
+  template void vzero<R>(cplx<R>*, size_t, hipStream_t);                                     \
+  template void vcopy<R>(const cplx<R>*, cplx<R>*, size_t, hipStream_t);                     \
+  template void vaxpy<R>(cplx<double>, const cplx<R>*, cplx<R>*, size_t, hipStream_t);       \
+  template void vxpay<R>(const cplx<R>*, cplx<double>, cplx<R>*, size_t, hipStream_t);       \
+  template void vscale<R>(double, cplx<R>*, size_t, hipStream_t);                            \
+  template void vsub<R>(const cplx<R>*, const cplx<R>*, cplx<R>*, size_t, hipStream_t);      \
+  template cplx<double> vdot<R>(Blas&, const cplx<R>*, const cplx<R>*, size_t);              \
+  template double vnorm2<R>(Blas&, const cplx<R>*, size_t);                                  \
+  template void vmultidot<R>(Blas&, const cplx<R>*, size_t, int, const cplx<R>*, size_t,     \
+                             cplx<double>*);                                                 \
+  template void vmultiaxpy<R>(Blas&, const cplx<double>*, const cplx<R>*, size_t, int,       \
+                              cplx<R>*, size_t);
+INST(double)
+INST(float)
+#undef INST
+template void vconvert<double, float>(const cplx<double>*, cplx<float>*, size_t, double,
+                                      hipStream_t);
+template void vconvert<float, double>(const cplx<float>*, cplx<double>*, size_t, double,
+                                      hipStream_t);
+template void vconvert<double, double>(const cplx<double>*, cplx<double>*, size_t, double,
+                                       hipStream_t);
+template void vconvert<float, float>(const cplx<float>*, cplx<float>*, size_t, double,
+                                     hipStream_t);
+
+}  // namespace mgc

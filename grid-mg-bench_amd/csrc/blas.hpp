@@ -1,0 +1,48 @@
+// Complex BLAS-1 on device vectors with FP64 reductions, for the Krylov
+// solvers of the multigrid hierarchy.  All calls are stream-ordered; the
+// reductions return to the host (they synchronise the stream).
+#pragma once
+#include "common.hpp"
+
+namespace mgc {
+
+struct Blas {
+  hipStream_t s = nullptr;
+  double* d_part = nullptr;  // partial sums [kmax][NPART][2]
+  double* d_res = nullptr;   // final results [kmax][2]
+  double* h_res = nullptr;   // pinned host copy
+  cplx<double>* d_coef = nullptr;  // coefficients for multi-axpy [kmax]
+  static constexpr int NPART = 512;
+  static constexpr int KMAX = 64;
+  explicit Blas(hipStream_t st);
+  ~Blas();
+  Blas(const Blas&) = delete;
+  Blas& operator=(const Blas&) = delete;
+};
+
+template <typename R> void vzero(cplx<R>* x, size_t n, hipStream_t s);
+template <typename R> void vcopy(const cplx<R>* x, cplx<R>* y, size_t n, hipStream_t s);
+template <typename Rs, typename Rd>
+void vconvert(const cplx<Rs>* x, cplx<Rd>* y, size_t n, double scale, hipStream_t s);
+// y += a x
+template <typename R> void vaxpy(cplx<double> a, const cplx<R>* x, cplx<R>* y, size_t n,
+                                 hipStream_t s);
+// y = x + a y
+template <typename R> void vxpay(const cplx<R>* x, cplx<double> a, cplx<R>* y, size_t n,
+                                 hipStream_t s);
+// x *= a (real)
+template <typename R> void vscale(double a, cplx<R>* x, size_t n, hipStream_t s);
+// z = x - y
+template <typename R> void vsub(const cplx<R>* x, const cplx<R>* y, cplx<R>* z, size_t n,
+                                hipStream_t s);
+// <x, y> = sum conj(x) y
+template <typename R> cplx<double> vdot(Blas& b, const cplx<R>* x, const cplx<R>* y, size_t n);
+template <typename R> double vnorm2(Blas& b, const cplx<R>* x, size_t n);
+// h[j] = <V_j, w> for j < k, V_j = V + j*ldv
+template <typename R> void vmultidot(Blas& b, const cplx<R>* V, size_t ldv, int k,
+                                     const cplx<R>* w, size_t n, cplx<double>* h);
+// w += sum_j c[j] V_j  (c on the host)
+template <typename R> void vmultiaxpy(Blas& b, const cplx<double>* c, const cplx<R>* V, size_t ldv,
+                                      int k, cplx<R>* w, size_t n);
+
+}  // namespace mgc

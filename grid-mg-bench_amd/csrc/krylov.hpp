@@ -1,0 +1,176 @@
+// Krylov solvers for the multigrid hierarchy: restarted (F)GMRES with
+// classical Gram-Schmidt plus one re-orthogonalisation (CGS2: two fused
+// multi-dot reductions per step), complex Givens rotations, and the true
+// residual recomputed at every restart.  Templated on the vector precision.
+//
+// Reference use: the FGMRES outer solver and K-cycle wrapper, GMRES smoother
+// and coarsest-level solver of include/MultigridPreconditioner.h:253-309,
+// 622-641 (created by include/Helpers.h:38-56 with the defaults of
+// include/MultigridParams.h:95-133).
+#pragma once
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "blas.hpp"
+
+namespace mgc {
+
+template <typename R>
+struct LinOp {
+  size_t n = 0;  // complex elements of a vector
+  virtual ~LinOp() {}
+  virtual void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) = 0;
+};
+
+// out = P in (the initial content of out is ignored)
+template <typename R>
+struct Prec {
+  virtual ~Prec() {}
+  virtual void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) = 0;
+};
+
+struct KrylovStats {
+  int iters = 0;
+  int applies = 0;
+  double rel_res = 0.0;
+  bool converged = false;
+};
+
+template <typename R>
+class Gmres {
+ public:
+  Gmres(size_t n, int m, bool flexible) : n_(n), m_(m), flexible_(flexible) {
+    MGC_CHECK(m >= 1 && m + 1 <= Blas::KMAX, MGC_ERR_ARG, "restart length out of range");
+    MGC_HIP(hipMalloc(&V_, (size_t)(m + 1) * n * sizeof(cplx<R>)));
+    if (flexible) MGC_HIP(hipMalloc(&Z_, (size_t)m * n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&w_, n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&r_, n * sizeof(cplx<R>)));
+  }
+  ~Gmres() {
+    void* b[] = {V_, Z_, w_, r_};
+    for (void* p : b)
+      if (p) (void)hipFree(p);
+  }
+  Gmres(const Gmres&) = delete;
+  Gmres& operator=(const Gmres&) = delete;
+
+  // Solve A x = b to ||b - A x|| <= tol ||b|| or maxit iterations.  x is the
+  // initial guess unless x_zero.
+  KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>* P, const cplx<R>* b, cplx<R>* x, bool x_zero,
+                    double tol, int maxit) {
+    typedef std::complex<double> C;
+    hipStream_t s = bl.s;
+    const size_t n = n_;
+    KrylovStats st;
+    const double bnorm = std::sqrt(vnorm2<R>(bl, b, n));
+    if (bnorm == 0.0) {
+      vzero<R>(x, n, s);
+      st.converged = true;
+      return st;
+    }
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, r_, n, s);
+    } else {
+      A.apply(x, w_, s);
+      ++st.applies;
+      vsub<R>(b, w_, r_, n, s);
+    }
+    double beta = std::sqrt(vnorm2<R>(bl, r_, n));
+    std::vector<C> H((size_t)(m_ + 1) * m_), g(m_ + 1), cs_s(m_);
+    std::vector<double> cs_c(m_);
+    std::vector<cplx<double>> h(m_ + 1), h2(m_ + 1), y(m_);
+    while (true) {
+      st.rel_res = beta / bnorm;
+      if (st.rel_res <= tol) {
+        st.converged = true;
+        break;
+      }
+      if (st.iters >= maxit) break;
+      vconvert<R, R>(r_, V_, n, 1.0 / beta, s);
+      std::fill(g.begin(), g.end(), C(0.0));
+      g[0] = beta;
+      int j = 0;
+      for (; j < m_;) {
+        cplx<R>* vj = V_ + (size_t)j * n;
+        const cplx<R>* z = vj;
+        if (flexible_) {
+          cplx<R>* zj = Z_ + (size_t)j * n;
+          if (P) P->apply(vj, zj, s);
+          else vcopy<R>(vj, zj, n, s);
+          z = zj;
+        }
+        A.apply(z, w_, s);
+        ++st.applies;
+        // CGS2
+        vmultidot<R>(bl, V_, n, j + 1, w_, n, h.data());
+        for (int i = 0; i <= j; ++i) y[i] = {-h[i].re, -h[i].im};
+        vmultiaxpy<R>(bl, y.data(), V_, n, j + 1, w_, n);
+        vmultidot<R>(bl, V_, n, j + 1, w_, n, h2.data());
+        for (int i = 0; i <= j; ++i) {
+          y[i] = {-h2[i].re, -h2[i].im};
+          h[i].re += h2[i].re;
+          h[i].im += h2[i].im;
+        }
+        vmultiaxpy<R>(bl, y.data(), V_, n, j + 1, w_, n);
+        const double hn = std::sqrt(vnorm2<R>(bl, w_, n));
+        for (int i = 0; i <= j; ++i) H[(size_t)i * m_ + j] = C(h[i].re, h[i].im);
+        H[(size_t)(j + 1) * m_ + j] = hn;
+        if (hn > 0.0) vconvert<R, R>(w_, V_ + (size_t)(j + 1) * n, n, 1.0 / hn, s);
+        // previous rotations on column j
+        for (int i = 0; i < j; ++i) {
+          const C a = H[(size_t)i * m_ + j], bb = H[(size_t)(i + 1) * m_ + j];
+          H[(size_t)i * m_ + j] = cs_c[i] * a + cs_s[i] * bb;
+          H[(size_t)(i + 1) * m_ + j] = -std::conj(cs_s[i]) * a + cs_c[i] * bb;
+        }
+        // new rotation zeroing H[j+1][j]
+        const C a = H[(size_t)j * m_ + j];
+        const double bb = std::abs(H[(size_t)(j + 1) * m_ + j]);
+        const double t = std::sqrt(std::norm(a) + bb * bb);
+        if (std::abs(a) == 0.0) {
+          cs_c[j] = 0.0;
+          cs_s[j] = 1.0;
+        } else {
+          cs_c[j] = std::abs(a) / t;
+          cs_s[j] = (a / std::abs(a)) * (bb / t);
+        }
+        H[(size_t)j * m_ + j] = cs_c[j] * a + cs_s[j] * bb;
+        H[(size_t)(j + 1) * m_ + j] = 0.0;
+        g[j + 1] = -std::conj(cs_s[j]) * g[j];
+        g[j] = cs_c[j] * g[j];
+        ++st.iters;
+        ++j;
+        if (std::abs(g[j]) <= tol * bnorm || st.iters >= maxit || hn == 0.0) break;
+      }
+      // back substitution H y = g
+      std::vector<C> yy(j);
+      for (int i = j - 1; i >= 0; --i) {
+        C acc = g[i];
+        for (int k = i + 1; k < j; ++k) acc -= H[(size_t)i * m_ + k] * yy[k];
+        yy[i] = acc / H[(size_t)i * m_ + i];
+      }
+      for (int i = 0; i < j; ++i) y[i] = {yy[i].real(), yy[i].imag()};
+      vmultiaxpy<R>(bl, y.data(), flexible_ ? Z_ : V_, n, j, x, n);
+      // true residual
+      A.apply(x, w_, s);
+      ++st.applies;
+      vsub<R>(b, w_, r_, n, s);
+      beta = std::sqrt(vnorm2<R>(bl, r_, n));
+    }
+    return st;
+  }
+
+  size_t n() const { return n_; }
+
+ private:
+  size_t n_;
+  int m_;
+  bool flexible_;
+  cplx<R>* V_ = nullptr;
+  cplx<R>* Z_ = nullptr;
+  cplx<R>* w_ = nullptr;
+  cplx<R>* r_ = nullptr;
+};
+
+}  // namespace mgc

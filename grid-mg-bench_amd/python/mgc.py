@@ -26,6 +26,29 @@ _c = ctypes
 _vp, _i, _sz, _u64 = _c.c_void_p, _c.c_int, _c.c_size_t, _c.c_uint64
 _ip = _c.POINTER(_c.c_int)
 
+
+class MGParams(ctypes.Structure):
+    """mgc_mg_params of include/mgc.h."""
+    _fields_ = [("nlevels", _c.c_int), ("block", (_c.c_int * 4) * 3), ("nb", _c.c_int * 3),
+                ("precision", _c.c_int), ("coarsen_mode", _c.c_int),
+                ("setup_iters", _c.c_int), ("setup_solve_iters", _c.c_int),
+                ("setup_tol", _c.c_double), ("smoother_tol", _c.c_double),
+                ("smoother_iters", _c.c_int), ("smoother_restart", _c.c_int),
+                ("wrapper_tol", _c.c_double), ("wrapper_iters", _c.c_int),
+                ("wrapper_restart", _c.c_int), ("coarsest_tol", _c.c_double),
+                ("coarsest_iters", _c.c_int), ("coarsest_restart", _c.c_int),
+                ("seed", _c.c_ulonglong)]
+
+
+class SolveStats(ctypes.Structure):
+    """mgc_solve_stats of include/mgc.h."""
+    _fields_ = [("iterations", _c.c_int), ("converged", _c.c_int),
+                ("precond_applies", _c.c_int), ("true_residual", _c.c_double),
+                ("seconds", _c.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
 # argument types of every entry point (return type is int unless noted)
 _SIGS = {
     "mgc_last_error": ([], _c.c_char_p),
@@ -80,6 +103,17 @@ _SIGS = {
     "mgc_fine_op_Mdiag": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "mgc_fine_op_Mdir": ([_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i),
     "mgc_random_gauge": ([_vp, _ip, _c.c_double, _u64, _u64, _vp], _i),
+    "mgc_mg_params_default": ([_c.POINTER(MGParams)], _i),
+    "mgc_mg_create": ([_c.POINTER(_vp), _vp, _c.POINTER(MGParams)], _i),
+    "mgc_mg_destroy": ([_vp], _i),
+    "mgc_mg_setup": ([_vp, _vp], _i),
+    "mgc_mg_precondition": ([_vp, _vp, _vp, _vp], _i),
+    "mgc_mg_vcycle": ([_vp, _vp, _vp, _i, _c.c_double, _vp], _i),
+    "mgc_mg_solve": ([_vp, _vp, _vp, _c.c_double, _i, _i, _c.POINTER(SolveStats), _vp], _i),
+    "mgc_mg_coarse_op": ([_vp, _i, _c.POINTER(_vp)], _i),
+    "mgc_mg_aggregation": ([_vp, _i, _c.POINTER(_vp)], _i),
+    "mgc_fine_gmres_solve": ([_vp, _vp, _vp, _c.c_double, _i, _i, _c.POINTER(SolveStats),
+                              _vp], _i),
     "mgc_coarsen_operator": ([_vp, _vp, _vp, _i, _vp], _i),
     "mgc_coarsen_coarse_operator": ([_vp, _vp, _vp, _i, _vp], _i),
     "mgc_aggregation_create_ex": ([_c.POINTER(_vp), _ip, _ip, _i, _i], _i),
@@ -279,15 +313,25 @@ class CoarseOp:
         d = (_c.c_int * 4)(*[int(v) for v in dims])
         call("mgc_coarse_op_create", _c.byref(h), d, int(nbasis), int(prec), int(hermitian))
         self.handle = h.value
+        self.owned = True
         self.dims = tuple(int(v) for v in dims)
         self.N = int(nbasis)
         self.prec = int(prec)
         self.V = int(np.prod(self.dims))
 
+    @classmethod
+    def borrow(cls, handle: int, dims, nbasis: int, prec: int) -> "CoarseOp":
+        """Wrap a handle owned elsewhere (e.g. by a multigrid hierarchy)."""
+        self = cls.__new__(cls)
+        self.handle, self.owned = handle, False
+        self.dims, self.N, self.prec = tuple(dims), int(nbasis), int(prec)
+        self.V = int(np.prod(self.dims))
+        return self
+
     def close(self) -> None:
-        if self.handle:
+        if self.handle and self.owned:
             call("mgc_coarse_op_destroy", self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -381,6 +425,10 @@ class Aggregation:
         bd = (_c.c_int * 4)(*[int(v) for v in block])
         call("mgc_aggregation_create_ex", _c.byref(h), fd, bd, int(nb), int(site_elems))
         self.handle = h.value
+        self.owned = True
+        self._geometry(fine_dims, block, nb, site_elems)
+
+    def _geometry(self, fine_dims, block, nb, site_elems):
         self.Ef = int(site_elems)
         self.fine_dims = tuple(int(v) for v in fine_dims)
         self.block = tuple(int(v) for v in block)
@@ -390,10 +438,18 @@ class Aggregation:
         self.Vf = int(np.prod(self.fine_dims))
         self.Vc = int(np.prod(self.coarse_dims))
 
+    @classmethod
+    def borrow(cls, handle: int, fine_dims, block, nb: int, site_elems: int = 12):
+        """Wrap a handle owned elsewhere (e.g. by a multigrid hierarchy)."""
+        self = cls.__new__(cls)
+        self.handle, self.owned = handle, False
+        self._geometry(fine_dims, block, nb, site_elems)
+        return self
+
     def close(self) -> None:
-        if self.handle:
+        if self.handle and self.owned:
             call("mgc_aggregation_destroy", self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -505,6 +561,83 @@ def coarsen_coarse_operator(cop2: CoarseOp, cop1: CoarseOp, agg: "Aggregation",
     """CoarsenOperator of a coarse operator onto the next level."""
     call("mgc_coarsen_coarse_operator", cop2.handle, cop1.handle, agg.handle, int(mode),
          stream.handle if stream else None)
+
+
+def mg_params(**kw) -> MGParams:
+    """Reference defaults (include/MultigridParams.h:95-133) with overrides.
+    block / nb take per-coarsening lists."""
+    p = MGParams()
+    call("mgc_mg_params_default", _c.byref(p))
+    for k, v in kw.items():
+        if k == "block":
+            for l, b in enumerate(v):
+                for d in range(4):
+                    p.block[l][d] = int(b[d])
+        elif k == "nb":
+            for l, b in enumerate(v):
+                p.nb[l] = int(b)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Multigrid:
+    """Handle of the multigrid hierarchy (MultiGridPreconditioner)."""
+
+    def __init__(self, fop: "FineOp", params: MGParams):
+        h = _vp()
+        call("mgc_mg_create", _c.byref(h), fop.handle, _c.byref(params))
+        self.handle = h.value
+        self.fop = fop
+        self.params = params
+        self.stream = None
+
+    def close(self):
+        if self.handle:
+            call("mgc_mg_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _s(self):
+        return self.stream.handle if self.stream else None
+
+    def setup(self, stream=None):
+        self.stream = stream
+        call("mgc_mg_setup", self.handle, self._s())
+
+    def precondition(self, i, o):
+        call("mgc_mg_precondition", self.handle, _p(i), _p(o), self._s())
+
+    def vcycle(self, i, o, max_iter=1, tol=0.0):
+        call("mgc_mg_vcycle", self.handle, _p(i), _p(o), int(max_iter), float(tol), self._s())
+
+    def solve(self, b, x, tol=1e-10, maxit=200, restart=20) -> dict:
+        st = SolveStats()
+        call("mgc_mg_solve", self.handle, _p(b), _p(x), float(tol), int(maxit), int(restart),
+             _c.byref(st), self._s())
+        return st.as_dict()
+
+    def coarse_op_handle(self, level: int) -> int:
+        h = _vp()
+        call("mgc_mg_coarse_op", self.handle, int(level), _c.byref(h))
+        return h.value
+
+    def aggregation_handle(self, level: int) -> int:
+        h = _vp()
+        call("mgc_mg_aggregation", self.handle, int(level), _c.byref(h))
+        return h.value
+
+
+def fine_gmres_solve(fop, b, x, tol=1e-10, maxit=1000, restart=20, stream=None) -> dict:
+    st = SolveStats()
+    call("mgc_fine_gmres_solve", fop.handle, _p(b), _p(x), float(tol), int(maxit), int(restart),
+         _c.byref(st), stream.handle if stream else None)
+    return st.as_dict()
 
 
 def construct_remaining_links(cop: CoarseOp, disp_have=-1, stream=None):

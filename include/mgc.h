@@ -290,6 +290,70 @@ int mgc_coarsen_coarse_operator(mgc_coarse_op* cop2, mgc_coarse_op* cop1, mgc_ag
  * on the off-diagonal chirality blocks. */
 int mgc_coarse_op_construct_remaining_links(mgc_coarse_op* op, int disp_have, void* stream);
 
+/* ------------------------------------------------------------------ */
+/* multigrid hierarchy, K-cycle / V-cycle, outer solve                 */
+/* ------------------------------------------------------------------ */
+/* Parameters of the hierarchy (MultiGridParams, include/MultigridParams.h:73-139;
+ * mgc_mg_params_default fills the reference defaults of :95-133). */
+typedef struct mgc_mg_params {
+  int nlevels;             /* 2..4 */
+  int block[3][4];         /* aggregation block of each coarsening */
+  int nb[3];               /* basis vectors of each coarsening (N = 2*nb) */
+  int precision;           /* hierarchy precision: MGC_F64 (plain) or MGC_F32 */
+  int coarsen_mode;        /* MGC_COARSEN_FULL or MGC_COARSEN_DIRSAVE */
+  int setup_iters;         /* test-vector rounds (setupInitialIter) */
+  int setup_solve_iters;   /* GMRES iterations per test vector and round */
+  double setup_tol;
+  double smoother_tol;     /* post-smoother GMRES */
+  int smoother_iters;
+  int smoother_restart;
+  double wrapper_tol;      /* K-cycle FGMRES on the coarse levels */
+  int wrapper_iters;
+  int wrapper_restart;
+  double coarsest_tol;     /* coarsest-level GMRES */
+  int coarsest_iters;
+  int coarsest_restart;
+  unsigned long long seed; /* test-vector generator seed */
+} mgc_mg_params;
+
+typedef struct mgc_solve_stats {
+  int iterations;          /* outer (F)GMRES iterations */
+  int converged;
+  int precond_applies;
+  double true_residual;    /* ||b - D x|| / ||b||, recomputed in FP64 */
+  double seconds;          /* wall time of the solve */
+} mgc_solve_stats;
+
+typedef struct mgc_mg mgc_mg;
+
+int mgc_mg_params_default(mgc_mg_params* params);
+/* MultiGridPreconditioner over a fine Wilson-clover operator
+ * (include/MultigridPreconditioner.h:137-163, createMGInstance :665). */
+int mgc_mg_create(mgc_mg** mg, mgc_fine_op* fop, const mgc_mg_params* params);
+int mgc_mg_destroy(mgc_mg* mg);
+/* initialSetup (include/MultigridPreconditioner.h:165-211): test vectors on
+ * every level, aggregation + block orthonormalisation, FP64 Galerkin
+ * operators (rounded to FP32 for an FP32 hierarchy).  Later calls on the
+ * hierarchy must use the same stream. */
+int mgc_mg_setup(mgc_mg* mg, void* stream);
+/* one preconditioner application, FP64 vectors [site][12]
+ * (MultiGridPreconditioner::operator(), include/MultigridPreconditioner.h:253-309) */
+int mgc_mg_precondition(mgc_mg* mg, const void* in, void* out, void* stream);
+/* NonHermitianVCyclePreconditioner::operator() (include/VCycleRework.h:84-189):
+ * pre-smooth, coarse correction, post-smooth, up to max_iter cycles or
+ * ||r|| <= tol ||in|| (tol = 0: exactly max_iter cycles). */
+int mgc_mg_vcycle(mgc_mg* mg, const void* in, void* out, int max_iter, double tol, void* stream);
+/* FGMRES(D) x = b preconditioned by the hierarchy (src/Test_wilsonclover_mg.cc:140-151),
+ * FP64 outer iteration, zero initial guess, stop at ||b - D x|| <= tol ||b||. */
+int mgc_mg_solve(mgc_mg* mg, const void* b, void* x, double tol, int maxit, int restart,
+                 mgc_solve_stats* stats, void* stream);
+/* the FP64 coarse operator of level >= 1 and the aggregation of level >= 0 (owned by mg) */
+int mgc_mg_coarse_op(mgc_mg* mg, int level, mgc_coarse_op** cop);
+int mgc_mg_aggregation(mgc_mg* mg, int level, mgc_aggregation** agg);
+/* unpreconditioned restarted GMRES on the fine operator, FP64 (baseline solve) */
+int mgc_fine_gmres_solve(mgc_fine_op* fop, const void* b, void* x, double tol, int maxit,
+                         int restart, mgc_solve_stats* stats, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

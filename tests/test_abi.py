@@ -18,6 +18,20 @@ def test_binding_covers_declared_symbols():
     assert not [n for n in names if n not in mgc._SIGS]
 
 
+def test_mg_params_layout_and_reference_defaults():
+    """The ctypes mirror of mgc_mg_params matches the C layout: the C side fills
+    the reference defaults (include/MultigridParams.h:95-133) field by field."""
+    p = mgc.mg_params()
+    assert p.nlevels == 3
+    assert [list(p.block[0]), list(p.block[1])] == [[4, 4, 4, 4], [2, 2, 2, 2]]
+    assert (p.smoother_tol, p.smoother_iters, p.smoother_restart) == (1e-3, 4, 4)
+    assert (p.wrapper_tol, p.wrapper_iters, p.wrapper_restart) == (1e-1, 10, 5)
+    assert (p.coarsest_tol, p.coarsest_iters, p.coarsest_restart) == (5e-2, 50, 10)
+    assert p.seed == 1 | (2 << 16) | (3 << 32) | (4 << 48)
+    q = mgc.mg_params(precision=mgc.F64, nb=[12, 8], block=[(2, 2, 2, 2), (3, 3, 3, 3)])
+    assert q.precision == mgc.F64 and list(q.nb)[:2] == [12, 8] and list(q.block[1]) == [3] * 4
+
+
 def test_errors_are_status_codes_without_device():
     L = mgc.lib()
     # argument validation happens before any device call

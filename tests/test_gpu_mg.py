@@ -1,0 +1,126 @@
+"""GPU tests of the multigrid hierarchy and solve.
+
+Correctness of a solve is the true residual ||b - D x|| / ||b|| recomputed
+here with the library's FP64 fine operator and numpy norms (target 1e-10,
+accepted up to 1.2e-10 for the final rounding of x).  The mixed-precision
+hierarchy must converge in no more outer iterations than the FP64 plain
+form of the same hierarchy (BiCGSTAB/GMRES internals of Grid are not
+available, so the FP64 plain form of this build is the reference count).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import gauss, rel
+
+pytestmark = pytest.mark.gpu
+MASS, CSW = -0.1, 1.0
+
+
+def make_fine(gpu, dims, seed=91):
+    V = int(np.prod(dims))
+    g = gpu.DeviceBuffer(4 * V * 9 * 16)
+    gpu.random_gauge(g, dims, 0.4, seed, 1)
+    fop = gpu.FineOp(dims, MASS, CSW)
+    fop.set_gauge(g)
+    return fop
+
+
+def true_residual(gpu, fop, b, x, V):
+    r = gpu.DeviceBuffer(V * 12 * 16)
+    fop.M(x, r, gpu.F64)
+    gpu.synchronize()
+    Dx = r.to_numpy(np.complex128, (V, 12))
+    bb = b.to_numpy(np.complex128, (V, 12))
+    return rel(bb, Dx)
+
+
+SMALL = dict(nlevels=3, block=[(4, 4, 4, 4), (2, 2, 2, 2)], nb=[8, 8])
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    dims = (16, 16, 16, 16)
+    fop = make_fine(gpu, dims)
+    V = int(np.prod(dims))
+    b = gpu.DeviceBuffer(V * 12 * 16)
+    gpu.random_gaussian(b, V * 12, gpu.F64, 92, 1)
+    return dims, V, fop, b
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_mg_solve_reaches_1e10(gpu, small, prec):
+    dims, V, fop, b = small
+    print(f"[case] mg solve {dims} prec={prec}", flush=True)
+    mg = gpu.Multigrid(fop, gpu.mg_params(precision=prec, **SMALL))
+    mg.setup()
+    x = gpu.DeviceBuffer(V * 12 * 16)
+    st = mg.solve(b, x, tol=1e-10, maxit=200, restart=20)
+    res = true_residual(gpu, fop, b, x, V)
+    print(f"  {st}  recomputed residual {res:.3e}", flush=True)
+    assert st["converged"] == 1
+    assert res < 1.2e-10
+
+
+def test_mixed_no_more_outer_iterations_than_fp64(gpu, small):
+    dims, V, fop, b = small
+    its = {}
+    for prec in (0, 1):
+        mg = gpu.Multigrid(fop, gpu.mg_params(precision=prec, **SMALL))
+        mg.setup()
+        x = gpu.DeviceBuffer(V * 12 * 16)
+        its[prec] = mg.solve(b, x, tol=1e-10, maxit=200, restart=20)["iterations"]
+    print(f"  outer iterations fp64 {its[0]}  mixed {its[1]}", flush=True)
+    assert its[1] <= its[0]
+
+
+def test_hierarchy_galerkin_and_transfer_checks(gpu, small):
+    """The runChecks of include/MultigridPreconditioner.h:311-446 on level 1:
+    (1 - P R) v_i = 0 on the basis, (1 - R P) v_c = 0, R D P = D_c."""
+    dims, V, fop, b = small
+    mg = gpu.Multigrid(fop, gpu.mg_params(precision=0, **SMALL))
+    mg.setup()
+    cop = gpu.CoarseOp.borrow(mg.coarse_op_handle(1), (4, 4, 4, 4), 16, gpu.F64)
+    agg = gpu.Aggregation.borrow(mg.aggregation_handle(0), dims, (4, 4, 4, 4), 8)
+    Vc, N = 256, 16
+    vc = gpu.DeviceBuffer(Vc * N * 16)
+    gpu.random_gaussian(vc, Vc * N, gpu.F64, 93, 1)
+    f1, f2 = gpu.DeviceBuffer(V * 12 * 16), gpu.DeviceBuffer(V * 12 * 16)
+    c1, c2 = gpu.DeviceBuffer(Vc * N * 16), gpu.DeviceBuffer(Vc * N * 16)
+    agg.promote(vc, f1)
+    agg.project(f1, c1)
+    fop.M(f1, f2, gpu.F64)
+    agg.project(f2, c2)
+    cop.M(vc, f1)
+    gpu.synchronize()
+    a = vc.to_numpy(np.complex128, (Vc, N))
+    assert rel(a, c1.to_numpy(np.complex128, (Vc, N))) < 1e-13
+    assert rel(c2.to_numpy(np.complex128, (Vc, N)), f1.to_numpy(np.complex128, (Vc, N))) < 1e-12
+
+
+def test_vcycle_reduces_residual(gpu, small):
+    """One V-cycle (src/Test_rb_vcycle.cc:338-347 asks for < 3.8e-1 after one cycle)."""
+    dims, V, fop, b = small
+    mg = gpu.Multigrid(fop, gpu.mg_params(precision=1, **SMALL))
+    mg.setup()
+    x = gpu.DeviceBuffer(V * 12 * 16)
+    mg.vcycle(b, x, max_iter=1)
+    r1 = true_residual(gpu, fop, b, x, V)
+    mg.vcycle(b, x, max_iter=4)
+    r4 = true_residual(gpu, fop, b, x, V)
+    print(f"  V-cycle residual after 1 cycle {r1:.3e}, after 4 {r4:.3e}", flush=True)
+    assert r1 < 3.8e-1
+    assert r4 < r1
+
+
+def test_unpreconditioned_gmres_converges_small(gpu):
+    dims = (8, 8, 8, 8)
+    V = int(np.prod(dims))
+    fop = make_fine(gpu, dims, 95)
+    b = gpu.DeviceBuffer(V * 12 * 16)
+    gpu.random_gaussian(b, V * 12, gpu.F64, 96, 1)
+    x = gpu.DeviceBuffer(V * 12 * 16)
+    st = gpu.fine_gmres_solve(fop, b, x, tol=1e-10, maxit=5000, restart=30)
+    res = true_residual(gpu, fop, b, x, V)
+    print(f"  plain GMRES {st} recomputed {res:.3e}", flush=True)
+    assert st["converged"] == 1 and res < 1.2e-10

@@ -13,6 +13,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -129,6 +130,8 @@ def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, che
         bytes_in = V * nrhs * N * 8
     stream.sync()
     t_host, t_dev = Timer(mgc, stream).run(step, steps, warmup)
+    variant = ctypes.c_int(-1)
+    mgc.call("mgc_mrhs_get_variant", mp, N, nrhs, ctypes.byref(variant))
     bytes_alg = 9 * V * N * N * wl + (0 if mp == mgc.F32 else 9 * V * 4) + bytes_in + \
         V * nrhs * N * 8
     flops_alg = 8 * 9 * V * N * N * nrhs
@@ -151,6 +154,8 @@ def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, che
         "bytes_per_launch": bytes_alg,
         "flops_per_launch": flops_alg,
         "dtype": names[mp],
+        "kernel_variant": variant.value,
+        "kernel_name": ("k_mrhs" if variant.value == 0 else "k_mrhs_reg"),
     }
     if check:
         import oracle
@@ -169,30 +174,71 @@ def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, che
     return res
 
 
-def cpu_baseline(dims, N):
-    """FP64 oracle coarse M on the host cores, timed with the reference's
-    protocol (include/Benchmark_helpers.h:36-59: 10 calibration calls, then
-    min(1000, nSecMin*10/t10 + 1) calls, nSecMin = 5)."""
+def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10):
+    """Three-level K-cycle multigrid solve of the Wilson-clover system
+    (m = -0.1, csw = 1.0, random SU(3) field eps = 0.4) on one Gaussian rhs:
+    FGMRES(20) outer in FP64 with true-residual recomputation, hierarchy in
+    FP64 ("plain form") and in FP32 ("mixed").  Setup and solve timed apart."""
+    V = int(np.prod(dims))
+    g = mgc.DeviceBuffer(4 * V * 9 * 16)
+    mgc.random_gauge(g, dims, 0.4, SEED, 21, stream)
+    fop = mgc.FineOp(dims, -0.1, 1.0)
+    fop.set_gauge(g, stream)
+    b = mgc.DeviceBuffer(V * 12 * 16)
+    mgc.random_gaussian(b, V * 12, mgc.F64, SEED, 22, stream)
+    x = mgc.DeviceBuffer(V * 12 * 16)
+    stream.sync()
+    lat = [tuple(dims)]
+    for blk in blocks:
+        lat.append(tuple(a // b for a, b in zip(lat[-1], blk)))
+    out = {"workload": "Wilson-clover " + " -> ".join("x".join(map(str, d)) for d in lat)
+                       + f", nbasis {'/'.join(str(2 * n) for n in nbs)}, K-cycle, FGMRES(20) "
+                         f"to {tol:g}"}
+    for name, prec in (("fp64_plain", mgc.F64), ("mixed_fp32", mgc.F32)):
+        p = mgc.mg_params(nlevels=len(blocks) + 1, block=blocks, nb=nbs, precision=prec)
+        mg = mgc.Multigrid(fop, p)
+        t0 = time.perf_counter()
+        mg.setup(stream)
+        t_setup = time.perf_counter() - t0
+        st = mg.solve(b, x, tol=tol, maxit=500, restart=20)
+        out[name] = {"setup_s": t_setup, "solve_s": st["seconds"],
+                     "outer_iterations": st["iterations"], "converged": bool(st["converged"]),
+                     "true_residual": st["true_residual"],
+                     "precond_applies": st["precond_applies"]}
+        mg.close()
+    for buf in (g, b, x):
+        buf.close()
+    fop.close()
+    return out
+
+
+def cpu_baseline(dims, N, nrhs):
+    """FP64 oracle multi-rhs coarse M (one OpenMP-parallel site loop, dense
+    N x N matvec per point and rhs) on the host cores, timed with the
+    reference's protocol (include/Benchmark_helpers.h:36-59: calibration
+    calls, then min(1000, nSecMin*ncal/tcal + 1) calls) bounded to ~10 s."""
     import oracle
     V = int(np.prod(dims))
-    links = oracle.gaussian(9 * V * N * N, SEED, 1).reshape(9, V, N, N)
-    v = oracle.gaussian(V * N, SEED, 2).reshape(V, N)
+    links = oracle.gaussian(9 * V * N * N, SEED, 11).reshape(9, V, N, N)
+    v = oracle.gaussian(V * nrhs * N, SEED, 12).reshape(V, nrhs, N)
+    ncal = 2
     t0 = time.perf_counter()
-    for _ in range(10):
-        oracle.coarse_apply(dims, N, links, v)
-    t10 = time.perf_counter() - t0
-    n = int(min(1000, 5.0 * 10 / t10 + 1))
+    for _ in range(ncal):
+        oracle.coarse_mrhs_M(dims, N, nrhs, links, v)
+    tcal = time.perf_counter() - t0
+    n = int(min(1000, 8.0 * ncal / tcal + 1))
     t0 = time.perf_counter()
     for _ in range(n):
-        oracle.coarse_apply(dims, N, links, v)
+        oracle.coarse_mrhs_M(dims, N, nrhs, links, v)
     t = (time.perf_counter() - t0) / n
     return {
-        "value": V / t,
+        "value": V * nrhs / t,
         "unit": "rhs·sites/s",
         "cores": oracle.num_threads(),
         "kind": "port",
-        "sample": f"FP64 oracle coarse M, {'x'.join(map(str, dims))} nbasis={N}, 1 rhs, "
-                  f"{n} timed calls after 10 calibration calls ({n * t:.1f} s)",
+        "sample": f"FP64 oracle multi-rhs coarse M, {'x'.join(map(str, dims))} nbasis={N} "
+                  f"nrhs={nrhs}: {n} timed calls after {ncal} calibration calls "
+                  f"({n * t:.1f} s)",
     }
 
 
@@ -202,6 +248,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-mg", action="store_true", help="skip the multigrid-solve leg")
+    ap.add_argument("--mg-dims", default="48,48,48,48",
+                    help="fine lattice of the multigrid solve (blocks 4^4 then 4^4)")
+    ap.add_argument("--only-headline", action="store_true",
+                    help="run only the headline leg (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
@@ -214,26 +265,37 @@ def main() -> int:
     stream = mgc.Stream()
     legs = {}
     dims8 = (8, 8, 8, 8)
-    log("leg: single-rhs fp64 nbasis=32")
-    legs["M_1rhs_f64_n32"] = single_rhs_leg(mgc, stream, dims8, 32, mgc.F64, args.steps,
-                                            args.warmup)
-    log("leg: single-rhs fp32 nbasis=32")
-    legs["M_1rhs_f32_n32"] = single_rhs_leg(mgc, stream, dims8, 32, mgc.F32, args.steps,
-                                            args.warmup)
-    log("leg: single-rhs fp64 nbasis=64")
-    legs["M_1rhs_f64_n64"] = single_rhs_leg(mgc, stream, dims8, 64, mgc.F64, args.steps,
-                                            args.warmup)
-    for nrhs in (12, 24, 48):
-        for mp, nm in ((mgc.F32, "f32"), (mgc.F16, "f16"), (mgc.BF16, "bf16")):
-            log(f"leg: multi-rhs {nm} nbasis=64 nrhs={nrhs}")
-            legs[f"M_mrhs{nrhs}_{nm}_n64"] = mrhs_leg(mgc, stream, dims8, 64, nrhs, mp,
-                                                      args.steps, args.warmup)
-    log("leg: multi-rhs f16 nbasis=64 nrhs=48, fp32 input quantised in the step")
-    legs["M_mrhs48_f16_n64_f32in"] = mrhs_leg(mgc, stream, dims8, 64, 48, mgc.F16, args.steps,
-                                              args.warmup, half_input=False)
-
-    head_key = "M_1rhs_f64_n32"
+    # headline: multi-rhs coarse stencil, 8^4 coarse lattice, nbasis 64, 48 rhs,
+    # FP16 MFMA with FP32 accumulation (pre-quantised FP16 inputs, FP32 output)
+    head_key = "M_mrhs48_f16_n64"
+    log("leg (headline): multi-rhs f16 nbasis=64 nrhs=48")
+    legs[head_key] = mrhs_leg(mgc, stream, dims8, 64, 48, mgc.F16, args.steps, args.warmup)
+    if not args.only_headline:
+        log("leg: single-rhs fp64 nbasis=64 (reference point of the multi-rhs ratio)")
+        legs["M_1rhs_f64_n64"] = single_rhs_leg(mgc, stream, dims8, 64, mgc.F64, args.steps,
+                                                args.warmup)
+        log("leg: single-rhs fp64 / fp32 nbasis=32")
+        legs["M_1rhs_f64_n32"] = single_rhs_leg(mgc, stream, dims8, 32, mgc.F64, args.steps,
+                                                args.warmup)
+        legs["M_1rhs_f32_n32"] = single_rhs_leg(mgc, stream, dims8, 32, mgc.F32, args.steps,
+                                                args.warmup)
+        for nrhs in (12, 24, 48):
+            for mp, nm in ((mgc.F32, "f32"), (mgc.F16, "f16"), (mgc.BF16, "bf16")):
+                key = f"M_mrhs{nrhs}_{nm}_n64"
+                if key in legs:
+                    continue
+                log(f"leg: multi-rhs {nm} nbasis=64 nrhs={nrhs}")
+                legs[key] = mrhs_leg(mgc, stream, dims8, 64, nrhs, mp, args.steps, args.warmup)
+        log("leg: multi-rhs f16 nbasis=64 nrhs=48, fp32 input quantised in the step")
+        legs["M_mrhs48_f16_n64_f32in"] = mrhs_leg(mgc, stream, dims8, 64, 48, mgc.F16,
+                                                  args.steps, args.warmup, half_input=False)
     head = legs[head_key]
+    mg = None
+    if not args.no_mg and not args.only_headline:
+        d = tuple(int(v) for v in args.mg_dims.split(","))
+        log(f"leg: multigrid solve {d}")
+        mg = mg_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)], [16, 16])
+    ref64 = legs.get("M_1rhs_f64_n64")
     line = {
         "metric": METRIC,
         "value": head["rhs_sites_per_s"],
@@ -245,23 +307,30 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": head["dtype"],
+        "dtype": "f16",
         "data": "synthetic (counter-based Gaussian links and vectors, seed {1,2,3,4})",
         "config": {"workload": head["workload"], "headline_leg": head_key,
-                   "coarse_lattice": "8^4", "nbasis": 32, "nrhs": 1},
+                   "coarse_lattice": "8^4", "nbasis": 64, "nrhs": 48,
+                   "accumulate": "fp32", "output": "fp32"},
         "roofline": {
-            "bound": "hbm",
-            "achieved": head["GBps"],
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": head["hbm_frac"],
+            "bound": head["bound"],
+            "achieved": head["GBps"] if head["bound"] == "hbm" else head["TFLOPs"],
+            "peak": HBM_PEAK_GBS if head["bound"] == "hbm" else PEAK_TFLOPS["f16"],
+            "unit": "GB/s" if head["bound"] == "hbm" else "TFLOP/s",
+            "frac": head["hbm_frac"] if head["bound"] == "hbm" else head["mfma_frac"],
             "traffic": None,
+            "kernel": head["kernel_name"],
+            "bytes_per_launch": head["bytes_per_launch"],
+            "flops_per_launch": head["flops_per_launch"],
         },
+        "speedup_vs_fp64_single_rhs": (head["rhs_sites_per_s"] / ref64["rhs_sites_per_s"]
+                                       if ref64 else None),
+        "mg_solve": mg,
         "legs": legs,
     }
     if not args.no_cpu:
-        log("cpu baseline (FP64 oracle)")
-        line["cpu_baseline"] = cpu_baseline(dims8, 32)
+        log("cpu baseline (FP64 oracle, same multi-rhs workload)")
+        line["cpu_baseline"] = cpu_baseline(dims8, 64, 48)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

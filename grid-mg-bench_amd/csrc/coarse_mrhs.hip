@@ -306,15 +306,143 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
   }
 }
 
+// Register-only variant: no LDS and no barrier.  Each wavefront owns MPW
+// consecutive m-tiles of one site and builds the B' fragments it needs in
+// registers straight from 16-byte loads of the neighbour's input (lanes l and
+// l^1 read the same chunk), so wavefronts stream independently and the
+// hardware overlaps one wavefront's loads with another's MFMAs.  The B'
+// values, scale folding and MFMA order are those of k_mrhs.
+template <int KIND, int N, int NT, int MPW>
+__global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
+    const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
+    const float* __restrict__ in_scale, float* __restrict__ out, const int* __restrict__ nbr,
+    int nrhs, int rhs0, int nr, int dag) {
+  using M = MK<KIND>;
+  typedef typename M::T T;
+  constexpr int NMT = N / 16;
+  constexpr int NKC = 2 * N / M::KC;
+  constexpr bool SCALED = KIND != MGC_F32;
+  const int x = blockIdx.x;
+  const int w = threadIdx.x >> 6;
+  const int l = threadIdx.x & 63;
+  const T* in = reinterpret_cast<const T*>(in_);
+
+  int ys[9];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) ys[p] = nbr[(size_t)x * 8 + p];
+  ys[8] = x;
+  // this lane's columns: n = nt*16 + l%16, rhs j = n/2, part u = n%2
+  const int u = l & 1;
+  float invR[NT], Rj[NT];
+  int jj[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int j = (nt * 16 + (l & 15)) >> 1;
+    jj[nt] = j;
+    float R = 0.f;
+    if constexpr (SCALED) {
+      if (j < nr)
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+          R = fmaxf(R, ascale[(size_t)x * 9 + p] * in_scale[(size_t)ys[p] * nrhs + rhs0 + j]);
+    } else {
+      R = 1.f;
+    }
+    invR[nt] = R > 0.f ? 1.f / R : 0.f;
+    Rj[nt] = R;
+  }
+  const int cl = (l >> 4) * M::KCL / 2;  // this lane's first complex within a k-chunk
+
+  floatx4 C[MPW][NT];
+#pragma unroll
+  for (int mt = 0; mt < MPW; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) C[mt][nt] = {0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 1
+  for (int p = 0; p < 9; ++p) {
+    const uint4* Ap = A + (((size_t)x * 9 + p) * NMT + w * MPW) * NKC * 64 + l;
+    const T* base = in + (size_t)ys[p] * nrhs * N * 2;
+    const float sl = SCALED ? ascale[(size_t)x * 9 + p] : 1.f;
+    float f[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      f[nt] = 1.f;
+      if constexpr (SCALED)
+        f[nt] = jj[nt] < nr ? sl * in_scale[(size_t)ys[p] * nrhs + rhs0 + jj[nt]] * invR[nt] : 0.f;
+    }
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      uint4 a[MPW];
+#pragma unroll
+      for (int mt = 0; mt < MPW; ++mt) a[mt] = Ap[((size_t)mt * NKC + kc) * 64];
+      const int c0 = kc * M::KC / 2 + cl;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        uint4 raw4 = make_uint4(0u, 0u, 0u, 0u);
+        if (jj[nt] < nr)
+          raw4 = *reinterpret_cast<const uint4*>(base + (((size_t)(rhs0 + jj[nt])) * N + c0) * 2);
+        T raw[M::KCL], v[M::KCL];
+        __builtin_memcpy(raw, &raw4, 16);
+#pragma unroll
+        for (int h = 0; h < M::KCL / 2; ++h) {
+          const float sg = (dag && (c0 + h) >= N / 2) ? -f[nt] : f[nt];
+          const float re = (float)raw[2 * h] * sg, im = (float)raw[2 * h + 1] * sg;
+          v[2 * h] = (T)(u ? im : re);
+          v[2 * h + 1] = (T)(u ? re : -im);
+        }
+        uint4 b;
+        __builtin_memcpy(&b, v, 16);
+#pragma unroll
+        for (int mt = 0; mt < MPW; ++mt) C[mt][nt] = mfma_chunk<KIND>(a[mt], b, C[mt][nt]);
+      }
+    }
+  }
+#pragma unroll
+  for (int mt = 0; mt < MPW; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int j = jj[nt];
+      if (j < nr) {
+        const float R = Rj[nt];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = (w * MPW + mt) * 16 + 4 * (l >> 4) + i;
+          float val = C[mt][nt][i] * R;
+          if (dag && r >= N / 2) val = -val;
+          out[(((size_t)x * nrhs + rhs0 + j) * N + r) * 2 + u] = val;
+        }
+      }
+    }
+}
+
+// kernel variants: 0 = LDS-staged B', 1 = registers with 1 m-tile per
+// wavefront, 2 = registers with 2 m-tiles per wavefront
+constexpr int MRHS_VARIANTS = 3;
+
+template <int KIND, int N, int NT>
+static void launch_variant(int variant, int V, const uint4* A, const float* as, const void* in,
+                           const float* is, float* out, const int* nbr, int nrhs, int rhs0, int nr,
+                           int dag, hipStream_t s) {
+  const dim3 g(V);
+  if (variant == 0)
+    k_mrhs<KIND, N, NT><<<g, N / 16 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag);
+  else if (variant == 1)
+    k_mrhs_reg<KIND, N, NT, 1><<<g, N / 16 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr,
+                                                        dag);
+  else
+    k_mrhs_reg<KIND, N, NT, 2><<<g, N / 32 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr,
+                                                        dag);
+}
+
 template <int KIND, int N>
-static void launch_mrhs_n(int NT, int V, const uint4* A, const float* as, const void* in,
-                          const float* is, float* out, const int* nbr, int nrhs, int rhs0, int nr,
-                          int dag, hipStream_t s) {
-  const dim3 g(V), b(N / 16 * 64);
+static void launch_mrhs_n(int variant, int NT, int V, const uint4* A, const float* as,
+                          const void* in, const float* is, float* out, const int* nbr, int nrhs,
+                          int rhs0, int nr, int dag, hipStream_t s) {
   switch (NT) {
-#define CASE(nt)                                                                        \
-  case nt:                                                                              \
-    k_mrhs<KIND, N, nt><<<g, b, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag); \
+#define CASE(nt)                                                                              \
+  case nt:                                                                                    \
+    launch_variant<KIND, N, nt>(variant, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s); \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(6) CASE(8)
 #undef CASE
@@ -325,15 +453,64 @@ static void launch_mrhs_n(int NT, int V, const uint4* A, const float* as, const 
 }
 
 template <int KIND>
-static void launch_mrhs(int N, int NT, int V, const uint4* A, const float* as, const void* in,
-                        const float* is, float* out, const int* nbr, int nrhs, int rhs0, int nr,
-                        int dag, hipStream_t s) {
+static void launch_mrhs(int variant, int N, int NT, int V, const uint4* A, const float* as,
+                        const void* in, const float* is, float* out, const int* nbr, int nrhs,
+                        int rhs0, int nr, int dag, hipStream_t s) {
   if (N == 32)
-    launch_mrhs_n<KIND, 32>(NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs_n<KIND, 32>(variant, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
   else if (N == 64)
-    launch_mrhs_n<KIND, 64>(NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs_n<KIND, 64>(variant, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
   else
     throw Error(MGC_ERR_SHAPE, "multi-rhs stencil supports nbasis 32 and 64");
+}
+
+static void launch_any(int mp, int variant, int N, int NT, int V, const uint4* A, const float* as,
+                       const void* in, const float* is, float* out, const int* nbr, int nrhs,
+                       int rhs0, int nr, int dag, hipStream_t s) {
+  if (mp == MGC_F32)
+    launch_mrhs<MGC_F32>(variant, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+  else if (mp == MGC_F16)
+    launch_mrhs<MGC_F16>(variant, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+  else
+    launch_mrhs<MGC_BF16>(variant, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+}
+
+// Autotuner: the first launch of a (precision, nbasis, column tiles) shape
+// times every variant on the caller's own operands (three launches each,
+// HIP events on the caller's stream; the outputs are rewritten by the final
+// launch) and keeps the fastest for the rest of the process.
+static int g_mrhs_choice[4][2][9];  // [prec][N==64][NT] -> variant + 1 (0 = untuned)
+static int g_mrhs_forced = -1;      // >= 0: use this variant (set by mgc_mrhs_set_variant)
+
+static int tuned_variant(int mp, int N, int NT, int V, const uint4* A, const float* as,
+                         const void* in, const float* is, float* out, const int* nbr, int nrhs,
+                         int rhs0, int nr, int dag, hipStream_t s) {
+  if (g_mrhs_forced >= 0) return g_mrhs_forced;
+  int& slot = g_mrhs_choice[mp][N == 64 ? 1 : 0][NT];
+  if (slot) return slot - 1;
+  hipEvent_t e0, e1;
+  MGC_HIP(hipEventCreate(&e0));
+  MGC_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bestv = 0;
+  for (int v = 0; v < MRHS_VARIANTS; ++v) {
+    launch_any(mp, v, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);  // warm-up
+    MGC_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < 3; ++k)
+      launch_any(mp, v, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    MGC_HIP(hipEventRecord(e1, s));
+    MGC_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    MGC_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) {
+      best = ms;
+      bestv = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  slot = bestv + 1;
+  return bestv;
 }
 
 static int tiles_for(int nr) {
@@ -413,6 +590,23 @@ int mgc_quantize(const void* in_f32, size_t rows, int n, int prec, void* out, fl
   });
 }
 
+int mgc_mrhs_set_variant(int variant) {
+  return guarded([&] {
+    MGC_CHECK(variant >= -1 && variant < MRHS_VARIANTS, MGC_ERR_ARG, "unknown kernel variant");
+    g_mrhs_forced = variant;
+  });
+}
+
+int mgc_mrhs_get_variant(int mrhs_prec, int nbasis, int nrhs, int* variant) {
+  return guarded([&] {
+    MGC_CHECK(variant, MGC_ERR_ARG, "null output");
+    MGC_CHECK(mrhs_prec >= MGC_F32 && mrhs_prec <= MGC_BF16, MGC_ERR_PRECISION, "bad precision");
+    const int nr = nrhs < 64 ? nrhs : 64;
+    *variant = g_mrhs_forced >= 0 ? g_mrhs_forced
+                                  : g_mrhs_choice[mrhs_prec][nbasis == 64 ? 1 : 0][tiles_for(nr)] - 1;
+  });
+}
+
 int mgc_coarse_op_prepare_mrhs(mgc_coarse_op* op, int mrhs_prec, void* stream) {
   return guarded([&] {
     MGC_CHECK(op, MGC_ERR_ARG, "null operator handle");
@@ -457,15 +651,10 @@ int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scal
       const int nr = nrhs - r0 < 64 ? nrhs - r0 : 64;
       const int NT = tiles_for(nr);
       const uint4* A = (const uint4*)op->mrhs_links;
-      if (mrhs_prec == MGC_F32)
-        launch_mrhs<MGC_F32>(N, NT, V, A, op->mrhs_scales, src, sc, (float*)out, op->nbr_full,
-                             nrhs, r0, nr, dg, s);
-      else if (mrhs_prec == MGC_F16)
-        launch_mrhs<MGC_F16>(N, NT, V, A, op->mrhs_scales, src, sc, (float*)out, op->nbr_full,
-                             nrhs, r0, nr, dg, s);
-      else
-        launch_mrhs<MGC_BF16>(N, NT, V, A, op->mrhs_scales, src, sc, (float*)out, op->nbr_full,
-                              nrhs, r0, nr, dg, s);
+      const int v = tuned_variant(mrhs_prec, N, NT, V, A, op->mrhs_scales, src, sc, (float*)out,
+                                  op->nbr_full, nrhs, r0, nr, dg, s);
+      launch_any(mrhs_prec, v, N, NT, V, A, op->mrhs_scales, src, sc, (float*)out, op->nbr_full,
+                 nrhs, r0, nr, dg, s);
     }
     if (qbuf) {
       MGC_HIP(hipFreeAsync(qbuf, s));

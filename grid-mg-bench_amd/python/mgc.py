@@ -94,6 +94,8 @@ _SIGS = {
     "mgc_coarse_op_prepare_mrhs": ([_vp, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
+    "mgc_mrhs_set_variant": ([_i], _i),
+    "mgc_mrhs_get_variant": ([_i, _i, _i, _ip], _i),
     "mgc_fine_op_create": ([_c.POINTER(_vp), _ip, _c.c_double, _c.c_double, _i], _i),
     "mgc_fine_op_destroy": ([_vp], _i),
     "mgc_fine_op_set_gauge": ([_vp, _vp, _vp], _i),

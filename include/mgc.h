@@ -179,6 +179,15 @@ int mgc_coarse_op_MooeeInvDag(mgc_coarse_op* op, const void* in, void* out, int 
 int mgc_coarse_op_prepare_mrhs(mgc_coarse_op* op, int mrhs_prec, void* stream);
 int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scales, int in_prec,
                          void* out, int nrhs, int mrhs_prec, int dag, void* stream);
+/* Kernel-variant control of the multi-rhs stencil.  By default the first
+ * call for a (precision, nbasis, column-tile) shape times every variant on
+ * its own operands and keeps the fastest (autotuner).  variant >= 0 forces a
+ * variant (0 = LDS-staged operand, 1 = register operand with one m-tile per
+ * wavefront, 2 = register operand with two m-tiles per wavefront); -1
+ * restores autotuning.  get_variant reports the variant in use (-1 = not
+ * tuned yet). */
+int mgc_mrhs_set_variant(int variant);
+int mgc_mrhs_get_variant(int mrhs_prec, int nbasis, int nrhs, int* variant);
 /* rows x n complex FP32 -> rows x n complex F16/BF16 with scales[row] =
  * max |component| of the row (values divided by it). */
 int mgc_quantize(const void* in_f32, size_t rows, int n, int prec, void* out, float* scales,

@@ -56,6 +56,30 @@ def test_mrhs_matches_oracle(gpu, dims, N, nrhs, mp):
         assert max(rel(ref[:, k], got_h[:, k]) for k in range(nrhs)) < TOL[mp](N)
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("mp", [F32, F16, BF16])
+def test_mrhs_every_kernel_variant(gpu, variant, mp):
+    """Each kernel variant the autotuner may pick is checked on its own."""
+    dims, N, nrhs = (4, 2, 2, 2), 64, 20
+    print(f"[case] mrhs variant={variant} prec={mp}", flush=True)
+    links = rand_links(dims, N, 641)
+    op = gpu.CoarseOp(dims, N, gpu.F64)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 642, nrhs=nrhs)
+    vq = v.astype(np.complex64).astype(np.complex128)
+    ref = oracle.coarse_mrhs_M(dims, N, nrhs, links, vq)
+    gpu.call("mgc_mrhs_set_variant", variant)
+    try:
+        for dag in (0, 1):
+            got = run_mrhs(gpu, op, v, nrhs, mp, dag=dag)
+            for k in range(nrhs):
+                r = ref[:, k] if not dag else oracle.coarse_apply(
+                    dims, N, links, np.ascontiguousarray(vq[:, k]), dag=1)
+                assert rel(r, got[:, k]) < TOL[mp](N), (dag, k)
+    finally:
+        gpu.call("mgc_mrhs_set_variant", -1)
+
+
 @pytest.mark.parametrize("mp", [F32, F16])
 def test_mrhs_dagger(gpu, mp):
     dims, N, nrhs = (2, 2, 2, 2), 32, 6

@@ -253,7 +253,20 @@ def main() -> int:
                     help="fine lattice of the multigrid solve (blocks 4^4 then 4^4)")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the headline leg (for profiling)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy kernel of BYTES bytes (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiler run), then exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import mgc
+        nbytes = (args.copy_calibration + 15) // 16 * 16
+        src, dst = mgc.DeviceBuffer(nbytes), mgc.DeviceBuffer(nbytes)
+        src.zero()
+        for _ in range(max(1, args.steps)):
+            mgc.call("mgc_device_copy", dst.ptr, src.ptr, ctypes.c_size_t(nbytes), None)
+        mgc.synchronize()
+        print(json.dumps({"copy_calibration_bytes": nbytes, "launches": max(1, args.steps)}))
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
               "(the coarse operator shards by 4d domain decomposition; see DESIGN.md)",

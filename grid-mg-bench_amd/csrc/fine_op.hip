@@ -229,48 +229,57 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
         for (int b = 0; b < 6; ++b) cmac(acc[6 * l + a], D[a * 6 + b], v[b]);
     }
   }
-#pragma unroll 1
+  // Hopping terms with the rank-2 spin projector: for the upper spins s = 0, 1
+  // (g_mu maps them to s' = col(s) in {2, 3}),
+  //   h_s = psi_s - sg ph(s) psi_s',  (1 - sg g_mu) psi at s' = -sg ph(s') h_s,
+  // so only the half spinor h is multiplied by the SU(3) link.  The loop is
+  // unrolled so that every spin index is a compile-time constant.
+#pragma unroll
   for (int p = 0; p < 8; ++p) {
     if (!((mask >> p) & 1u)) continue;
     const int mu = p & 3;
     const R sg = p < 4 ? R(1) : R(-1);
     const int y = nbr[(size_t)x * 8 + p];
-    cplx<R> L[9];
-    if (p < 4) {
-      const cplx<R>* u = U + ((size_t)mu * V + x) * 9;
+    cplx<R> h[2][3];
 #pragma unroll
-      for (int k = 0; k < 9; ++k) L[k] = u[k];
-    } else {
-      const cplx<R>* u = U + ((size_t)mu * V + y) * 9;
+    for (int s = 0; s < 2; ++s) {
+      const int cs = G.col[mu][s];
+      const cplx<R> ph = {(R)(sg * G.re[mu][s]), (R)(sg * G.im[mu][s])};
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) L[i * 3 + j] = {u[j * 3 + i].re, -u[j * 3 + i].im};
-    }
-    cplx<R> chi[12];
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      cplx<R> w[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j) w[j] = ld(y, s * 3 + j);
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        cplx<R> a = {R(0), R(0)};
-#pragma unroll
-        for (int j = 0; j < 3; ++j) cmac(a, L[i * 3 + j], w[j]);
-        chi[s * 3 + i] = a;
+      for (int j = 0; j < 3; ++j) {
+        const cplx<R> a = ld(y, s * 3 + j), b = ld(y, cs * 3 + j);
+        const cplx<R> t = cmul(ph, b);
+        h[s][j] = {a.re - t.re, a.im - t.im};
       }
     }
-    // acc += -1/2 (chi - sg * g_mu chi)
+    const cplx<R>* u = U + ((size_t)mu * V + (p < 4 ? x : y)) * 9;
+    cplx<R> chi[2][3];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int i = 0; i < 3; ++i) {
+      cplx<R> L[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j)
+        L[j] = p < 4 ? u[i * 3 + j] : cplx<R>{u[j * 3 + i].re, -u[j * 3 + i].im};
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        cplx<R> a = {R(0), R(0)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) cmac(a, L[j], h[s][j]);
+        chi[s][i] = a;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
       const int cs = G.col[mu][s];
-      const cplx<R> ph = {(R)G.re[mu][s], (R)G.im[mu][s]};
+      // lower spin cs: -1/2 * (-sg ph(cs)) chi_s = (sg/2) ph(cs) chi_s
+      const cplx<R> ph = {(R)(0.5 * sg * G.re[mu][cs]), (R)(0.5 * sg * G.im[mu][cs])};
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const cplx<R> g = cmul(ph, chi[cs * 3 + i]);
-        acc[s * 3 + i].re += R(-0.5) * (chi[s * 3 + i].re - sg * g.re);
-        acc[s * 3 + i].im += R(-0.5) * (chi[s * 3 + i].im - sg * g.im);
+        acc[s * 3 + i].re -= R(0.5) * chi[s][i].re;
+        acc[s * 3 + i].im -= R(0.5) * chi[s][i].im;
+        const cplx<R> t = cmul(ph, chi[s][i]);
+        acc[cs * 3 + i].re += t.re;
+        acc[cs * 3 + i].im += t.im;
       }
     }
   }

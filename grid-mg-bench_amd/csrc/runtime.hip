@@ -25,6 +25,12 @@ __global__ void k_fill_zero_u8(uint8_t* p, size_t n) {
   if (i < n) p[i] = 0u;
 }
 
+__global__ void k_copy_u4(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
 void fill_zero(void* ptr, size_t bytes, hipStream_t s) {
   if (bytes == 0) return;
   size_t words = bytes / 4;
@@ -123,6 +129,18 @@ int mgc_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream) {
     MGC_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, as_stream(stream)));
   });
 }
+int mgc_device_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(dst && src, MGC_ERR_ARG, "null pointer");
+    MGC_CHECK(bytes % 16 == 0 && ((uintptr_t)dst % 16) == 0 && ((uintptr_t)src % 16) == 0,
+              MGC_ERR_ARG, "device_copy needs 16-byte aligned sizes and pointers");
+    const size_t n = bytes / 16;
+    k_copy_u4<<<grid_for(n, 256, 16384), 256, 0, as_stream(stream)>>>((const uint4*)src,
+                                                                      (uint4*)dst, n);
+    MGC_LAUNCH_CHECK();
+  });
+}
+
 int mgc_fill_zero(void* ptr, size_t bytes, void* stream) {
   return guarded([&] {
     MGC_CHECK(ptr || bytes == 0, MGC_ERR_ARG, "null ptr");

@@ -61,6 +61,7 @@ _SIGS = {
     "mgc_memcpy_d2h": ([_vp, _vp, _sz, _vp], _i),
     "mgc_memcpy_d2d": ([_vp, _vp, _sz, _vp], _i),
     "mgc_fill_zero": ([_vp, _sz, _vp], _i),
+    "mgc_device_copy": ([_vp, _vp, _sz, _vp], _i),
     "mgc_stream_create": ([_c.POINTER(_vp)], _i),
     "mgc_stream_destroy": ([_vp], _i),
     "mgc_stream_synchronize": ([_vp], _i),

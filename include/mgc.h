@@ -76,6 +76,9 @@ int mgc_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 int mgc_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int mgc_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
 int mgc_fill_zero(void* ptr, size_t bytes, void* stream);
+/* device-to-device copy by a kernel (16-byte aligned), e.g. for calibrating
+ * hardware byte counters against a copy of known size */
+int mgc_device_copy(void* dst, const void* src, size_t bytes, void* stream);
 int mgc_stream_create(void** stream);
 int mgc_stream_destroy(void* stream);
 int mgc_stream_synchronize(void* stream);

@@ -5,16 +5,21 @@
 namespace mgc {
 
 Blas::Blas(hipStream_t st) : s(st) {
-  MGC_HIP(hipMalloc(&d_part, (size_t)KMAX * NPART * 2 * sizeof(double)));
-  MGC_HIP(hipMalloc(&d_res, (size_t)KMAX * 2 * sizeof(double)));
+  MGC_HIP(hipMalloc(&d_part, (size_t)(KMAX + 1) * NPART * 2 * sizeof(double)));
+  MGC_HIP(hipMalloc(&d_res, (size_t)(KMAX + 1) * 2 * sizeof(double)));
   MGC_HIP(hipMalloc(&d_coef, (size_t)KMAX * sizeof(cplx<double>)));
-  MGC_HIP(hipHostMalloc(&h_res, (size_t)KMAX * 2 * sizeof(double), hipHostMallocDefault));
+  MGC_HIP(hipHostMalloc(&h_res, (size_t)4 * (KMAX + 1) * 2 * sizeof(double),
+                        hipHostMallocDefault));
+  MGC_HIP(hipHostMalloc(&h_coef, (size_t)KMAX * sizeof(cplx<double>), hipHostMallocDefault));
+  MGC_HIP(hipEventCreateWithFlags(&coef_done, hipEventDisableTiming));
 }
 Blas::~Blas() {
   if (d_part) (void)hipFree(d_part);
   if (d_res) (void)hipFree(d_res);
   if (d_coef) (void)hipFree(d_coef);
   if (h_res) (void)hipHostFree(h_res);
+  if (h_coef) (void)hipHostFree(h_coef);
+  if (coef_done) (void)hipEventDestroy(coef_done);
 }
 
 static unsigned ew_grid(size_t n) { return grid_for(n, 256, 8192); }
@@ -127,6 +132,65 @@ __global__ __launch_bounds__(256) void k_dot_final(const double* part, int npart
   }
 }
 
+// partial sums of <V_j, w> for blockIdx.y = j < k, and of |w|^2 for
+// blockIdx.y == k (launched with k + 1 rows when the norm is wanted)
+template <typename R>
+__global__ __launch_bounds__(256) void k_mdot_part(const cplx<R>* V, size_t ldv, int k,
+                                                   const cplx<R>* w, size_t n, double* part) {
+  const int j = blockIdx.y;
+  const bool nrm = j == k;
+  const cplx<R>* v = nrm ? w : V + (size_t)j * ldv;
+  double re = 0.0, im = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double ar = v[i].re, ai = v[i].im;
+    if (nrm) {
+      re += ar * ar + ai * ai;
+    } else {
+      const double br = w[i].re, bi = w[i].im;
+      re += ar * br + ai * bi;
+      im += ar * bi - ai * br;
+    }
+  }
+  const double2 t = block_sum2(re, im);
+  if (threadIdx.x == 0) {
+    part[((size_t)j * gridDim.x + blockIdx.x) * 2] = t.x;
+    part[((size_t)j * gridDim.x + blockIdx.x) * 2 + 1] = t.y;
+  }
+}
+
+template <typename R>
+__global__ void k_vmultisub(const cplx<double>* c, const cplx<R>* V, size_t ldv, int k,
+                            cplx<R>* w, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double re = w[i].re, im = w[i].im;
+    for (int j = 0; j < k; ++j) {
+      const cplx<R> v = V[(size_t)j * ldv + i];
+      re -= c[j].re * v.re - c[j].im * v.im;
+      im -= c[j].re * v.im + c[j].im * v.re;
+    }
+    w[i] = {(R)re, (R)im};
+  }
+}
+
+template <typename R>
+__global__ void k_vnormalize(const cplx<R>* w, cplx<R>* v, size_t n, cplx<double>* h2, int k) {
+  __shared__ double s_inv;
+  if (threadIdx.x == 0) {
+    double q = h2[k].re;
+    for (int j = 0; j < k; ++j) q -= h2[j].re * h2[j].re + h2[j].im * h2[j].im;
+    const double hn = q > 0.0 ? sqrt(q) : 0.0;
+    s_inv = hn > 0.0 ? 1.0 / hn : 0.0;
+    if (blockIdx.x == 0) h2[k].im = hn;
+  }
+  __syncthreads();
+  const double a = s_inv;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    v[i] = {(R)(a * w[i].re), (R)(a * w[i].im)};
+}
+
 template <typename R>
 __global__ void k_vmultiaxpy(const cplx<double>* c, const cplx<R>* V, size_t ldv, int k,
                              cplx<R>* w, size_t n) {
@@ -212,12 +276,44 @@ template <typename R> void vmultiaxpy(Blas& b, const cplx<double>* c, const cplx
                                       int k, cplx<R>* w, size_t n) {
   MGC_CHECK(k >= 0 && k <= Blas::KMAX, MGC_ERR_ARG, "too many vectors");
   if (k == 0) return;
-  MGC_HIP(hipMemcpyAsync(b.d_coef, c, (size_t)k * sizeof(cplx<double>), hipMemcpyHostToDevice,
-                         b.s));
-  // the host array may be reused right after: wait for the copy
-  MGC_HIP(hipStreamSynchronize(b.s));
+  // the previous upload must have been read before the staging is rewritten
+  MGC_HIP(hipEventSynchronize(b.coef_done));
+  for (int j = 0; j < k; ++j) b.h_coef[j] = c[j];
+  MGC_HIP(hipMemcpyAsync(b.d_coef, b.h_coef, (size_t)k * sizeof(cplx<double>),
+                         hipMemcpyHostToDevice, b.s));
+  MGC_HIP(hipEventRecord(b.coef_done, b.s));
   k_vmultiaxpy<R><<<ew_grid(n), 256, 0, b.s>>>(b.d_coef, V, ldv, k, w, n);
   MGC_LAUNCH_CHECK();
+}
+
+template <typename R> void vmultidot_dev(Blas& b, const cplx<R>* V, size_t ldv, int k,
+                                         const cplx<R>* w, size_t n, bool with_norm,
+                                         cplx<double>* d_h) {
+  MGC_CHECK(k >= 0 && k <= Blas::KMAX && (k > 0 || with_norm), MGC_ERR_ARG,
+            "too many simultaneous reductions");
+  const int rows = k + (with_norm ? 1 : 0);
+  const int np = nparts_for(n);
+  k_mdot_part<R><<<dim3(np, rows), 256, 0, b.s>>>(V, ldv, k, w, n, b.d_part);
+  MGC_LAUNCH_CHECK();
+  k_dot_final<<<rows, 256, 0, b.s>>>(b.d_part, np, reinterpret_cast<double*>(d_h));
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv,
+                                         int k, cplx<R>* w, size_t n, hipStream_t s) {
+  if (k == 0) return;
+  k_vmultisub<R><<<ew_grid(n), 256, 0, s>>>(d_c, V, ldv, k, w, n);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vnormalize_dev(const cplx<R>* w, cplx<R>* v, size_t n,
+                                          cplx<double>* d_h2, int k, hipStream_t s) {
+  k_vnormalize<R><<<ew_grid(n), 256, 0, s>>>(w, v, n, d_h2, k);
+  MGC_LAUNCH_CHECK();
+}
+void fetch(Blas& b, const double* d_src, int count) {
+  MGC_CHECK(count >= 0 && count <= 4 * (Blas::KMAX + 1) * 2, MGC_ERR_ARG, "fetch too large");
+  MGC_HIP(hipMemcpyAsync(b.h_res, d_src, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
+                         b.s));
+  MGC_HIP(hipStreamSynchronize(b.s));
 }
 
 #define INST(R)                                                                              \
@@ -232,7 +328,13 @@ template <typename R> void vmultiaxpy(Blas& b, const cplx<double>* c, const cplx
   template void vmultidot<R>(Blas&, const cplx<R>*, size_t, int, const cplx<R>*, size_t,     \
                              cplx<double>*);                                                 \
   template void vmultiaxpy<R>(Blas&, const cplx<double>*, const cplx<R>*, size_t, int,       \
-                              cplx<R>*, size_t);
+                              cplx<R>*, size_t);                                             \
+  template void vmultidot_dev<R>(Blas&, const cplx<R>*, size_t, int, const cplx<R>*, size_t, \
+                                 bool, cplx<double>*);                                       \
+  template void vmultisub_dev<R>(const cplx<double>*, const cplx<R>*, size_t, int, cplx<R>*, \
+                                 size_t, hipStream_t);                                       \
+  template void vnormalize_dev<R>(const cplx<R>*, cplx<R>*, size_t, cplx<double>*, int,      \
+                                  hipStream_t);
 INST(double)
 INST(float)
 #undef INST

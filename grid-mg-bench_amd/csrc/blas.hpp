@@ -1,6 +1,7 @@
 // Complex BLAS-1 on device vectors with FP64 reductions, for the Krylov
-// solvers of the multigrid hierarchy.  All calls are stream-ordered; the
-// reductions return to the host (they synchronise the stream).
+// solvers of the multigrid hierarchy.  All calls are stream-ordered.  The
+// host-result reductions (vdot, vnorm2, vmultidot) synchronise the stream;
+// the *_dev forms keep their scalars in device memory and do not.
 #pragma once
 #include "common.hpp"
 
@@ -8,10 +9,12 @@ namespace mgc {
 
 struct Blas {
   hipStream_t s = nullptr;
-  double* d_part = nullptr;  // partial sums [kmax][NPART][2]
+  double* d_part = nullptr;  // partial sums [kmax + 1][NPART][2]
   double* d_res = nullptr;   // final results [kmax][2]
   double* h_res = nullptr;   // pinned host copy
   cplx<double>* d_coef = nullptr;  // coefficients for multi-axpy [kmax]
+  cplx<double>* h_coef = nullptr;  // pinned staging of host coefficients [kmax]
+  hipEvent_t coef_done = nullptr;  // the last upload of h_coef has been read
   static constexpr int NPART = 512;
   static constexpr int KMAX = 64;
   explicit Blas(hipStream_t st);
@@ -41,8 +44,24 @@ template <typename R> double vnorm2(Blas& b, const cplx<R>* x, size_t n);
 // h[j] = <V_j, w> for j < k, V_j = V + j*ldv
 template <typename R> void vmultidot(Blas& b, const cplx<R>* V, size_t ldv, int k,
                                      const cplx<R>* w, size_t n, cplx<double>* h);
-// w += sum_j c[j] V_j  (c on the host)
+// w += sum_j c[j] V_j  (c on the host; uploaded through pinned memory, no sync)
 template <typename R> void vmultiaxpy(Blas& b, const cplx<double>* c, const cplx<R>* V, size_t ldv,
                                       int k, cplx<R>* w, size_t n);
+
+// Device-scalar forms (no host synchronisation):
+// d_h[j] = <V_j, w> for j < k, and, if with_norm, d_h[k].re = |w|^2
+template <typename R> void vmultidot_dev(Blas& b, const cplx<R>* V, size_t ldv, int k,
+                                         const cplx<R>* w, size_t n, bool with_norm,
+                                         cplx<double>* d_h);
+// w -= sum_j d_c[j] V_j
+template <typename R> void vmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv,
+                                         int k, cplx<R>* w, size_t n, hipStream_t s);
+// CGS2 tail: hn = sqrt(d_h2[k].re - sum_{j<k} |d_h2[j]|^2) (the norm of w after
+// the second projection, by Pythagoras from the norm before it), v = w / hn
+// (v = 0 if hn <= 0); hn is stored in d_h2[k].im
+template <typename R> void vnormalize_dev(const cplx<R>* w, cplx<R>* v, size_t n,
+                                          cplx<double>* d_h2, int k, hipStream_t s);
+// copy count doubles from device to the pinned h_res and wait for them
+void fetch(Blas& b, const double* d_src, int count);
 
 }  // namespace mgc

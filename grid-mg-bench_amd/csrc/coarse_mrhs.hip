@@ -164,19 +164,78 @@ __global__ __launch_bounds__(256) void k_quantize(const float2* __restrict__ in,
 // ---------------------------------------------------------------------------
 // the multi-rhs stencil
 // ---------------------------------------------------------------------------
-// Slot s of the staged B' operand (kc = (s/64)/NT, nt = (s/64)%NT, lane = s%64)
-// holds B'[k][n] for k = kc*KC + (lane/16)*KCL + e (e < KCL), n = nt*16 + lane%16,
-// i.e. the KCL/2 complex entries c0.. of input column j = n/2, permuted into
-// (Re, -Im) for n even and (Im, Re) for n odd.  For the half precisions every
-// point's column is pre-multiplied by f(p, j) = s_link(x,p) s_in(y_p, j) / R_j
-// with R_j = max_p s_link s_in (so |f| <= 1 and nothing overflows), and the
-// accumulated tile is multiplied by R_j once at the end: all nine points then
-// accumulate into a single FP32 tile.
-template <int KIND, int N, int NT>
-__global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
-    const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
-    const float* __restrict__ in_scale, float* __restrict__ out, const int* __restrict__ nbr,
-    int nrhs, int rhs0, int nr, int dag) {
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Link fragment loads.  The links are read exactly once, the neighbour
+// inputs by up to nine workgroups, so the link stream should not displace
+// the inputs in L2.  LAUX selects the cache policy of the link loads: 0 = a
+// plain global load; otherwise a buffer load over the site's link block with
+// the policy bits LAUX (2 = nt: non-temporal).
+constexpr int MRHS_BUF_DWORD3 = 0x00020000;  // buffer resource word 3 (raw, 32-bit format)
+
+template <int LAUX>
+struct LinkSrc {
+  const uint4* base;  // plain loads: this wavefront's lane pointer into the site block
+  __amdgpu_buffer_rsrc_t rsrc;
+  __device__ LinkSrc(const uint4* site_block, int bytes, int lane_off) {
+    if constexpr (LAUX == 0) {
+      base = site_block + lane_off;
+    } else {
+      rsrc = __builtin_amdgcn_make_buffer_rsrc(const_cast<uint4*>(site_block), (short)0, bytes,
+                                               MRHS_BUF_DWORD3);
+      base = nullptr;
+    }
+  }
+  // fragment at element offset e (uint4 units) past this lane's position
+  __device__ __forceinline__ uint4 load(int lane_off, int e) const {
+    if constexpr (LAUX == 0) {
+      return base[e];
+    } else {
+      const u32x4 v =
+          __builtin_amdgcn_raw_buffer_load_b128(rsrc, (lane_off + e) * 16, 0, LAUX);
+      return make_uint4(v.x, v.y, v.z, v.w);
+    }
+  }
+};
+
+// One 16-byte B' fragment from the stored input chunk with bit operations
+// only: u = 0 gives (Re, -Im), u = 1 gives (Im, Re) for every complex entry,
+// and neg negates both parts (the G5C sign of Mdag).
+template <int KIND>
+__device__ __forceinline__ uint4 bprime(uint4 r, int u, bool neg) {
+  constexpr unsigned S = 0x80000000u;
+  uint4 o;
+  if constexpr (KIND == MGC_F32) {
+    // words: Re0, Im0, Re1, Im1
+    const unsigned mn = neg ? S : 0u;
+    const unsigned mi = (u ? 0u : S) ^ mn;
+    o.x = (u ? r.y : r.x) ^ mn;
+    o.y = (u ? r.x : r.y) ^ mi;
+    o.z = (u ? r.w : r.z) ^ mn;
+    o.w = (u ? r.z : r.w) ^ mi;
+  } else {
+    // one complex entry per word: Re in the low half, Im in the high half
+    const unsigned m = (u ? 0u : S) ^ (neg ? 0x80008000u : 0u);
+    o.x = (u ? __builtin_rotateright32(r.x, 16) : r.x) ^ m;
+    o.y = (u ? __builtin_rotateright32(r.y, 16) : r.y) ^ m;
+    o.z = (u ? __builtin_rotateright32(r.z, 16) : r.z) ^ m;
+    o.w = (u ? __builtin_rotateright32(r.w, 16) : r.w) ^ m;
+  }
+  return o;
+}
+
+// LDS-staged form for one site x.  Slot s of the staged B' operand
+// (kc = (s/64)/NT, nt = (s/64)%NT, lane = s%64) holds B'[k][n] for
+// k = kc*KC + (lane/16)*KCL + e (e < KCL), n = nt*16 + lane%16: the KCL/2
+// complex entries c0.. of input column j = n/2.  Each point's loads are
+// issued before the previous point's MFMAs.  For FP16/BF16 the point's
+// product T_p is accumulated from zero and folded into the FP32 result as
+// C += s_link(x,p) * s_in(x+d_p, j) * T_p.
+template <int KIND, int N, int NT, int LAUX>
+__device__ __forceinline__ void mrhs_lds_site(
+    const int x, const uint4* __restrict__ A, const float* __restrict__ ascale,
+    const void* __restrict__ in_, const float* __restrict__ in_scale, float* __restrict__ out,
+    const int* __restrict__ nbr, int nrhs, int rhs0, int nr, int dag) {
   using M = MK<KIND>;
   typedef typename M::T T;
   constexpr int NMT = N / 16;
@@ -186,13 +245,13 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
   constexpr int SPT = NSLOT / THREADS;  // staged slots per thread
   constexpr int NCOL = NT * 8;          // rhs columns covered by the tiles
   constexpr bool SCALED = KIND != MGC_F32;
+  static_assert(NCOL <= THREADS, "one thread per column for the scale table");
   __shared__ uint4 sB[NSLOT];
   __shared__ float s_fac[SCALED ? 9 : 1][NCOL];
-  __shared__ float s_R[NCOL];
 
-  const int x = blockIdx.x;
   const int w = threadIdx.x >> 6;
   const int l = threadIdx.x & 63;
+  const int u = l & 1;  // every slot of this thread has lane l, so column parity l%2
   const T* in = reinterpret_cast<const T*>(in_);  // [site][nrhs][N] complex (T pairs)
 
   int ys[9];
@@ -201,39 +260,32 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
   ys[8] = x;
 
   if constexpr (SCALED) {
-    for (int j = threadIdx.x; j < NCOL; j += THREADS) {
-      float v[9], R = 0.f;
+    if (threadIdx.x < NCOL) {
+      const int j = threadIdx.x;
 #pragma unroll
-      for (int p = 0; p < 9; ++p) {
-        v[p] = j < nr ? ascale[(size_t)x * 9 + p] * in_scale[(size_t)ys[p] * nrhs + rhs0 + j] : 0.f;
-        R = fmaxf(R, v[p]);
-      }
-      const float inv = R > 0.f ? 1.f / R : 0.f;
-#pragma unroll
-      for (int p = 0; p < 9; ++p) s_fac[p][j] = v[p] * inv;
-      s_R[j] = R;
+      for (int p = 0; p < 9; ++p)
+        s_fac[p][j] =
+            j < nr ? ascale[(size_t)x * 9 + p] * in_scale[(size_t)ys[p] * nrhs + rhs0 + j] : 0.f;
     }
   }
 
   // per-thread staging geometry (independent of the point)
-  int soff[SPT];  // element offset of the 16-byte source chunk within a site, -1 if padding
-  int scol[SPT];
+  int soff[SPT];   // element offset of the 16-byte source chunk within a site, -1 if padding
+  bool sneg[SPT];  // chunk lies in the lower chirality half (negated by Mdag)
 #pragma unroll
   for (int q = 0; q < SPT; ++q) {
     const int s = threadIdx.x + q * THREADS;
-    const int lane = s & 63, nt = (s >> 6) % NT, kc = (s >> 6) / NT;
-    const int n = nt * 16 + (lane & 15);
-    const int j = n >> 1;
-    const int c0 = (kc * M::KC + (lane >> 4) * M::KCL) >> 1;
+    const int nt = (s >> 6) % NT, kc = (s >> 6) / NT;
+    const int j = (nt * 16 + (l & 15)) >> 1;
+    const int c0 = (kc * M::KC + (l >> 4) * M::KCL) >> 1;
     soff[q] = j < nr ? ((rhs0 + j) * N + c0) * 2 : -1;
-    scol[q] = n;
+    sneg[q] = dag && c0 >= N / 2;
   }
 
-  uint4 a_n[NKC], b_n[SPT];
-  auto issue = [&](int p) {
-    const uint4* Ap = A + ((((size_t)x * 9 + p) * NMT + w) * NKC) * 64 + l;
-#pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) a_n[kc] = Ap[(size_t)kc * 64];
+  const int lane_off = w * NKC * 64 + l;
+  const LinkSrc<LAUX> src(A + (size_t)x * 9 * NMT * NKC * 64, 9 * NMT * NKC * 64 * 16, lane_off);
+  uint4 a[NKC], b_n[SPT];
+  auto issue_b = [&](int p) {
     const T* base = in + (size_t)ys[p] * nrhs * N * 2;
 #pragma unroll
     for (int q = 0; q < SPT; ++q)
@@ -245,60 +297,60 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) C[nt] = {0.f, 0.f, 0.f, 0.f};
 
-  issue(0);
+  // A is single-buffered: the fragment for point p+1 is loaded into a[kc] as
+  // soon as point p's MFMAs have consumed it; B' is staged one point ahead
+#pragma unroll
+  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, kc * 64);
+  issue_b(0);
 #pragma unroll 1
   for (int p = 0; p < 9; ++p) {
-    uint4 a[NKC], b[SPT];
-#pragma unroll
-    for (int kc = 0; kc < NKC; ++kc) a[kc] = a_n[kc];
+    uint4 b[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; ++q) b[q] = b_n[q];
 
     __syncthreads();  // previous point's MFMA readers are done with sB
 #pragma unroll
-    for (int q = 0; q < SPT; ++q) {
-      const int s = threadIdx.x + q * THREADS;
-      const int n = scol[q];
-      const int u = n & 1;
-      const int c0 = (soff[q] >= 0) ? ((soff[q] >> 1) % N) : 0;
-      float f = 1.f;
-      if constexpr (SCALED) f = s_fac[p][n >> 1];
-      T raw[M::KCL], v[M::KCL];
-      __builtin_memcpy(raw, &b[q], 16);
-#pragma unroll
-      for (int h = 0; h < M::KCL / 2; ++h) {
-        float re = (float)raw[2 * h], im = (float)raw[2 * h + 1];
-        const float sg = (dag && (c0 + h) >= N / 2) ? -f : f;
-        re *= sg;
-        im *= sg;
-        v[2 * h] = (T)(u ? im : re);
-        v[2 * h + 1] = (T)(u ? re : -im);
-      }
-      uint4 wv;
-      __builtin_memcpy(&wv, v, 16);
-      sB[s] = wv;
-    }
+    for (int q = 0; q < SPT; ++q) sB[threadIdx.x + q * THREADS] = bprime<KIND>(b[q], u, sneg[q]);
     __syncthreads();
-    if (p < 8) issue(p + 1);  // next point's loads overlap this point's MFMAs
+    if (p < 8) issue_b(p + 1);  // next point's input loads overlap this point's MFMAs
+    const int anext = ((p + 1) * NMT * NKC) * 64;
 
+    if constexpr (SCALED) {
+      floatx4 Tp[NT];
 #pragma unroll
-    for (int kc = 0; kc < NKC; ++kc)
+      for (int nt = 0; nt < NT; ++nt) Tp[nt] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-        C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
+      for (int kc = 0; kc < NKC; ++kc) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          Tp[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], Tp[nt]);
+        if (p < 8) a[kc] = src.load(lane_off, anext + kc * 64);
+      }
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const float f = s_fac[p][(nt * 16 + (l & 15)) >> 1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) C[nt][i] = fmaf(f, Tp[nt][i], C[nt][i]);
+      }
+    } else {
+#pragma unroll
+      for (int kc = 0; kc < NKC; ++kc) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
+        if (p < 8) a[kc] = src.load(lane_off, anext + kc * 64);
+      }
+    }
   }
   // D[m][n]: lane l holds rows 4*(l/16)+i, column l%16
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {
-    const int n = nt * 16 + (l & 15);
-    const int j = n >> 1, u = n & 1;
+    const int j = (nt * 16 + (l & 15)) >> 1;
     if (j < nr) {
-      float R = 1.f;
-      if constexpr (SCALED) R = s_R[j];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int r = w * 16 + 4 * (l >> 4) + i;
-        float v = C[nt][i] * R;
+        float v = C[nt][i];
         if (dag && r >= N / 2) v = -v;
         out[(((size_t)x * nrhs + rhs0 + j) * N + r) * 2 + u] = v;
       }
@@ -306,12 +358,38 @@ __global__ __launch_bounds__(N / 16 * 64) void k_mrhs(
   }
 }
 
-// Register-only variant: no LDS and no barrier.  Each wavefront owns MPW
-// consecutive m-tiles of one site and builds the B' fragments it needs in
-// registers straight from 16-byte loads of the neighbour's input (lanes l and
-// l^1 read the same chunk), so wavefronts stream independently and the
-// hardware overlaps one wavefront's loads with another's MFMAs.  The B'
-// values, scale folding and MFMA order are those of k_mrhs.
+// Workgroup -> site order.  Consecutive workgroups are dispatched to the
+// eight XCDs in turn, each with its own L2.  With xchunk = V/8 > 0, XCD k
+// receives the contiguous site range [k*xchunk, (k+1)*xchunk) in lexicographic
+// order (for a lattice with L3 % 8 == 0 a set of whole time slices), so that
+// most neighbour inputs a workgroup reads were fetched into the same L2 by a
+// workgroup that ran shortly before; xchunk = 0 keeps the identity order.
+__device__ __forceinline__ int site_of_block(int b, int xchunk) {
+  return xchunk > 0 ? (b & 7) * xchunk + (b >> 3) : b;
+}
+
+// Occupancy floor of the LDS-staged kernels: four wavefronts per SIMD (the
+// register budget 128) where the body fits it without spilling, as measured
+// from the compiler's resource report; no floor for the widest tiles.
+template <int KIND, int NT>
+constexpr int mrhs_min_waves() {
+  return (KIND == MGC_F32 ? NT <= 4 : NT <= 6) ? 4 : 1;
+}
+
+template <int KIND, int N, int NT, int LAUX>
+__global__ __launch_bounds__(N / 16 * 64, (mrhs_min_waves<KIND, NT>())) void k_mrhs(
+    const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
+    const float* __restrict__ in_scale, float* __restrict__ out, const int* __restrict__ nbr,
+    int nrhs, int rhs0, int nr, int dag, int xchunk) {
+  mrhs_lds_site<KIND, N, NT, LAUX>(site_of_block((int)blockIdx.x, xchunk), A, ascale, in_,
+                                  in_scale, out, nbr, nrhs, rhs0, nr, dag);
+}
+
+// Register form: no LDS and no barrier.  Each wavefront owns MPW consecutive
+// m-tiles of one site and builds the B' fragments it needs in registers
+// straight from 16-byte loads of the neighbour's input (lanes l and l^1 read
+// the same chunk), so wavefronts stream independently.  B' values, scale
+// folding and MFMA order are those of the LDS-staged form.
 template <int KIND, int N, int NT, int MPW>
 __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
     const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
@@ -325,32 +403,16 @@ __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
   const int x = blockIdx.x;
   const int w = threadIdx.x >> 6;
   const int l = threadIdx.x & 63;
+  const int u = l & 1;
   const T* in = reinterpret_cast<const T*>(in_);
 
   int ys[9];
 #pragma unroll
   for (int p = 0; p < 8; ++p) ys[p] = nbr[(size_t)x * 8 + p];
   ys[8] = x;
-  // this lane's columns: n = nt*16 + l%16, rhs j = n/2, part u = n%2
-  const int u = l & 1;
-  float invR[NT], Rj[NT];
   int jj[NT];
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    const int j = (nt * 16 + (l & 15)) >> 1;
-    jj[nt] = j;
-    float R = 0.f;
-    if constexpr (SCALED) {
-      if (j < nr)
-#pragma unroll
-        for (int p = 0; p < 9; ++p)
-          R = fmaxf(R, ascale[(size_t)x * 9 + p] * in_scale[(size_t)ys[p] * nrhs + rhs0 + j]);
-    } else {
-      R = 1.f;
-    }
-    invR[nt] = R > 0.f ? 1.f / R : 0.f;
-    Rj[nt] = R;
-  }
+  for (int nt = 0; nt < NT; ++nt) jj[nt] = (nt * 16 + (l & 15)) >> 1;
   const int cl = (l >> 4) * M::KCL / 2;  // this lane's first complex within a k-chunk
 
   floatx4 C[MPW][NT];
@@ -363,39 +425,44 @@ __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
   for (int p = 0; p < 9; ++p) {
     const uint4* Ap = A + (((size_t)x * 9 + p) * NMT + w * MPW) * NKC * 64 + l;
     const T* base = in + (size_t)ys[p] * nrhs * N * 2;
-    const float sl = SCALED ? ascale[(size_t)x * 9 + p] : 1.f;
-    float f[NT];
+    floatx4 Tp[MPW][NT];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      f[nt] = 1.f;
-      if constexpr (SCALED)
-        f[nt] = jj[nt] < nr ? sl * in_scale[(size_t)ys[p] * nrhs + rhs0 + jj[nt]] * invR[nt] : 0.f;
-    }
+    for (int mt = 0; mt < MPW; ++mt)
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) Tp[mt][nt] = SCALED ? floatx4{0.f, 0.f, 0.f, 0.f} : C[mt][nt];
 #pragma unroll
     for (int kc = 0; kc < NKC; ++kc) {
       uint4 a[MPW];
 #pragma unroll
       for (int mt = 0; mt < MPW; ++mt) a[mt] = Ap[((size_t)mt * NKC + kc) * 64];
       const int c0 = kc * M::KC / 2 + cl;
+      const bool neg = dag && c0 >= N / 2;
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
-        uint4 raw4 = make_uint4(0u, 0u, 0u, 0u);
+        uint4 raw = make_uint4(0u, 0u, 0u, 0u);
         if (jj[nt] < nr)
-          raw4 = *reinterpret_cast<const uint4*>(base + (((size_t)(rhs0 + jj[nt])) * N + c0) * 2);
-        T raw[M::KCL], v[M::KCL];
-        __builtin_memcpy(raw, &raw4, 16);
+          raw = *reinterpret_cast<const uint4*>(base + (((size_t)(rhs0 + jj[nt])) * N + c0) * 2);
+        const uint4 b = bprime<KIND>(raw, u, neg);
 #pragma unroll
-        for (int h = 0; h < M::KCL / 2; ++h) {
-          const float sg = (dag && (c0 + h) >= N / 2) ? -f[nt] : f[nt];
-          const float re = (float)raw[2 * h] * sg, im = (float)raw[2 * h + 1] * sg;
-          v[2 * h] = (T)(u ? im : re);
-          v[2 * h + 1] = (T)(u ? re : -im);
-        }
-        uint4 b;
-        __builtin_memcpy(&b, v, 16);
-#pragma unroll
-        for (int mt = 0; mt < MPW; ++mt) C[mt][nt] = mfma_chunk<KIND>(a[mt], b, C[mt][nt]);
+        for (int mt = 0; mt < MPW; ++mt) Tp[mt][nt] = mfma_chunk<KIND>(a[mt], b, Tp[mt][nt]);
       }
+    }
+    if constexpr (SCALED) {
+      const float sl = ascale[(size_t)x * 9 + p];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const float f =
+            jj[nt] < nr ? sl * in_scale[(size_t)ys[p] * nrhs + rhs0 + jj[nt]] : 0.f;
+#pragma unroll
+        for (int mt = 0; mt < MPW; ++mt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) C[mt][nt][i] = fmaf(f, Tp[mt][nt][i], C[mt][nt][i]);
+      }
+    } else {
+#pragma unroll
+      for (int mt = 0; mt < MPW; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) C[mt][nt] = Tp[mt][nt];
     }
   }
 #pragma unroll
@@ -404,11 +471,10 @@ __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
     for (int nt = 0; nt < NT; ++nt) {
       const int j = jj[nt];
       if (j < nr) {
-        const float R = Rj[nt];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int r = (w * MPW + mt) * 16 + 4 * (l >> 4) + i;
-          float val = C[mt][nt][i] * R;
+          float val = C[mt][nt][i];
           if (dag && r >= N / 2) val = -val;
           out[(((size_t)x * nrhs + rhs0 + j) * N + r) * 2 + u] = val;
         }
@@ -416,23 +482,291 @@ __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
     }
 }
 
-// kernel variants: 0 = LDS-staged B', 1 = registers with 1 m-tile per
-// wavefront, 2 = registers with 2 m-tiles per wavefront
-constexpr int MRHS_VARIANTS = 3;
+// Several coarse lattices of (possibly) different volume and one nbasis in a
+// single launch: block b belongs to the lattice whose site range
+// [first[l], first[l+1]) contains b.
+constexpr int MRHS_BATCH_MAX = 8;
+struct MrhsBatch {
+  const uint4* A[MRHS_BATCH_MAX];
+  const float* ascale[MRHS_BATCH_MAX];
+  const void* in[MRHS_BATCH_MAX];
+  const float* in_scale[MRHS_BATCH_MAX];
+  float* out[MRHS_BATCH_MAX];
+  const int* nbr[MRHS_BATCH_MAX];
+  int dag[MRHS_BATCH_MAX];
+  int first[MRHS_BATCH_MAX + 1];
+  int count;
+};
+
+template <int KIND, int N, int NT, int LAUX>
+__global__ __launch_bounds__(N / 16 * 64, (mrhs_min_waves<KIND, NT>())) void k_mrhs_batched(MrhsBatch bt, int nrhs, int rhs0,
+                                                              int nr, int xchunk) {
+  const int b = site_of_block((int)blockIdx.x, xchunk);
+  int l = 0;
+  while (l + 1 < bt.count && b >= bt.first[l + 1]) ++l;
+  mrhs_lds_site<KIND, N, NT, LAUX>(b - bt.first[l], bt.A[l], bt.ascale[l], bt.in[l],
+                                  bt.in_scale[l], bt.out[l], bt.nbr[l], nrhs, rhs0, nr,
+                                  bt.dag[l]);
+}
+
+// Persistent form: a grid of about one resident workgroup per slot (CUs x
+// occupancy) walks the sites of every lattice in the batch, iteration
+// it = blockIdx.x + k * gridDim.x.  The loads of the next site's first point
+// are issued during the current site's last point, so a workgroup's memory
+// pipeline does not drain between sites.  Arithmetic per site is that of
+// mrhs_lds_site.
+template <int KIND, int NT>
+constexpr int mrhs_persistent_min_waves() {
+  return KIND == MGC_F32 ? (NT <= 3 ? 3 : 1) : (NT <= 3 ? 4 : NT <= 6 ? 3 : 1);
+}
+
+template <int KIND, int N, int NT, int LAUX>
+__global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>())) void k_mrhs_persistent(
+    MrhsBatch bt, int nrhs, int rhs0, int nr, int xchunk) {
+  using M = MK<KIND>;
+  typedef typename M::T T;
+  constexpr int NMT = N / 16;
+  constexpr int NKC = 2 * N / M::KC;
+  constexpr int THREADS = NMT * 64;
+  constexpr int NSLOT = NKC * NT * 64;
+  constexpr int SPT = NSLOT / THREADS;
+  constexpr int NCOL = NT * 8;
+  constexpr int SITE_FRAGS = 9 * NMT * NKC * 64;  // uint4 fragments of one site's links
+  constexpr bool SCALED = KIND != MGC_F32;
+  static_assert(NCOL <= THREADS, "one thread per column for the scale table");
+  __shared__ uint4 sB[NSLOT];
+  __shared__ float s_fac[2][SCALED ? 9 : 1][NCOL];
+
+  const int total = bt.first[bt.count];
+  int it = blockIdx.x;
+  if (it >= total) return;
+  const int w = threadIdx.x >> 6;
+  const int l = threadIdx.x & 63;
+  const int u = l & 1;
+  const int lane_off = w * NKC * 64 + l;
+
+  auto decode = [&](int i, int& lat, int& x) {
+    const int b = site_of_block(i, xchunk);
+    lat = 0;
+    while (lat + 1 < bt.count && b >= bt.first[lat + 1]) ++lat;
+    x = b - bt.first[lat];
+  };
+  auto load_ys = [&](int lat, int x, int* ys) {
+#pragma unroll
+    for (int p = 0; p < 8; ++p) ys[p] = bt.nbr[lat][(size_t)x * 8 + p];
+    ys[8] = x;
+  };
+  auto fill_fac = [&](int buf, int lat, int x, const int* ys) {
+    if constexpr (SCALED) {
+      if (threadIdx.x < NCOL) {
+        const int j = threadIdx.x;
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+          s_fac[buf][p][j] = j < nr ? bt.ascale[lat][(size_t)x * 9 + p] *
+                                          bt.in_scale[lat][(size_t)ys[p] * nrhs + rhs0 + j]
+                                    : 0.f;
+      }
+    }
+  };
+
+  int soff[SPT];
+  bool shi[SPT];  // chunk in the lower chirality half
+#pragma unroll
+  for (int q = 0; q < SPT; ++q) {
+    const int s = threadIdx.x + q * THREADS;
+    const int nt = (s >> 6) % NT, kc = (s >> 6) / NT;
+    const int j = (nt * 16 + (l & 15)) >> 1;
+    const int c0 = (kc * M::KC + (l >> 4) * M::KCL) >> 1;
+    soff[q] = j < nr ? ((rhs0 + j) * N + c0) * 2 : -1;
+    shi[q] = c0 >= N / 2;
+  }
+  uint4 a[NKC], b_n[SPT];
+  auto issue_b = [&](const T* base) {
+#pragma unroll
+    for (int q = 0; q < SPT; ++q)
+      if (soff[q] >= 0) b_n[q] = *reinterpret_cast<const uint4*>(base + soff[q]);
+      else b_n[q] = make_uint4(0u, 0u, 0u, 0u);
+  };
+
+  int cl, cx;
+  decode(it, cl, cx);
+  int ys[9];
+  load_ys(cl, cx, ys);
+  int buf = 0;
+  fill_fac(0, cl, cx, ys);
+  LinkSrc<LAUX> src(bt.A[cl] + (size_t)cx * SITE_FRAGS, SITE_FRAGS * 16, lane_off);
+#pragma unroll
+  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, kc * 64);
+  const T* cin = reinterpret_cast<const T*>(bt.in[cl]);
+  issue_b(cin + (size_t)ys[0] * nrhs * N * 2);
+
+  floatx4 C[NT];
+  int nl = cl, nx = cx;
+  int nys[9];
+  while (true) {
+    const int nit = it + (int)gridDim.x;
+    const bool has_next = nit < total;
+    const int cdag = bt.dag[cl];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) C[nt] = {0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 1
+    for (int p = 0; p < 9; ++p) {
+      uint4 b[SPT];
+#pragma unroll
+      for (int q = 0; q < SPT; ++q) b[q] = b_n[q];
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < SPT; ++q)
+        sB[threadIdx.x + q * THREADS] = bprime<KIND>(b[q], u, cdag && shi[q]);
+      __syncthreads();
+      // where the next point's operands come from: this site's point p+1, or
+      // the next site's point 0
+      const T* bnext = cin;
+      int ynext = 0, anext = 0;
+      bool more = true;
+      if (p < 8) {
+        ynext = ys[p + 1];
+        anext = (p + 1) * NMT * NKC * 64;
+      } else if (has_next) {
+        decode(nit, nl, nx);
+        load_ys(nl, nx, nys);
+        fill_fac(buf ^ 1, nl, nx, nys);
+        src = LinkSrc<LAUX>(bt.A[nl] + (size_t)nx * SITE_FRAGS, SITE_FRAGS * 16, lane_off);
+        cin = reinterpret_cast<const T*>(bt.in[nl]);
+        bnext = cin;
+        ynext = nys[0];
+      } else {
+        more = false;
+      }
+      if (more) issue_b(bnext + (size_t)ynext * nrhs * N * 2);
+      auto reload = [&](int kc) {
+        if (more) a[kc] = src.load(lane_off, anext + kc * 64);
+      };
+      if constexpr (SCALED) {
+        floatx4 Tp[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) Tp[nt] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt)
+            Tp[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], Tp[nt]);
+          reload(kc);
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const float f = s_fac[buf][p][(nt * 16 + (l & 15)) >> 1];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) C[nt][i] = fmaf(f, Tp[nt][i], C[nt][i]);
+        }
+      } else {
+#pragma unroll
+        for (int kc = 0; kc < NKC; ++kc) {
+#pragma unroll
+          for (int nt = 0; nt < NT; ++nt)
+            C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
+          reload(kc);
+        }
+      }
+    }
+    float* out = bt.out[cl];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int j = (nt * 16 + (l & 15)) >> 1;
+      if (j < nr) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = w * 16 + 4 * (l >> 4) + i;
+          float v = C[nt][i];
+          if (cdag && r >= N / 2) v = -v;
+          out[(((size_t)cx * nrhs + rhs0 + j) * N + r) * 2 + u] = v;
+        }
+      }
+    }
+    if (!has_next) break;
+    it = nit;
+    cl = nl;
+    cx = nx;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) ys[p] = nys[p];
+    buf ^= 1;
+  }
+}
+
+// kernel variants: 0 = LDS-staged B', 1 = LDS-staged B' with non-temporal
+// link loads, 2 = register B' with two m-tiles per wavefront, 3 and 4 =
+// variants 0 and 1 with the XCD-contiguous site order (identity order when
+// the site count is not a multiple of 8), 5 = persistent LDS-staged form,
+// 6 = persistent with non-temporal link loads and the XCD-contiguous order
+constexpr int MRHS_VARIANTS = 7;
+static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6) ? 2 : 0; }
+static int variant_xchunk(int v, int blocks) {
+  return ((v == 3 || v == 4 || v == 6) && blocks % 8 == 0) ? blocks / 8 : 0;
+}
+
+// persistent grid: every resident workgroup slot of the device, a multiple of
+// 8 so that a workgroup keeps its XCD across iterations
+template <typename K>
+static int persistent_grid(K kernel, int threads, int sites) {
+  static int slots = 0;  // per kernel instantiation
+  if (!slots) {
+    int dev = 0, cus = 0, per_cu = 0;
+    MGC_HIP(hipGetDevice(&dev));
+    MGC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    MGC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0));
+    slots = cus * (per_cu > 0 ? per_cu : 1);
+    slots = slots >= 8 ? slots / 8 * 8 : slots;
+  }
+  return sites < slots ? sites : slots;
+}
+
+template <int KIND, int N, int NT>
+static void launch_persistent(int variant, const MrhsBatch& bt, int nrhs, int rhs0, int nr,
+                              hipStream_t s) {
+  const int total = bt.first[bt.count];
+  const int xc = variant_xchunk(variant, total);
+  const int b = N / 16 * 64;
+  if (variant_laux(variant) == 2) {
+    auto k = k_mrhs_persistent<KIND, N, NT, 2>;
+    k<<<persistent_grid(k, b, total), b, 0, s>>>(bt, nrhs, rhs0, nr, xc);
+  } else {
+    auto k = k_mrhs_persistent<KIND, N, NT, 0>;
+    k<<<persistent_grid(k, b, total), b, 0, s>>>(bt, nrhs, rhs0, nr, xc);
+  }
+}
 
 template <int KIND, int N, int NT>
 static void launch_variant(int variant, int V, const uint4* A, const float* as, const void* in,
                            const float* is, float* out, const int* nbr, int nrhs, int rhs0, int nr,
                            int dag, hipStream_t s) {
   const dim3 g(V);
-  if (variant == 0)
-    k_mrhs<KIND, N, NT><<<g, N / 16 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag);
-  else if (variant == 1)
-    k_mrhs_reg<KIND, N, NT, 1><<<g, N / 16 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr,
-                                                        dag);
-  else
+  const int xc = variant_xchunk(variant, V);
+  const int b = N / 16 * 64;
+  if (variant == 5 || variant == 6) {
+    MrhsBatch bt{};
+    bt.A[0] = A;
+    bt.ascale[0] = as;
+    bt.in[0] = in;
+    bt.in_scale[0] = is;
+    bt.out[0] = out;
+    bt.nbr[0] = nbr;
+    bt.dag[0] = dag;
+    bt.first[0] = 0;
+    bt.first[1] = V;
+    bt.count = 1;
+    launch_persistent<KIND, N, NT>(variant, bt, nrhs, rhs0, nr, s);
+    return;
+  }
+  if (variant == 2) {
     k_mrhs_reg<KIND, N, NT, 2><<<g, N / 32 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr,
                                                         dag);
+    return;
+  }
+  if (variant_laux(variant) == 2)
+    k_mrhs<KIND, N, NT, 2><<<g, b, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, xc);
+  else
+    k_mrhs<KIND, N, NT, 0><<<g, b, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, xc);
 }
 
 template <int KIND, int N>
@@ -482,22 +816,17 @@ static void launch_any(int mp, int variant, int N, int NT, int V, const uint4* A
 static int g_mrhs_choice[4][2][9];  // [prec][N==64][NT] -> variant + 1 (0 = untuned)
 static int g_mrhs_forced = -1;      // >= 0: use this variant (set by mgc_mrhs_set_variant)
 
-static int tuned_variant(int mp, int N, int NT, int V, const uint4* A, const float* as,
-                         const void* in, const float* is, float* out, const int* nbr, int nrhs,
-                         int rhs0, int nr, int dag, hipStream_t s) {
-  if (g_mrhs_forced >= 0) return g_mrhs_forced;
-  int& slot = g_mrhs_choice[mp][N == 64 ? 1 : 0][NT];
-  if (slot) return slot - 1;
+template <typename F>
+static int time_variants(int count, F&& launch, hipStream_t s) {
   hipEvent_t e0, e1;
   MGC_HIP(hipEventCreate(&e0));
   MGC_HIP(hipEventCreate(&e1));
   float best = 1e30f;
   int bestv = 0;
-  for (int v = 0; v < MRHS_VARIANTS; ++v) {
-    launch_any(mp, v, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);  // warm-up
+  for (int v = 0; v < count; ++v) {
+    launch(v);  // warm-up
     MGC_HIP(hipEventRecord(e0, s));
-    for (int k = 0; k < 3; ++k)
-      launch_any(mp, v, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    for (int k = 0; k < 3; ++k) launch(v);
     MGC_HIP(hipEventRecord(e1, s));
     MGC_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -509,8 +838,83 @@ static int tuned_variant(int mp, int N, int NT, int V, const uint4* A, const flo
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  slot = bestv + 1;
   return bestv;
+}
+
+static int tuned_variant(int mp, int N, int NT, int V, const uint4* A, const float* as,
+                         const void* in, const float* is, float* out, const int* nbr, int nrhs,
+                         int rhs0, int nr, int dag, hipStream_t s) {
+  if (g_mrhs_forced >= 0) return g_mrhs_forced;
+  int& slot = g_mrhs_choice[mp][N == 64 ? 1 : 0][NT];
+  if (slot) return slot - 1;
+  const int v = time_variants(
+      MRHS_VARIANTS,
+      [&](int v) { launch_any(mp, v, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s); },
+      s);
+  slot = v + 1;
+  return v;
+}
+
+// batched launch over several lattices: the LDS-staged variants
+static const int BATCH_VARIANTS[6] = {0, 1, 3, 4, 5, 6};
+template <int KIND, int N, int NT>
+static void launch_batched_nt(int variant, int blocks, const MrhsBatch& bt, int nrhs, int rhs0,
+                              int nr, hipStream_t s) {
+  const int xc = variant_xchunk(variant, blocks);
+  const int b = N / 16 * 64;
+  if (variant == 5 || variant == 6)
+    launch_persistent<KIND, N, NT>(variant, bt, nrhs, rhs0, nr, s);
+  else if (variant_laux(variant) == 2)
+    k_mrhs_batched<KIND, N, NT, 2><<<blocks, b, 0, s>>>(bt, nrhs, rhs0, nr, xc);
+  else
+    k_mrhs_batched<KIND, N, NT, 0><<<blocks, b, 0, s>>>(bt, nrhs, rhs0, nr, xc);
+}
+template <int KIND, int N>
+static void launch_batched_n(int variant, int NT, int blocks, const MrhsBatch& bt, int nrhs,
+                             int rhs0, int nr, hipStream_t s) {
+  switch (NT) {
+#define CASE(nt)                                                            \
+  case nt:                                                                  \
+    launch_batched_nt<KIND, N, nt>(variant, blocks, bt, nrhs, rhs0, nr, s); \
+    break;
+    CASE(1) CASE(2) CASE(3) CASE(4) CASE(6) CASE(8)
+#undef CASE
+    default:
+      throw Error(MGC_ERR_ARG, "internal: unsupported column tile count");
+  }
+  MGC_LAUNCH_CHECK();
+}
+
+static void launch_batched(int mp, int N, int variant, int NT, int blocks, const MrhsBatch& bt,
+                           int nrhs, int rhs0, int nr, hipStream_t s) {
+#define BY_N(KIND)                                                                      \
+  if (N == 32) launch_batched_n<KIND, 32>(variant, NT, blocks, bt, nrhs, rhs0, nr, s); \
+  else launch_batched_n<KIND, 64>(variant, NT, blocks, bt, nrhs, rhs0, nr, s);
+  if (mp == MGC_F32) {
+    BY_N(MGC_F32)
+  } else if (mp == MGC_F16) {
+    BY_N(MGC_F16)
+  } else {
+    BY_N(MGC_BF16)
+  }
+#undef BY_N
+}
+
+static int g_batch_choice[4][2][9];  // [prec][N==64][NT] -> variant + 1 (0 = untuned)
+
+static int tuned_batched(int mp, int N, int NT, int blocks, const MrhsBatch& bt, int nrhs,
+                         int rhs0, int nr, hipStream_t s) {
+  if (g_mrhs_forced >= 0 && g_mrhs_forced != 2) return g_mrhs_forced;
+  int& slot = g_batch_choice[mp][N == 64 ? 1 : 0][NT];
+  if (slot) return slot - 1;
+  const int v = BATCH_VARIANTS[time_variants(
+      6,
+      [&](int i) {
+        launch_batched(mp, N, BATCH_VARIANTS[i], NT, blocks, bt, nrhs, rhs0, nr, s);
+      },
+      s)];
+  slot = v + 1;
+  return v;
 }
 
 static int tiles_for(int nr) {
@@ -659,6 +1063,52 @@ int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scal
     if (qbuf) {
       MGC_HIP(hipFreeAsync(qbuf, s));
       MGC_HIP(hipFreeAsync(qsc, s));
+    }
+  });
+}
+
+int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const void* const* in,
+                                 const float* const* in_scales, int in_prec, void* const* out,
+                                 int nrhs, int mrhs_prec, int dag, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(ops && in && out, MGC_ERR_ARG, "null argument");
+    MGC_CHECK(count >= 1 && count <= MRHS_BATCH_MAX, MGC_ERR_ARG, "count must be 1..8");
+    MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
+    MGC_CHECK(mrhs_prec == MGC_F32 || mrhs_prec == MGC_F16 || mrhs_prec == MGC_BF16,
+              MGC_ERR_PRECISION, "multi-rhs precision must be F32, F16 or BF16");
+    MGC_CHECK(in_prec == mrhs_prec, MGC_ERR_PRECISION,
+              "batched apply takes vectors already in the multi-rhs precision");
+    hipStream_t s = as_stream(stream);
+    MrhsBatch bt{};
+    bt.count = count;
+    const int N = ops[0] ? ops[0]->N : 0;
+    long long blocks = 0;
+    for (int l = 0; l < count; ++l) {
+      mgc_coarse_op* op = ops[l];
+      MGC_CHECK(op && in[l] && out[l] && in[l] != out[l], MGC_ERR_ARG,
+                "null or aliased argument in the batch");
+      MGC_CHECK(op->links_set, MGC_ERR_STATE, "links not set");
+      MGC_CHECK(op->N == N, MGC_ERR_SHAPE, "every lattice in a batch needs the same nbasis");
+      MGC_CHECK(mrhs_prec == MGC_F32 || (in_scales && in_scales[l]), MGC_ERR_ARG,
+                "half-precision input needs its scales");
+      prepare_mrhs(op, mrhs_prec, s);
+      bt.A[l] = (const uint4*)op->mrhs_links;
+      bt.ascale[l] = op->mrhs_scales;
+      bt.in[l] = in[l];
+      bt.in_scale[l] = mrhs_prec == MGC_F32 ? nullptr : in_scales[l];
+      bt.out[l] = (float*)out[l];
+      bt.nbr[l] = op->nbr_full;
+      bt.dag[l] = (dag && !op->hermitian) ? 1 : 0;
+      bt.first[l] = (int)blocks;
+      blocks += op->V;
+    }
+    MGC_CHECK(blocks <= 0x7fffffffLL, MGC_ERR_SHAPE, "batch too large for one launch");
+    bt.first[count] = (int)blocks;
+    for (int r0 = 0; r0 < nrhs; r0 += 64) {
+      const int nr = nrhs - r0 < 64 ? nrhs - r0 : 64;
+      const int NT = tiles_for(nr);
+      const int v = tuned_batched(mrhs_prec, N, NT, (int)blocks, bt, nrhs, r0, nr, s);
+      launch_batched(mrhs_prec, N, v, NT, (int)blocks, bt, nrhs, r0, nr, s);
     }
   });
 }

@@ -3,6 +3,14 @@
 // multi-dot reductions per step), complex Givens rotations, and the true
 // residual recomputed at every restart.  Templated on the vector precision.
 //
+// The projection coefficients stay in device memory: one Arnoldi step is
+// dot, subtract, dot (+ norm), subtract, normalise, all stream-ordered, and
+// the host reads the step's scalars once (one synchronisation per step) for
+// the Hessenberg column and its Givens rotation.  The new vector's norm comes
+// from Pythagoras, |w''|^2 = |w'|^2 - sum |h2_i|^2 (the second projection
+// only removes round-off); when that cancels by more than a quarter the norm
+// is recomputed directly.
+//
 // Reference use: the FGMRES outer solver and K-cycle wrapper, GMRES smoother
 // and coarsest-level solver of include/MultigridPreconditioner.h:253-309,
 // 622-641 (created by include/Helpers.h:38-56 with the defaults of
@@ -46,9 +54,10 @@ class Gmres {
     if (flexible) MGC_HIP(hipMalloc(&Z_, (size_t)m * n * sizeof(cplx<R>)));
     MGC_HIP(hipMalloc(&w_, n * sizeof(cplx<R>)));
     MGC_HIP(hipMalloc(&r_, n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&d_h_, (size_t)(2 * Blas::KMAX + 3) * sizeof(cplx<double>)));
   }
   ~Gmres() {
-    void* b[] = {V_, Z_, w_, r_};
+    void* b[] = {V_, Z_, w_, r_, d_h_};
     for (void* p : b)
       if (p) (void)hipFree(p);
   }
@@ -80,7 +89,7 @@ class Gmres {
     double beta = std::sqrt(vnorm2<R>(bl, r_, n));
     std::vector<C> H((size_t)(m_ + 1) * m_), g(m_ + 1), cs_s(m_);
     std::vector<double> cs_c(m_);
-    std::vector<cplx<double>> h(m_ + 1), h2(m_ + 1), y(m_);
+    std::vector<cplx<double>> h(m_ + 1), y(m_);
     while (true) {
       st.rel_res = beta / bnorm;
       if (st.rel_res <= tol) {
@@ -103,21 +112,33 @@ class Gmres {
         }
         A.apply(z, w_, s);
         ++st.applies;
-        // CGS2
-        vmultidot<R>(bl, V_, n, j + 1, w_, n, h.data());
-        for (int i = 0; i <= j; ++i) y[i] = {-h[i].re, -h[i].im};
-        vmultiaxpy<R>(bl, y.data(), V_, n, j + 1, w_, n);
-        vmultidot<R>(bl, V_, n, j + 1, w_, n, h2.data());
+        // CGS2 with device-resident coefficients: h1 at d_h_[0..j], h2 and
+        // |w'|^2 at d_h_[j+1 .. 2j+2]
+        cplx<double>* dh1 = d_h_;
+        cplx<double>* dh2 = d_h_ + (j + 1);
+        cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
+        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, false, dh1);
+        vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);
+        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);
+        vmultisub_dev<R>(dh2, V_, n, j + 1, w_, n, s);
+        vnormalize_dev<R>(w_, vnext, n, dh2, j + 1, s);
+        fetch(bl, reinterpret_cast<const double*>(d_h_), 2 * (2 * j + 3));
+        const double* hr = bl.h_res;
+        double sum2 = 0.0;
         for (int i = 0; i <= j; ++i) {
-          y[i] = {-h2[i].re, -h2[i].im};
-          h[i].re += h2[i].re;
-          h[i].im += h2[i].im;
+          const double ar = hr[2 * (j + 1 + i)], ai = hr[2 * (j + 1 + i) + 1];
+          h[i] = {hr[2 * i] + ar, hr[2 * i + 1] + ai};
+          sum2 += ar * ar + ai * ai;
         }
-        vmultiaxpy<R>(bl, y.data(), V_, n, j + 1, w_, n);
-        const double hn = std::sqrt(vnorm2<R>(bl, w_, n));
+        const double wn2 = hr[2 * (2 * j + 2)];
+        double hn = hr[2 * (2 * j + 2) + 1];
+        if (!(hn > 0.0) || sum2 > 0.25 * wn2) {
+          // strong cancellation: the norm of w'' directly
+          hn = std::sqrt(vnorm2<R>(bl, w_, n));
+          if (hn > 0.0) vconvert<R, R>(w_, vnext, n, 1.0 / hn, s);
+        }
         for (int i = 0; i <= j; ++i) H[(size_t)i * m_ + j] = C(h[i].re, h[i].im);
         H[(size_t)(j + 1) * m_ + j] = hn;
-        if (hn > 0.0) vconvert<R, R>(w_, V_ + (size_t)(j + 1) * n, n, 1.0 / hn, s);
         // previous rotations on column j
         for (int i = 0; i < j; ++i) {
           const C a = H[(size_t)i * m_ + j], bb = H[(size_t)(i + 1) * m_ + j];
@@ -171,6 +192,7 @@ class Gmres {
   cplx<R>* Z_ = nullptr;
   cplx<R>* w_ = nullptr;
   cplx<R>* r_ = nullptr;
+  cplx<double>* d_h_ = nullptr;  // device-resident CGS2 coefficients
 };
 
 }  // namespace mgc

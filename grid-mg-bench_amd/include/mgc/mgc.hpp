@@ -259,6 +259,20 @@ class CoarsenedMatrix {
   void* stream_;
 };
 
+// multi-rhs M of several coarse lattices (one nbasis) in one launch; inputs
+// already in mrhs_prec (scales required for F16/BF16, from mgc_quantize)
+inline void M_mrhs_batched(const std::vector<CoarsenedMatrix*>& ops,
+                           const std::vector<const void*>& in,
+                           const std::vector<const float*>& in_scales,
+                           const std::vector<void*>& out, int nrhs, int mrhs_prec, int dag = 0,
+                           void* stream = nullptr) {
+  std::vector<mgc_coarse_op*> h;
+  for (CoarsenedMatrix* op : ops) h.push_back(op->handle());
+  check(mgc_coarse_op_M_mrhs_batched(h.data(), (int)h.size(), in.data(),
+                                     in_scales.empty() ? nullptr : in_scales.data(), mrhs_prec,
+                                     out.data(), nrhs, mrhs_prec, dag, stream));
+}
+
 class MultiGridPreconditioner {
  public:
   MultiGridPreconditioner(WilsonCloverOperator& fine, const mgc_mg_params& params,

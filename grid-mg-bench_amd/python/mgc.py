@@ -94,6 +94,7 @@ _SIGS = {
     "mgc_coarse_op_MooeeInvDag": ([_vp, _vp, _vp, _i, _vp], _i),
     "mgc_coarse_op_prepare_mrhs": ([_vp, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
+    "mgc_coarse_op_M_mrhs_batched": ([_vp, _i, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
     "mgc_mrhs_set_variant": ([_i], _i),
     "mgc_mrhs_get_variant": ([_i, _i, _i, _ip], _i),
@@ -408,6 +409,18 @@ class CoarseOp:
                stream=None):
         call("mgc_coarse_op_M_mrhs", self.handle, _p(i), _p(in_scales), int(in_prec), _p(o),
              int(nrhs), int(mrhs_prec), int(dag), self._s(stream))
+
+
+def M_mrhs_batched(ops, ins, outs, nrhs, mrhs_prec=F32, in_scales=None, dag=0, stream=None):
+    """One launch of the multi-rhs M over several coarse lattices (same nbasis)."""
+    n = len(ops)
+    if not (len(ins) == n and len(outs) == n):
+        raise ValueError("ops, ins and outs must have the same length")
+    arr = ctypes.c_void_p * n
+    sc = arr(*[_p(x) for x in in_scales]) if in_scales is not None else None
+    call("mgc_coarse_op_M_mrhs_batched", arr(*[o.handle for o in ops]), n,
+         arr(*[_p(x) for x in ins]), sc, int(mrhs_prec), arr(*[_p(x) for x in outs]),
+         int(nrhs), int(mrhs_prec), int(dag), stream.handle if stream else None)
 
 
 def quantize(inp, rows, n, prec, out, scales, stream=None):

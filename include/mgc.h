@@ -182,13 +182,27 @@ int mgc_coarse_op_MooeeInvDag(mgc_coarse_op* op, const void* in, void* out, int 
 int mgc_coarse_op_prepare_mrhs(mgc_coarse_op* op, int mrhs_prec, void* stream);
 int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scales, int in_prec,
                          void* out, int nrhs, int mrhs_prec, int dag, void* stream);
+/* The multi-rhs M of up to 8 coarse lattices of any volume and one nbasis in
+ * a single launch (one workgroup per site of every lattice): lattice l maps
+ * in[l] (and in_scales[l] for F16/BF16) to out[l] exactly as
+ * mgc_coarse_op_M_mrhs would.  Inputs must already be in mrhs_prec
+ * (in_prec == mrhs_prec; mgc_quantize makes half-precision ones).  Replaces
+ * one call of the MRHS M per coarse lattice
+ * (include/CoarsenedMatrixUpstreamImprovedDirsaveLutMRHS.h:736-804), so that
+ * the small coarse levels of several solves fill the device together. */
+int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const void* const* in,
+                                 const float* const* in_scales, int in_prec, void* const* out,
+                                 int nrhs, int mrhs_prec, int dag, void* stream);
 /* Kernel-variant control of the multi-rhs stencil.  By default the first
  * call for a (precision, nbasis, column-tile) shape times every variant on
  * its own operands and keeps the fastest (autotuner).  variant >= 0 forces a
- * variant (0 = LDS-staged operand, 1 = register operand with one m-tile per
- * wavefront, 2 = register operand with two m-tiles per wavefront); -1
- * restores autotuning.  get_variant reports the variant in use (-1 = not
- * tuned yet). */
+ * variant (0 = LDS-staged operand, 1 = LDS-staged operand with non-temporal
+ * link loads, 2 = register operand with two m-tiles per wavefront, 3 and 4 =
+ * variants 0 and 1 with sites dealt to the eight XCDs in contiguous ranges,
+ * 5 = persistent workgroups walking the sites, 6 = persistent with
+ * non-temporal link loads and the XCD-contiguous order; the batched apply
+ * uses every variant but 2); -1 restores autotuning.  get_variant reports the
+ * variant in use (-1 = not tuned yet). */
 int mgc_mrhs_set_variant(int variant);
 int mgc_mrhs_get_variant(int mrhs_prec, int nbasis, int nrhs, int* variant);
 /* rows x n complex FP32 -> rows x n complex F16/BF16 with scales[row] =

@@ -56,7 +56,7 @@ def test_mrhs_matches_oracle(gpu, dims, N, nrhs, mp):
         assert max(rel(ref[:, k], got_h[:, k]) for k in range(nrhs)) < TOL[mp](N)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 def test_mrhs_every_kernel_variant(gpu, variant, mp):
     """Each kernel variant the autotuner may pick is checked on its own."""
@@ -129,3 +129,79 @@ def test_mrhs_full_size_cfg3(gpu, nrhs):
         e = max(rel(ref[:, k], got[:, k]) for k in range(nrhs))
         print(f"  prec={mp} max per-rhs rel err {e:.3e}", flush=True)
         assert e < TOL[mp](N)
+
+
+def _batched_case(gpu, dims_list, N, nrhs, mp, seed, herm=None):
+    ops, vs, links_l = [], [], []
+    for i, dims in enumerate(dims_list):
+        links = rand_links(dims, N, seed + 10 * i)
+        op = gpu.CoarseOp(dims, N, gpu.F64, hermitian=(herm[i] if herm else 0))
+        op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+        ops.append(op)
+        links_l.append(links)
+        vs.append(rand_vec(dims, N, seed + 10 * i + 1, nrhs=nrhs))
+    ins, scs = [], []
+    for op, v in zip(ops, vs):
+        din = gpu.DeviceBuffer.from_numpy(v.astype(np.complex64))
+        if mp == F32:
+            ins.append(din)
+        else:
+            q = gpu.DeviceBuffer(op.V * nrhs * N * 4)
+            sc = gpu.DeviceBuffer(op.V * nrhs * 4)
+            gpu.quantize(din, op.V * nrhs, N, mp, q, sc)
+            ins.append(q)
+            scs.append(sc)
+    return ops, vs, links_l, ins, (scs if mp != F32 else None)
+
+
+@pytest.mark.parametrize("mp", [F32, F16, BF16])
+@pytest.mark.parametrize("N,nrhs", [(32, 12), (64, 24), (32, 70)])
+def test_mrhs_batched_lattices(gpu, N, nrhs, mp):
+    """Lattices of uneven volume in one launch: each matches the oracle, and,
+    with an LDS-staged variant forced, matches its own single-lattice launch
+    bit for bit (same kernel body, same accumulation order)."""
+    dims_list = [(2, 2, 2, 2), (4, 2, 2, 2), (2, 4, 2, 4)]
+    print(f"[case] mrhs batched N={N} nrhs={nrhs} prec={mp}", flush=True)
+    ops, vs, links_l, ins, scs = _batched_case(gpu, dims_list, N, nrhs, mp, 700)
+    outs = [gpu.DeviceBuffer(op.V * nrhs * N * 8) for op in ops]
+    for dag in (0, 1):
+        gpu.M_mrhs_batched(ops, ins, outs, nrhs, mrhs_prec=mp, in_scales=scs, dag=dag)
+        gpu.synchronize()
+        for op, v, links, o, dims in zip(ops, vs, links_l, outs, dims_list):
+            got = o.to_numpy(np.complex64, (op.V, nrhs, N)).astype(np.complex128)
+            vq = v.astype(np.complex64).astype(np.complex128)
+            for k in range(nrhs):
+                r = oracle.coarse_apply(dims, N, links, np.ascontiguousarray(vq[:, k]), dag=dag)
+                assert rel(r, got[:, k]) < TOL[mp](N), (dims, dag, k)
+    for variant in (0, 1, 3, 4, 5, 6):  # LDS-staged variants, shared with the batched launch
+        gpu.call("mgc_mrhs_set_variant", variant)
+        try:
+            gpu.M_mrhs_batched(ops, ins, outs, nrhs, mrhs_prec=mp, in_scales=scs)
+            singles = []
+            for i, op in enumerate(ops):
+                so = gpu.DeviceBuffer(op.V * nrhs * N * 8)
+                op.M_mrhs(ins[i], so, nrhs, mrhs_prec=mp, in_prec=mp,
+                          in_scales=(scs[i] if scs else None))
+                singles.append(so)
+            gpu.synchronize()
+            for op, o, so in zip(ops, outs, singles):
+                a = o.to_numpy(np.float32, (op.V * nrhs * N * 2,))
+                b = so.to_numpy(np.float32, (op.V * nrhs * N * 2,))
+                assert np.array_equal(a, b), variant
+        finally:
+            gpu.call("mgc_mrhs_set_variant", -1)
+
+
+def test_mrhs_batched_rejects_bad_batches(gpu):
+    ops, vs, links_l, ins, scs = _batched_case(gpu, [(2, 2, 2, 2)], 32, 4, F32, 720)
+    other = gpu.CoarseOp((2, 2, 2, 2), 64, gpu.F64)
+    other.set_links(gpu.DeviceBuffer.from_numpy(rand_links((2, 2, 2, 2), 64, 721)),
+                    gpu.LAYOUT_FLAT, invert=False)
+    oin = gpu.DeviceBuffer(other.V * 4 * 64 * 8)
+    outs = [gpu.DeviceBuffer(16 * 4 * 64 * 8) for _ in range(2)]
+    with pytest.raises(gpu.MgcError):
+        gpu.M_mrhs_batched([ops[0], other], [ins[0], oin], outs, 4, mrhs_prec=F32)
+    with pytest.raises(gpu.MgcError):
+        gpu.M_mrhs_batched(ops, ins, outs[:1], 4, mrhs_prec=F16)  # half precision needs scales
+    with pytest.raises(gpu.MgcError):
+        gpu.M_mrhs_batched(ops * 9, ins * 9, outs[:1] * 9, 4, mrhs_prec=F32)

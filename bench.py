@@ -253,6 +253,11 @@ def main() -> int:
                     help="fine lattice of the multigrid solve (blocks 4^4 then 4^4)")
     ap.add_argument("--only-headline", action="store_true",
                     help="run only the headline leg (for profiling)")
+    ap.add_argument("--only-mrhs", type=int, default=0, metavar="NRHS",
+                    help="run only the f16 nbasis-64 multi-rhs leg with NRHS right-hand sides "
+                         "and print it (for profiling)")
+    ap.add_argument("--only-mg", action="store_true",
+                    help="run only the multigrid-solve leg and print its result (for profiling)")
     ap.add_argument("--mrhs-variant", type=int, default=-1,
                     help="force one multi-rhs kernel variant instead of autotuning (profiling)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -280,6 +285,17 @@ def main() -> int:
     if args.mrhs_variant >= 0:
         mgc.call("mgc_mrhs_set_variant", args.mrhs_variant)
     stream = mgc.Stream()
+    if args.only_mrhs:
+        res = mrhs_leg(mgc, stream, (8, 8, 8, 8), 64, args.only_mrhs, mgc.F16, args.steps,
+                       args.warmup)
+        print(json.dumps({"mrhs": res}), flush=True)
+        return 0
+    if args.only_mg:
+        d = tuple(int(v) for v in args.mg_dims.split(","))
+        log(f"leg: multigrid solve {d}")
+        print(json.dumps({"mg_solve": mg_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)],
+                                             [16, 16])}), flush=True)
+        return 0
     legs = {}
     dims8 = (8, 8, 8, 8)
     # headline: multi-rhs coarse stencil, 8^4 coarse lattice, nbasis 64, 48 rhs,

@@ -12,6 +12,7 @@
 //                                               inner/outer masks of a point)
 #include "aggregation.hpp"
 
+#include <algorithm>
 #include <vector>
 
 namespace mgc {
@@ -101,6 +102,10 @@ __device__ inline bool site_in_mask(const Lat4& blk, int xb, int kind, int p) {
 // the nb threads of a group read one row's nb basis entries (contiguous) and
 // the G groups consecutive rows, so a wavefront streams the panel coalesced;
 // f(q) is a same-address broadcast load.  Group partial sums meet in LDS.
+// When the coarse lattice has few sites, blockIdx.z splits each panel into
+// gridDim.z row ranges of rows_per rows; the workgroups then write partial
+// sums to part[((xc * gridDim.y + y) * gridDim.z + z)][RC][2][nb], which
+// k_project_reduce adds in a fixed order.
 // ---------------------------------------------------------------------------
 constexpr int RC = 4;  // right-hand sides per workgroup
 
@@ -109,7 +114,8 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ f,
                                                  cplx<R>* __restrict__ c,
                                                  const int* __restrict__ blk_fine, int nb, int Vb,
-                                                 int Ef, int nrhs, Lat4 blk, int kind, int point) {
+                                                 int Ef, int nrhs, Lat4 blk, int kind, int point,
+                                                 int rows_per, cplx<R>* __restrict__ part) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* s_red = reinterpret_cast<cplx<R>*>(smem_raw);  // [G][RC][2][nb]
   const int xc = blockIdx.x;
@@ -126,7 +132,8 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
     const int rows = Vb * Ef;
     const cplx<R>* Ub = U + (size_t)xc * rows * nb;
     const int* bf = blk_fine + (size_t)xc * Vb;
-    for (int q = g; q < rows; q += G) {
+    const int qbeg = blockIdx.z * rows_per, qend = min(rows, qbeg + rows_per);
+    for (int q = qbeg + g; q < qend; q += G) {
       const int xb = q / Ef, e = q - xb * Ef;
       if (kind != 0 && !site_in_mask(blk, xb, kind, point)) continue;
       const int k = e >= half;
@@ -140,32 +147,81 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
           else cmac_conj(acc[r][1], u, x);
         }
     }
-    for (int r = 0; r < nr; ++r)
-      for (int k = 0; k < 2; ++k) s_red[(((size_t)g * RC + r) * 2 + k) * nb + i] = acc[r][k];
+#pragma unroll
+    for (int r = 0; r < RC; ++r)
+      if (r < nr)
+#pragma unroll
+        for (int k = 0; k < 2; ++k) s_red[(((size_t)g * RC + r) * 2 + k) * nb + i] = acc[r][k];
   }
   __syncthreads();
   for (int o = tid; o < nr * 2 * nb; o += blockDim.x) {
     const int r = o / (2 * nb), kn = o - r * 2 * nb;  // kn = k*nb + i
     cplx<R> t = {R(0), R(0)};
     for (int gg = 0; gg < G; ++gg) t = cadd(t, s_red[((size_t)gg * RC + r) * 2 * nb + kn]);
-    c[((size_t)xc * nrhs + r0 + r) * 2 * nb + kn] = t;
+    if (gridDim.z == 1)
+      c[((size_t)xc * nrhs + r0 + r) * 2 * nb + kn] = t;
+    else
+      part[(((size_t)xc * gridDim.y + blockIdx.y) * gridDim.z + blockIdx.z) * RC * 2 * nb + o] = t;
+  }
+}
+
+// c(xc)(y*RC + r)(kn) = sum over the S row ranges of the partial sums.  One
+// workgroup per (xc, y): thread (o, zg) sums z = zg, zg + ZG, ... for output
+// o, and the ZG group sums are added in order in LDS.
+template <typename R>
+__global__ __launch_bounds__(256) void k_project_reduce(const cplx<R>* __restrict__ part,
+                                                        cplx<R>* __restrict__ c, int nchunk,
+                                                        int S, int nrhs, int nb) {
+  extern __shared__ unsigned char smem_raw[];
+  cplx<R>* s_acc = reinterpret_cast<cplx<R>*>(smem_raw);  // [ZG][RC*2*nb]
+  const int no = RC * 2 * nb;
+  const size_t xy = blockIdx.x;
+  const int y = (int)(xy % nchunk), xc = (int)(xy / nchunk);
+  if (no > (int)blockDim.x) {  // wide outputs: one thread per output, z in order
+    for (int oo = threadIdx.x; oo < no; oo += blockDim.x) {
+      cplx<R> t = {R(0), R(0)};
+      for (int z = 0; z < S; ++z) t = cadd(t, part[(xy * S + z) * no + oo]);
+      const int r = oo / (2 * nb), kn = oo - r * 2 * nb;
+      if (y * RC + r < nrhs) c[((size_t)xc * nrhs + y * RC + r) * 2 * nb + kn] = t;
+    }
+    return;
+  }
+  const int ZG = (int)blockDim.x / no;
+  const int o = threadIdx.x % no, zg = threadIdx.x / no;
+  cplx<R> acc = {R(0), R(0)};
+  if (zg < ZG)
+    for (int z = zg; z < S; z += ZG) acc = cadd(acc, part[(xy * S + z) * no + o]);
+  if (zg < ZG) s_acc[zg * no + o] = acc;
+  __syncthreads();
+  for (int oo = threadIdx.x; oo < no; oo += blockDim.x) {
+    cplx<R> t = {R(0), R(0)};
+    for (int g = 0; g < ZG; ++g) t = cadd(t, s_acc[g * no + oo]);
+    const int r = oo / (2 * nb), kn = oo - r * 2 * nb;
+    if (y * RC + r < nrhs) c[((size_t)xc * nrhs + y * RC + r) * 2 * nb + kn] = t;
   }
 }
 
 // ---------------------------------------------------------------------------
 // promote: f(fine(xc,xb))(rhs)(e) = sum_i c(xc)(rhs)(chirality(e), i) U(xc,xb,e,i)
-// W lanes (W = next power of two >= nb, W <= 64) per row: each lane loads one
-// basis entry, multiplies by the LDS-resident coarse coefficient, and the W
-// lanes reduce with cross-lane shuffles.
+// One workgroup per (coarse site, chunk of RC right-hand sides).  The block's
+// basis panel (rows q = (xb, e), nb entries each, row-major) is streamed in
+// tiles of 256/LPR rows: each tile is read with contiguous loads
+// (consecutive threads take consecutive panel entries), stored transposed in
+// LDS ([i][row], one row of padding against bank conflicts), and each group of
+// LPR adjacent lanes then sums one row, nb/LPR entries each, joined by lane
+// exchanges.  LPR = 2, or 4 where a 128-row tile would not fit in 64 KB.
 // ---------------------------------------------------------------------------
-template <typename R>
+template <typename R, int LPR>
 __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ c,
                                                  cplx<R>* __restrict__ f,
                                                  const int* __restrict__ blk_fine, int nb, int Vb,
-                                                 int Ef, int nrhs, int W) {
+                                                 int Ef, int nrhs, int rows_per) {
   extern __shared__ unsigned char smem_raw[];
   cplx<R>* s_c = reinterpret_cast<cplx<R>*>(smem_raw);  // [RC][2][nb]
+  constexpr int PR = 256 / LPR;                         // rows per tile
+  constexpr int LD = PR + 1;
+  cplx<R>* s_u = s_c + RC * 2 * nb;                     // [nb][LD]
   const int xc = blockIdx.x;
   const int r0 = blockIdx.y * RC;
   const int nr = min(RC, nrhs - r0);
@@ -173,34 +229,62 @@ __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
   const int half = Ef / 2;
   for (int o = tid; o < nr * 2 * nb; o += blockDim.x)
     s_c[o] = c[((size_t)xc * nrhs + r0) * 2 * nb + o];
-  __syncthreads();
-  const int lane = tid % W;
-  const int rpp = blockDim.x / W;  // rows per pass
   const int rows = Vb * Ef;
   const cplx<R>* Ub = U + (size_t)xc * rows * nb;
   const int* bf = blk_fine + (size_t)xc * Vb;
-  for (int q0 = 0; q0 < rows; q0 += rpp) {
-    const int q = q0 + tid / W;
-    const bool active = q < rows;
-    const int qq = active ? q : 0;
-    const int xb = qq / Ef, e = qq - xb * Ef;
-    const int k = e >= half;
-    cplx<R> u = {R(0), R(0)};
-    if (active && lane < nb) u = Ub[(size_t)qq * nb + lane];
+  const int row_l = tid / LPR, h = tid % LPR;
+  const int nh = (nb + LPR - 1) / LPR;
+  const int i0 = min(nb, h * nh), i1 = min(nb, i0 + nh);
+  // per-thread walk of the tile load (t = tid + 256 k): row t / nb, entry t % nb
+  const int step_r = 256 / nb, step_i = 256 % nb;
+  // this workgroup's row range (blockIdx.z splits the panel on small lattices)
+  const int qbeg = blockIdx.z * rows_per, qend = min(rows, qbeg + rows_per);
+  for (int q0 = qbeg; q0 < qend; q0 += PR) {
+    const int nrow = min(PR, qend - q0);
+    __syncthreads();  // the previous tile has been consumed (and s_c is visible)
+    {
+      const cplx<R>* src = Ub + (size_t)q0 * nb;  // rows q0 .. q0+nrow-1
+      int rl = tid / nb, i = tid - rl * nb;
+      for (int t = tid; t < nrow * nb; t += 256) {
+        s_u[i * LD + rl] = src[t];
+        rl += step_r;
+        i += step_i;
+        if (i >= nb) {
+          i -= nb;
+          ++rl;
+        }
+      }
+    }
+    __syncthreads();
     cplx<R> acc[RC];
 #pragma unroll
-    for (int r = 0; r < RC; ++r) {
-      acc[r] = {R(0), R(0)};
-      if (r < nr && lane < nb) cmac(acc[r], s_c[(r * 2 + k) * nb + lane], u);
+    for (int r = 0; r < RC; ++r) acc[r] = {R(0), R(0)};
+    const bool active = row_l < nrow;
+    int k = 0;
+    if (active) {
+      const int q = q0 + row_l;
+      k = (q % Ef) >= half;
+      for (int i = i0; i < i1; ++i) {
+        const cplx<R> u = s_u[i * LD + row_l];
+#pragma unroll
+        for (int r = 0; r < RC; ++r)
+          if (r < nr) cmac(acc[r], s_c[(r * 2 + k) * nb + i], u);
+      }
     }
-    for (int off = W / 2; off > 0; off >>= 1)
+#pragma unroll
+    for (int off = 1; off < LPR; off <<= 1)
 #pragma unroll
       for (int r = 0; r < RC; ++r) {
-        acc[r].re += __shfl_xor(acc[r].re, off, W);
-        acc[r].im += __shfl_xor(acc[r].im, off, W);
+        acc[r].re += __shfl_xor(acc[r].re, off);
+        acc[r].im += __shfl_xor(acc[r].im, off);
       }
-    if (active && lane == 0)
-      for (int r = 0; r < nr; ++r) f[((size_t)bf[xb] * nrhs + r0 + r) * Ef + e] = acc[r];
+    if (active && h == 0) {
+      const int q = q0 + row_l;
+      const int xb = q / Ef, e = q - xb * Ef;
+#pragma unroll
+      for (int r = 0; r < RC; ++r)
+        if (r < nr) f[((size_t)bf[xb] * nrhs + r0 + r) * Ef + e] = acc[r];
+    }
   }
 }
 
@@ -307,7 +391,7 @@ __global__ __launch_bounds__(256) void k_check_orthogonal(const cplx<double>* __
 
 static void free_agg(mgc_aggregation* a) {
   if (!a) return;
-  void* bufs[] = {a->basis64, a->basis32, a->blk_fine, a->fine_row};
+  void* bufs[] = {a->basis64, a->basis32, a->blk_fine, a->fine_row, a->part};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;
@@ -318,47 +402,81 @@ static void check_agg(const mgc_aggregation* a) {
   MGC_CHECK(a->basis_set, MGC_ERR_STATE, "basis not set");
 }
 
-static int next_pow2(int n) {
-  int w = 1;
-  while (w < n) w <<= 1;
-  return w;
-}
-
 void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int nrhs,
                   bool project, int kind, int point, hipStream_t s) {
   check_agg(a);
   MGC_CHECK(in && out && in != out, MGC_ERR_ARG, "null or aliased vector");
   MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
-  const dim3 grid(a->Vc, (nrhs + RC - 1) / RC);
+  const int nchunk = (nrhs + RC - 1) / RC;
   const int nb = a->nb;
+  const int rows = a->Vb * a->Ef;
+  // split each panel over S workgroups when the coarse lattice alone gives
+  // fewer than ~1024 workgroups (at least 64 rows per workgroup)
+  int S = 1;
+  if ((size_t)a->Vc * nchunk < 1024) {
+    S = (int)((1024 + (size_t)a->Vc * nchunk - 1) / ((size_t)a->Vc * nchunk));
+    S = std::min(S, std::max(1, rows / 64));
+  }
+  const int rows_per = (rows + S - 1) / S;
+  S = (rows + rows_per - 1) / rows_per;
+  const dim3 grid(a->Vc, nchunk, S);
+  const size_t es = prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>);
   if (project) {
+    MGC_CHECK(prec == MGC_F64 || prec == MGC_F32, MGC_ERR_PRECISION,
+              "project: F64 or F32 vectors");
     const int block = (256 / nb) * nb;
-    const size_t sh = (size_t)(block / nb) * RC * 2 * nb *
-                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
-    if (prec == MGC_F64)
+    const size_t sh = (size_t)(block / nb) * RC * 2 * nb * es;
+    void* part = nullptr;
+    if (S > 1) {
+      const size_t need = (size_t)a->Vc * nchunk * S * RC * 2 * nb * es;
+      if (need > a->part_bytes) {
+        if (a->part) MGC_HIP(hipFree(a->part));
+        a->part = nullptr;
+        a->part_bytes = 0;
+        MGC_HIP(hipMalloc(&a->part, need));
+        a->part_bytes = need;
+      }
+      part = a->part;
+    }
+    const int nred = a->Vc * nchunk;  // reduction workgroups
+    const size_t red_sh = (size_t)256 * es;  // ZG * (RC*2*nb) <= 256 entries
+    if (prec == MGC_F64) {
       k_project<double><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
                                                 (cplx<double>*)out, a->blk_fine, nb, a->Vb, a->Ef,
-                                                nrhs, a->blk, kind, point);
-    else if (prec == MGC_F32)
+                                                nrhs, a->blk, kind, point, rows_per,
+                                                (cplx<double>*)part);
+      if (S > 1)
+        k_project_reduce<double><<<nred, 256, red_sh, s>>>((const cplx<double>*)part,
+                                                            (cplx<double>*)out, nchunk, S, nrhs,
+                                                            nb);
+    } else {
       k_project<float><<<grid, block, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
                                                (cplx<float>*)out, a->blk_fine, nb, a->Vb, a->Ef,
-                                               nrhs, a->blk, kind, point);
-    else
-      throw Error(MGC_ERR_PRECISION, "project: F64 or F32 vectors");
+                                               nrhs, a->blk, kind, point, rows_per,
+                                               (cplx<float>*)part);
+      if (S > 1)
+        k_project_reduce<float><<<nred, 256, red_sh, s>>>((const cplx<float>*)part,
+                                                           (cplx<float>*)out, nchunk, S, nrhs,
+                                                           nb);
+    }
   } else {
-    const int W = next_pow2(nb);
-    const size_t sh = (size_t)RC * 2 * nb *
-                      (prec == MGC_F64 ? sizeof(cplx<double>) : sizeof(cplx<float>));
-    if (prec == MGC_F64)
-      k_promote<double><<<grid, 256, sh, s>>>(a->basis64, (const cplx<double>*)in,
-                                              (cplx<double>*)out, a->blk_fine, nb, a->Vb, a->Ef,
-                                              nrhs, W);
-    else if (prec == MGC_F32)
-      k_promote<float><<<grid, 256, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
-                                             (cplx<float>*)out, a->blk_fine, nb, a->Vb, a->Ef,
-                                             nrhs, W);
-    else
+    auto lds = [&](int lpr) { return ((size_t)RC * 2 * nb + (size_t)nb * (256 / lpr + 1)) * es; };
+    const int lpr = lds(2) <= 65536 ? 2 : 4;
+    const size_t sh = lds(lpr);
+    MGC_CHECK(sh <= 65536, MGC_ERR_SHAPE, "promote: nbasis too large for the LDS tile");
+#define PROMOTE(R, LPR, basis)                                                                \
+  k_promote<R, LPR><<<grid, 256, sh, s>>>(basis, (const cplx<R>*)in, (cplx<R>*)out, a->blk_fine, \
+                                          nb, a->Vb, a->Ef, nrhs, rows_per)
+    if (prec == MGC_F64) {
+      if (lpr == 2) PROMOTE(double, 2, a->basis64);
+      else PROMOTE(double, 4, a->basis64);
+    } else if (prec == MGC_F32) {
+      if (lpr == 2) PROMOTE(float, 2, agg_basis32(a, s));
+      else PROMOTE(float, 4, agg_basis32(a, s));
+    } else {
       throw Error(MGC_ERR_PRECISION, "promote: F64 or F32 vectors");
+    }
+#undef PROMOTE
   }
   MGC_LAUNCH_CHECK();
 }

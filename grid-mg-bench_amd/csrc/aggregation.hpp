@@ -20,6 +20,8 @@ struct mgc_aggregation {
   bool basis32_valid = false;
   int* blk_fine = nullptr;  // [Vc][Vb] fine site of (block, block site)
   int* fine_row = nullptr;  // [Vf] panel row (xc*Vb + xb) of a fine site
+  void* part = nullptr;     // partial sums of split projections (grown on demand)
+  size_t part_bytes = 0;
 };
 
 namespace mgc {

@@ -2,6 +2,8 @@
 // two-stage block reductions: NPART partial sums per result, then one block).
 #include "blas.hpp"
 
+#include <type_traits>
+
 namespace mgc {
 
 Blas::Blas(hipStream_t st) : s(st) {
@@ -132,30 +134,47 @@ __global__ __launch_bounds__(256) void k_dot_final(const double* part, int npart
   }
 }
 
-// partial sums of <V_j, w> for blockIdx.y = j < k, and of |w|^2 for
-// blockIdx.y == k (launched with k + 1 rows when the norm is wanted)
+// Partial sums of <V_j, w> for j < k and, for row j == k when launched with
+// k + 1 rows, of |w|^2.  blockIdx.y covers rows [MD_ROWS*y, MD_ROWS*(y+1)):
+// each thread loads w[i] once for all of its rows, so w is read once per
+// MD_ROWS rows instead of once per row.
+constexpr int MD_ROWS = 8;
 template <typename R>
-__global__ __launch_bounds__(256) void k_mdot_part(const cplx<R>* V, size_t ldv, int k,
+__global__ __launch_bounds__(256) void k_mdot_part(const cplx<R>* V, size_t ldv, int k, int rows,
                                                    const cplx<R>* w, size_t n, double* part) {
-  const int j = blockIdx.y;
-  const bool nrm = j == k;
-  const cplx<R>* v = nrm ? w : V + (size_t)j * ldv;
-  double re = 0.0, im = 0.0;
+  const int j0 = blockIdx.y * MD_ROWS;
+  const int nj = min(MD_ROWS, rows - j0);
+  double re[MD_ROWS], im[MD_ROWS];
+#pragma unroll
+  for (int jj = 0; jj < MD_ROWS; ++jj) re[jj] = im[jj] = 0.0;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
-    const double ar = v[i].re, ai = v[i].im;
-    if (nrm) {
-      re += ar * ar + ai * ai;
-    } else {
-      const double br = w[i].re, bi = w[i].im;
-      re += ar * br + ai * bi;
-      im += ar * bi - ai * br;
+    const double br = w[i].re, bi = w[i].im;
+#pragma unroll
+    for (int jj = 0; jj < MD_ROWS; ++jj) {
+      if (jj < nj) {
+        const int j = j0 + jj;
+        if (j == k) {
+          re[jj] += br * br + bi * bi;
+        } else {
+          const cplx<R> a = V[(size_t)j * ldv + i];
+          const double ar = a.re, ai = a.im;
+          re[jj] += ar * br + ai * bi;
+          im[jj] += ar * bi - ai * br;
+        }
+      }
     }
   }
-  const double2 t = block_sum2(re, im);
-  if (threadIdx.x == 0) {
-    part[((size_t)j * gridDim.x + blockIdx.x) * 2] = t.x;
-    part[((size_t)j * gridDim.x + blockIdx.x) * 2 + 1] = t.y;
+#pragma unroll
+  for (int jj = 0; jj < MD_ROWS; ++jj) {
+    if (jj < nj) {  // uniform over the block
+      const double2 t = block_sum2(re[jj], im[jj]);
+      if (threadIdx.x == 0) {
+        part[((size_t)(j0 + jj) * gridDim.x + blockIdx.x) * 2] = t.x;
+        part[((size_t)(j0 + jj) * gridDim.x + blockIdx.x) * 2 + 1] = t.y;
+      }
+      __syncthreads();  // block_sum2's LDS is reused by the next row
+    }
   }
 }
 
@@ -189,6 +208,84 @@ __global__ void k_vnormalize(const cplx<R>* w, cplx<R>* v, size_t n, cplx<double
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     v[i] = {(R)(a * w[i].re), (R)(a * w[i].im)};
+}
+
+// k_mdot_part for complex FP32 vectors, two elements per 16-byte load (n,
+// ldv even and the pointers 16-byte aligned); the same partial sums per
+// thread in the same element order
+__global__ __launch_bounds__(256) void k_mdot_part_f2(const cplx<float>* V, size_t ldv, int k,
+                                                      int rows, const cplx<float>* w, size_t n,
+                                                      double* part) {
+  const int j0 = blockIdx.y * MD_ROWS;
+  const int nj = min(MD_ROWS, rows - j0);
+  double re[MD_ROWS], im[MD_ROWS];
+#pragma unroll
+  for (int jj = 0; jj < MD_ROWS; ++jj) re[jj] = im[jj] = 0.0;
+  const size_t n2 = n / 2;
+  for (size_t i2 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i2 < n2;
+       i2 += (size_t)gridDim.x * blockDim.x) {
+    const float4 b = reinterpret_cast<const float4*>(w)[i2];
+#pragma unroll
+    for (int jj = 0; jj < MD_ROWS; ++jj) {
+      if (jj < nj) {
+        const int j = j0 + jj;
+        if (j == k) {
+          re[jj] += (double)b.x * b.x + (double)b.y * b.y;
+          re[jj] += (double)b.z * b.z + (double)b.w * b.w;
+        } else {
+          const float4 a = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          re[jj] += (double)a.x * b.x + (double)a.y * b.y;
+          im[jj] += (double)a.x * b.y - (double)a.y * b.x;
+          re[jj] += (double)a.z * b.z + (double)a.w * b.w;
+          im[jj] += (double)a.z * b.w - (double)a.w * b.z;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < MD_ROWS; ++jj) {
+    if (jj < nj) {
+      const double2 t = block_sum2(re[jj], im[jj]);
+      if (threadIdx.x == 0) {
+        part[((size_t)(j0 + jj) * gridDim.x + blockIdx.x) * 2] = t.x;
+        part[((size_t)(j0 + jj) * gridDim.x + blockIdx.x) * 2 + 1] = t.y;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+static bool pairable(const void* a, const void* b, size_t ldv, size_t n) {
+  return n % 2 == 0 && ldv % 2 == 0 && ((uintptr_t)a % 16) == 0 && ((uintptr_t)b % 16) == 0;
+}
+
+// v = (w - sum_j h2[j] V_j) / hn, hn = sqrt(h2[k].re - sum_j |h2[j]|^2)
+template <typename R>
+__global__ __launch_bounds__(256) void k_subnorm(cplx<double>* h2, const cplx<R>* __restrict__ V,
+                                                 size_t ldv, int k, const cplx<R>* __restrict__ w,
+                                                 cplx<R>* __restrict__ v, size_t n) {
+  __shared__ cplx<double> s_h[Blas::KMAX];
+  __shared__ double s_inv;
+  for (int j = threadIdx.x; j < k; j += blockDim.x) s_h[j] = h2[j];
+  if (threadIdx.x == 0) {
+    double q = h2[k].re;
+    for (int j = 0; j < k; ++j) q -= h2[j].re * h2[j].re + h2[j].im * h2[j].im;
+    const double hn = q > 0.0 ? sqrt(q) : 0.0;
+    s_inv = hn > 0.0 ? 1.0 / hn : 0.0;
+    if (blockIdx.x == 0) h2[k].im = hn;  // other blocks read only h2[k].re
+  }
+  __syncthreads();
+  const double a = s_inv;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    double xr = w[i].re, xi = w[i].im;
+    for (int j = 0; j < k; ++j) {
+      const cplx<R> b = V[(size_t)j * ldv + i];
+      xr -= s_h[j].re * b.re - s_h[j].im * b.im;
+      xi -= s_h[j].re * b.im + s_h[j].im * b.re;
+    }
+    v[i] = {(R)(a * xr), (R)(a * xi)};
+  }
 }
 
 template <typename R>
@@ -292,8 +389,16 @@ template <typename R> void vmultidot_dev(Blas& b, const cplx<R>* V, size_t ldv, 
   MGC_CHECK(k >= 0 && k <= Blas::KMAX && (k > 0 || with_norm), MGC_ERR_ARG,
             "too many simultaneous reductions");
   const int rows = k + (with_norm ? 1 : 0);
-  const int np = nparts_for(n);
-  k_mdot_part<R><<<dim3(np, rows), 256, 0, b.s>>>(V, ldv, k, w, n, b.d_part);
+  const dim3 g(nparts_for(std::is_same<R, float>::value ? n / 2 : n), (rows + MD_ROWS - 1) / MD_ROWS);
+  const int np = (int)g.x;
+  if constexpr (std::is_same<R, float>::value) {
+    if (pairable(V, w, ldv, n))
+      k_mdot_part_f2<<<g, 256, 0, b.s>>>(V, ldv, k, rows, w, n, b.d_part);
+    else
+      k_mdot_part<R><<<g, 256, 0, b.s>>>(V, ldv, k, rows, w, n, b.d_part);
+  } else {
+    k_mdot_part<R><<<g, 256, 0, b.s>>>(V, ldv, k, rows, w, n, b.d_part);
+  }
   MGC_LAUNCH_CHECK();
   k_dot_final<<<rows, 256, 0, b.s>>>(b.d_part, np, reinterpret_cast<double*>(d_h));
   MGC_LAUNCH_CHECK();
@@ -307,6 +412,12 @@ template <typename R> void vmultisub_dev(const cplx<double>* d_c, const cplx<R>*
 template <typename R> void vnormalize_dev(const cplx<R>* w, cplx<R>* v, size_t n,
                                           cplx<double>* d_h2, int k, hipStream_t s) {
   k_vnormalize<R><<<ew_grid(n), 256, 0, s>>>(w, v, n, d_h2, k);
+  MGC_LAUNCH_CHECK();
+}
+template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, size_t ldv, int k,
+                                        const cplx<R>* w, cplx<R>* v, size_t n, hipStream_t s) {
+  MGC_CHECK(k >= 1 && k <= Blas::KMAX, MGC_ERR_ARG, "fused CGS2 pass: k out of range");
+  k_subnorm<R><<<ew_grid(n), 256, 0, s>>>(d_h2, V, ldv, k, w, v, n);
   MGC_LAUNCH_CHECK();
 }
 void fetch(Blas& b, const double* d_src, int count) {
@@ -334,7 +445,9 @@ void fetch(Blas& b, const double* d_src, int count) {
   template void vmultisub_dev<R>(const cplx<double>*, const cplx<R>*, size_t, int, cplx<R>*, \
                                  size_t, hipStream_t);                                       \
   template void vnormalize_dev<R>(const cplx<R>*, cplx<R>*, size_t, cplx<double>*, int,      \
-                                  hipStream_t);
+                                  hipStream_t);                                              \
+  template void vsubnorm_dev<R>(cplx<double>*, const cplx<R>*, size_t, int, const cplx<R>*,   \
+                                cplx<R>*, size_t, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

@@ -15,7 +15,7 @@ struct Blas {
   cplx<double>* d_coef = nullptr;  // coefficients for multi-axpy [kmax]
   cplx<double>* h_coef = nullptr;  // pinned staging of host coefficients [kmax]
   hipEvent_t coef_done = nullptr;  // the last upload of h_coef has been read
-  static constexpr int NPART = 512;
+  static constexpr int NPART = 2048;
   static constexpr int KMAX = 64;
   explicit Blas(hipStream_t st);
   ~Blas();
@@ -61,6 +61,11 @@ template <typename R> void vmultisub_dev(const cplx<double>* d_c, const cplx<R>*
 // (v = 0 if hn <= 0); hn is stored in d_h2[k].im
 template <typename R> void vnormalize_dev(const cplx<R>* w, cplx<R>* v, size_t n,
                                           cplx<double>* d_h2, int k, hipStream_t s);
+// Fused CGS2 tail: v = (w - sum_j d_h2[j] V_j) / hn with hn = sqrt(d_h2[k].re - sum_j |d_h2[j]|^2)
+// (Pythagoras, as vnormalize_dev; v = 0 if hn <= 0; hn stored in d_h2[k].im);
+// w is left unchanged
+template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, size_t ldv, int k,
+                                        const cplx<R>* w, cplx<R>* v, size_t n, hipStream_t s);
 // copy count doubles from device to the pinned h_res and wait for them
 void fetch(Blas& b, const double* d_src, int count);
 

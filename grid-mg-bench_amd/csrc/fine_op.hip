@@ -175,13 +175,6 @@ __global__ void k_clover(const cplx<double>* __restrict__ U, Lat4 lat, double ma
         diag[(((size_t)x * 2 + l) * 6 + a) * 6 + b] = D[(6 * l + a) * 12 + 6 * l + b];
 }
 
-template <typename R>
-__global__ void k_convert(const cplx<double>* __restrict__ s, cplx<R>* __restrict__ d, size_t n) {
-  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (; i < n; i += stride) d[i] = {(R)s[i].re, (R)s[i].im};
-}
-
 __global__ void k_antiperiodic(cplx<double>* __restrict__ U, Lat4 lat) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= lat.volume()) return;
@@ -193,9 +186,10 @@ __global__ void k_antiperiodic(cplx<double>* __restrict__ U, Lat4 lat) {
 }
 
 // ---------------------------------------------------------------------------
-// Wilson-clover apply: thread per (site, rhs); vectors [V][nrhs][12]
-// mask bit p < 8 selects hopping point p, bit 8 the diagonal; dag applies
-// g5 D g5 (= D^dag)
+// Wilson-clover apply: thread per (site, rhs); vectors [V][nrhs][12]; links
+// U [4][9][V] and packed hermitian diagonal blocks diag [2][18][V], site index
+// fastest, so a wavefront's link and clover loads are contiguous.  mask bit p < 8 selects
+// hopping point p, bit 8 the diagonal; dag applies g5 D g5 (= D^dag)
 // ---------------------------------------------------------------------------
 template <typename R>
 __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
@@ -219,14 +213,27 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
   if (mask & 0x100u) {
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-      const cplx<R>* D = diag + ((size_t)x * 2 + l) * 36;
+      const cplx<R>* D = diag + (size_t)l * 18 * V + x;  // packed slot s at D[s * V]
       cplx<R> v[6];
 #pragma unroll
       for (int b = 0; b < 6; ++b) v[b] = ld(x, 6 * l + b);
 #pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const cplx<R> dd = D[(size_t)a * V];  // (d_{2a}, d_{2a+1})
+        acc[6 * l + 2 * a].re += dd.re * v[2 * a].re;
+        acc[6 * l + 2 * a].im += dd.re * v[2 * a].im;
+        acc[6 * l + 2 * a + 1].re += dd.im * v[2 * a + 1].re;
+        acc[6 * l + 2 * a + 1].im += dd.im * v[2 * a + 1].im;
+      }
+      int slot = 3;
+#pragma unroll
       for (int a = 0; a < 6; ++a)
 #pragma unroll
-        for (int b = 0; b < 6; ++b) cmac(acc[6 * l + a], D[a * 6 + b], v[b]);
+        for (int b = a + 1; b < 6; ++b, ++slot) {
+          const cplx<R> u = D[(size_t)slot * V];
+          cmac(acc[6 * l + a], u, v[b]);       // upper: D_ab v_b
+          cmac_conj(acc[6 * l + b], u, v[a]);  // lower: conj(D_ab) v_a
+        }
     }
   }
   // Hopping terms with the rank-2 spin projector: for the upper spins s = 0, 1
@@ -252,14 +259,15 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
         h[s][j] = {a.re - t.re, a.im - t.im};
       }
     }
-    const cplx<R>* u = U + ((size_t)mu * V + (p < 4 ? x : y)) * 9;
+    const cplx<R>* u = U + (size_t)mu * 9 * V + (p < 4 ? x : y);  // element k at u[k * V]
     cplx<R> chi[2][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       cplx<R> L[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        L[j] = p < 4 ? u[i * 3 + j] : cplx<R>{u[j * 3 + i].re, -u[j * 3 + i].im};
+        L[j] = p < 4 ? u[(size_t)(i * 3 + j) * V]
+                     : cplx<R>{u[(size_t)(j * 3 + i) * V].re, -u[(size_t)(j * 3 + i) * V].im};
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         cplx<R> a = {R(0), R(0)};
@@ -291,9 +299,56 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
   }
 }
 
+// [4][V][9] -> [4][9][V] and [V][2][36] -> [2][36][V] (with precision change)
+template <typename Rd>
+__global__ void k_gauge_soa(const cplx<double>* __restrict__ g, cplx<Rd>* __restrict__ o, int V) {
+  const size_t n = (size_t)36 * V;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t x = t % V, mk = t / V, mu = mk / 9, k = mk % 9;
+    const cplx<double> v = g[(mu * V + x) * 9 + k];
+    o[t] = {(Rd)v.re, (Rd)v.im};
+  }
+}
+// packed hermitian blocks: [V][2][6][6] -> [2][18][V] (see fine_op.hpp); the
+// upper triangle is kept, the lower one is its conjugate by construction
+// (F = (Q - Q^dag)/(8i) and sigma_{mu nu} are hermitian)
+template <typename Rd>
+__global__ void k_diag_soa(const cplx<double>* __restrict__ d, cplx<Rd>* __restrict__ o, int V) {
+  const size_t n = (size_t)36 * V;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t x = t % V, ls = t / V;
+    const int l = (int)(ls / 18), slot = (int)(ls % 18);
+    const cplx<double>* D = d + (x * 2 + l) * 36;
+    cplx<double> v;
+    if (slot < 3) {
+      v = {D[(2 * slot) * 7].re, D[(2 * slot + 1) * 7].re};
+    } else {
+      int q = slot - 3, a = 0;
+      while (q >= 5 - a) {  // row a holds 5 - a upper entries
+        q -= 5 - a;
+        ++a;
+      }
+      v = D[a * 6 + a + 1 + q];
+    }
+    o[t] = {(Rd)v.re, (Rd)v.im};
+  }
+}
+
+// (re)build the site-fastest copies the apply reads, in precision Rd
+template <typename Rd>
+static void build_soa(mgc_fine_op* op, cplx<Rd>* gs, cplx<Rd>* ds, hipStream_t s) {
+  k_gauge_soa<Rd><<<grid_for((size_t)36 * op->V, 256, 16384), 256, 0, s>>>(op->gauge64, gs,
+                                                                           op->V);
+  k_diag_soa<Rd><<<grid_for((size_t)36 * op->V, 256, 16384), 256, 0, s>>>(op->diag64, ds, op->V);
+  MGC_LAUNCH_CHECK();
+}
+
 static void free_fine(mgc_fine_op* op) {
   if (!op) return;
-  void* bufs[] = {op->gauge64, op->gauge32, op->diag64, op->diag32, op->nbr};
+  void* bufs[] = {op->gauge64, op->diag64, op->gsoa64, op->dsoa64,
+                  op->gsoa32,  op->dsoa32, op->nbr};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete op;
@@ -308,18 +363,15 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
   const size_t n = (size_t)op->V * nrhs;
   const unsigned g = (unsigned)((n + 255) / 256);
   if (prec == MGC_F64) {
-    k_wilson<double><<<g, 256, 0, s>>>(op->gauge64, op->diag64, (const cplx<double>*)in,
+    k_wilson<double><<<g, 256, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in,
                                        (cplx<double>*)out, op->nbr, op->V, nrhs, mask, dag);
   } else if (prec == MGC_F32) {
-    if (!op->gauge32) {
-      MGC_HIP(hipMalloc(&op->gauge32, (size_t)4 * op->V * 9 * sizeof(cplx<float>)));
-      MGC_HIP(hipMalloc(&op->diag32, (size_t)op->V * 72 * sizeof(cplx<float>)));
-      k_convert<float><<<grid_for((size_t)4 * op->V * 9, 256, 16384), 256, 0, s>>>(
-          op->gauge64, op->gauge32, (size_t)4 * op->V * 9);
-      k_convert<float><<<grid_for((size_t)op->V * 72, 256, 16384), 256, 0, s>>>(
-          op->diag64, op->diag32, (size_t)op->V * 72);
+    if (!op->gsoa32) {
+      MGC_HIP(hipMalloc(&op->gsoa32, (size_t)36 * op->V * sizeof(cplx<float>)));
+      MGC_HIP(hipMalloc(&op->dsoa32, (size_t)36 * op->V * sizeof(cplx<float>)));
+      build_soa<float>(op, op->gsoa32, op->dsoa32, s);
     }
-    k_wilson<float><<<g, 256, 0, s>>>(op->gauge32, op->diag32, (const cplx<float>*)in,
+    k_wilson<float><<<g, 256, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in,
                                       (cplx<float>*)out, op->nbr, op->V, nrhs, mask, dag);
   } else {
     throw Error(MGC_ERR_PRECISION, "fine operator: F64 or F32");
@@ -368,6 +420,8 @@ int mgc_fine_op_create(mgc_fine_op** out, const int dims[4], double mass, double
     MGC_HIP(hipMemcpy(op->nbr, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
     MGC_HIP(hipMalloc(&op->gauge64, (size_t)4 * op->V * 9 * sizeof(cplx<double>)));
     MGC_HIP(hipMalloc(&op->diag64, (size_t)op->V * 72 * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&op->gsoa64, (size_t)36 * op->V * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&op->dsoa64, (size_t)36 * op->V * sizeof(cplx<double>)));
     *out = op;
   });
   if (st != MGC_OK) free_fine(op);
@@ -391,13 +445,8 @@ int mgc_fine_op_set_gauge(mgc_fine_op* op, const void* gauge_f64, void* stream) 
     k_clover<<<(op->V + 63) / 64, 64, 0, s>>>(op->gauge64, op->lat, op->mass, op->csw,
                                               op->diag64);
     MGC_LAUNCH_CHECK();
-    if (op->gauge32) {  // refresh the FP32 copies
-      k_convert<float><<<grid_for((size_t)4 * op->V * 9, 256, 16384), 256, 0, s>>>(
-          op->gauge64, op->gauge32, (size_t)4 * op->V * 9);
-      k_convert<float><<<grid_for((size_t)op->V * 72, 256, 16384), 256, 0, s>>>(
-          op->diag64, op->diag32, (size_t)op->V * 72);
-      MGC_LAUNCH_CHECK();
-    }
+    build_soa<double>(op, op->gsoa64, op->dsoa64, s);
+    if (op->gsoa32) build_soa<float>(op, op->gsoa32, op->dsoa32, s);  // refresh the FP32 copies
     op->gauge_set = true;
   });
 }

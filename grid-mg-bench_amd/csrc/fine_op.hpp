@@ -15,9 +15,17 @@ struct mgc_fine_op {
   int antiperiodic_t = 0;
   bool gauge_set = false;
   mgc::cplx<double>* gauge64 = nullptr;  // [4][V][3][3], boundary phase folded in
-  mgc::cplx<float>* gauge32 = nullptr;
   mgc::cplx<double>* diag64 = nullptr;   // [V][2][6][6]
-  mgc::cplx<float>* diag32 = nullptr;
+  // site-fastest copies read by the apply kernel, so that consecutive lanes
+  // (consecutive sites) read consecutive addresses:
+  mgc::cplx<double>* gsoa64 = nullptr;   // [4][9][V]   (mu, row*3+col, site)
+  mgc::cplx<double>* dsoa64 = nullptr;   // [2][18][V]  (chirality, packed slot, site):
+                                         // each 6x6 diagonal block is hermitian; slots
+                                         // 0..2 hold the real diagonal as pairs
+                                         // (d0,d1) (d2,d3) (d4,d5), slots 3..17 the
+                                         // upper entries (a<b) in row order
+  mgc::cplx<float>* gsoa32 = nullptr;
+  mgc::cplx<float>* dsoa32 = nullptr;
   int* nbr = nullptr;                    // [V][8]
 };
 

@@ -4,9 +4,9 @@
 // residual recomputed at every restart.  Templated on the vector precision.
 //
 // The projection coefficients stay in device memory: one Arnoldi step is
-// dot, subtract, dot (+ norm), subtract, normalise, all stream-ordered, and
-// the host reads the step's scalars once (one synchronisation per step) for
-// the Hessenberg column and its Givens rotation.  The new vector's norm comes
+// dot, subtract, dot (+ norm), fused subtract + normalise, all
+// stream-ordered, and the host reads the step's scalars once (one
+// synchronisation per step) for the Hessenberg column and its Givens rotation.  The new vector's norm comes
 // from Pythagoras, |w''|^2 = |w'|^2 - sum |h2_i|^2 (the second projection
 // only removes round-off); when that cancels by more than a quarter the norm
 // is recomputed directly.
@@ -118,10 +118,9 @@ class Gmres {
         cplx<double>* dh2 = d_h_ + (j + 1);
         cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
         vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, false, dh1);
-        vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);
-        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);
-        vmultisub_dev<R>(dh2, V_, n, j + 1, w_, n, s);
-        vnormalize_dev<R>(w_, vnext, n, dh2, j + 1, s);
+        vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);          // w' = w - V h1
+        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);   // h2 = V^H w', |w'|^2
+        vsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, n, s);    // v = (w' - V h2) / hn
         fetch(bl, reinterpret_cast<const double*>(d_h_), 2 * (2 * j + 3));
         const double* hr = bl.h_res;
         double sum2 = 0.0;
@@ -133,7 +132,8 @@ class Gmres {
         const double wn2 = hr[2 * (2 * j + 2)];
         double hn = hr[2 * (2 * j + 2) + 1];
         if (!(hn > 0.0) || sum2 > 0.25 * wn2) {
-          // strong cancellation: the norm of w'' directly
+          // strong cancellation: form w'' = w' - V h2 and take its norm directly
+          vmultisub_dev<R>(dh2, V_, n, j + 1, w_, n, s);
           hn = std::sqrt(vnorm2<R>(bl, w_, n));
           if (hn > 0.0) vconvert<R, R>(w_, vnext, n, 1.0 / hn, s);
         }

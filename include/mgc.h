@@ -12,7 +12,9 @@
  *   - Every call that launches work takes a HIP stream (`void*`, a
  *     hipStream_t; NULL is the null stream) and is asynchronous on it.
  *   - A handle is used by one host thread at a time; distinct handles are
- *     independent.
+ *     independent.  A handle keeps device workspaces (e.g. the partial sums
+ *     of a split projection), so work of one handle issued on different
+ *     streams must not overlap on the device.
  *
  * Layouts (all complex numbers are interleaved re,im)
  *   - Lattice sites are lexicographic with dimension 0 fastest:

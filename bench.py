@@ -92,6 +92,14 @@ def single_rhs_leg(mgc, stream, dims, N, prec, steps, warmup):
     return res
 
 
+# HBM bytes moved per launch by the headline kernel (k_mrhs<F16, N=64, 6 column
+# tiles, non-temporal link loads>, the instantiation behind variants 1 and 4)
+# on the headline workload: two rocprofv3 --pmc runs, FETCH_SIZE then
+# WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy kernel
+# (FETCH x 2.000, WRITE x 1.000); the command lines and readings are in the
+# profile summary for this kernel under profiles/.
+HEADLINE_TRAFFIC_BYTES = {1: 969.9e6, 4: 969.9e6}
+
 MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 1: "k_mrhs (LDS-staged B', non-temporal link loads)",
                 2: "k_mrhs_reg (register B', 2 m-tiles per wavefront)",
@@ -209,6 +217,58 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10):
     return out
 
 
+def mg_batched_leg(mgc, stream, dims, blocks, nbs, nrhs, coarse_prec, tol=1e-10):
+    """The same K-cycle multigrid solve for nrhs Gaussian right-hand sides in
+    lockstep (FP64 outer FGMRES(20), FP32 hierarchy, coarse levels on the
+    multi-rhs MFMA stencil in coarse_prec), next to one single-rhs solve of
+    the first rhs with the same hierarchy."""
+    V = int(np.prod(dims))
+    g = mgc.DeviceBuffer(4 * V * 9 * 16)
+    mgc.random_gauge(g, dims, 0.4, SEED, 21, stream)
+    fop = mgc.FineOp(dims, -0.1, 1.0)
+    fop.set_gauge(g, stream)
+    b = mgc.DeviceBuffer(V * nrhs * 12 * 16)
+    mgc.random_gaussian(b, V * nrhs * 12, mgc.F64, SEED, 23, stream)
+    x = mgc.DeviceBuffer(V * nrhs * 12 * 16)
+    stream.sync()
+    names = {mgc.F32: "f32", mgc.F16: "f16", mgc.BF16: "bf16"}
+    p = mgc.mg_params(nlevels=len(blocks) + 1, block=blocks, nb=nbs, precision=mgc.F32)
+    mg = mgc.Multigrid(fop, p)
+    t0 = time.perf_counter()
+    mg.setup(stream)
+    t_setup = time.perf_counter() - t0
+    sts = mg.solve_mrhs(b, x, nrhs, coarse_prec=coarse_prec, tol=tol, maxit=500, restart=20)
+    its = [st["iterations"] for st in sts]
+    res = [st["true_residual"] for st in sts]
+    # one rhs alone with the same hierarchy (single-rhs kernels throughout)
+    b1 = mgc.DeviceBuffer(V * 12 * 16)
+    mgc.random_gaussian(b1, V * 12, mgc.F64, SEED, 22, stream)
+    x1 = mgc.DeviceBuffer(V * 12 * 16)
+    st1 = mg.solve(b1, x1, tol=tol, maxit=500, restart=20)
+    mg.close()
+    for buf in (g, b, x, b1, x1):
+        buf.close()
+    fop.close()
+    lat = [tuple(dims)]
+    for blk in blocks:
+        lat.append(tuple(a // b for a, b in zip(lat[-1], blk)))
+    return {
+        "workload": "Wilson-clover " + " -> ".join("x".join(map(str, d)) for d in lat)
+                    + f", nbasis {'/'.join(str(2 * n) for n in nbs)}, {nrhs} rhs in lockstep, "
+                      f"coarse levels {names[coarse_prec]} MFMA, FGMRES(20) to {tol:g}",
+        "nrhs": nrhs,
+        "setup_s": t_setup,
+        "solve_s": sts[0]["seconds"],
+        "solve_s_per_rhs": sts[0]["seconds"] / nrhs,
+        "outer_iterations_per_rhs": its,
+        "true_residual_max": max(res),
+        "all_converged": all(st["converged"] for st in sts),
+        "single_rhs_solve_s": st1["seconds"],
+        "single_rhs_outer_iterations": st1["iterations"],
+        "speedup_per_rhs_vs_single": st1["seconds"] / (sts[0]["seconds"] / nrhs),
+    }
+
+
 def cpu_baseline(dims, N, nrhs, head):
     """FP64 oracle multi-rhs coarse M (one OpenMP-parallel site loop, dense
     N x N matvec per point and rhs) on the host cores, on the headline's own
@@ -256,6 +316,11 @@ def main() -> int:
     ap.add_argument("--only-mrhs", type=int, default=0, metavar="NRHS",
                     help="run only the f16 nbasis-64 multi-rhs leg with NRHS right-hand sides "
                          "and print it (for profiling)")
+    ap.add_argument("--mg-batch", default="32,32,32,32:12",
+                    help="fine lattice and rhs count of the batched multigrid solve "
+                         "(DIMS:NRHS; empty to skip)")
+    ap.add_argument("--only-mg-batch", action="store_true",
+                    help="run only the batched multigrid-solve leg and print it (for profiling)")
     ap.add_argument("--only-mg", action="store_true",
                     help="run only the multigrid-solve leg and print its result (for profiling)")
     ap.add_argument("--mrhs-variant", type=int, default=-1,
@@ -289,6 +354,15 @@ def main() -> int:
         res = mrhs_leg(mgc, stream, (8, 8, 8, 8), 64, args.only_mrhs, mgc.F16, args.steps,
                        args.warmup)
         print(json.dumps({"mrhs": res}), flush=True)
+        return 0
+    def batch_leg():
+        dstr, nstr = args.mg_batch.split(":")
+        d = tuple(int(v) for v in dstr.split(","))
+        log(f"leg: batched multigrid solve {d} nrhs={nstr}")
+        return mg_batched_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)], [16, 16], int(nstr),
+                              mgc.F16)
+    if args.only_mg_batch:
+        print(json.dumps({"mg_batched": batch_leg()}), flush=True)
         return 0
     if args.only_mg:
         d = tuple(int(v) for v in args.mg_dims.split(","))
@@ -330,6 +404,8 @@ def main() -> int:
         d = tuple(int(v) for v in args.mg_dims.split(","))
         log(f"leg: multigrid solve {d}")
         mg = mg_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)], [16, 16])
+        if args.mg_batch:
+            mg["batched"] = batch_leg()
     ref64 = legs.get("M_1rhs_f64_n64")
     line = {
         "metric": METRIC,
@@ -353,7 +429,7 @@ def main() -> int:
             "peak": HBM_PEAK_GBS if head["bound"] == "hbm" else PEAK_TFLOPS["f16"],
             "unit": "GB/s" if head["bound"] == "hbm" else "TFLOP/s",
             "frac": head["hbm_frac"] if head["bound"] == "hbm" else head["mfma_frac"],
-            "traffic": None,
+            "traffic": HEADLINE_TRAFFIC_BYTES.get(head["kernel_variant"]),
             "kernel": head["kernel_name"],
             "bytes_per_launch": head["bytes_per_launch"],
             "flops_per_launch": head["flops_per_launch"],

@@ -1,0 +1,302 @@
+// Batched complex BLAS-1 with per-rhs FP64 reductions (see bblas.hpp).
+#include "bblas.hpp"
+
+#include <algorithm>
+#include <vector>
+
+namespace mgc {
+
+BBlas::BBlas(hipStream_t st) : s(st) {
+  MGC_HIP(hipHostMalloc(&h_res, (size_t)(2 * KMAX + 3) * RMAX * 2 * sizeof(double),
+                        hipHostMallocDefault));
+  MGC_HIP(hipMalloc(&d_coef, (size_t)(KMAX + 1) * RMAX * sizeof(cplx<double>)));
+  MGC_HIP(hipHostMalloc(&h_coef, (size_t)(KMAX + 1) * RMAX * sizeof(cplx<double>),
+                        hipHostMallocDefault));
+  MGC_HIP(hipEventCreateWithFlags(&coef_done, hipEventDisableTiming));
+}
+BBlas::~BBlas() {
+  if (d_part) (void)hipFree(d_part);
+  if (h_res) (void)hipHostFree(h_res);
+  if (d_coef) (void)hipFree(d_coef);
+  if (h_coef) (void)hipHostFree(h_coef);
+  if (coef_done) (void)hipEventDestroy(coef_done);
+}
+
+static unsigned bew_grid(size_t n) { return grid_for(n, 256, 8192); }
+
+static size_t gcd_sz(size_t a, size_t b) {
+  while (b) {
+    const size_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// reduction grid: a multiple of cols/gcd(256, cols) blocks, so that the total
+// thread count is a multiple of cols and each thread keeps one rhs
+static int bred_blocks(const BShape& sh) {
+  const size_t C = (size_t)sh.cols();
+  const size_t base = C / gcd_sz(256, C);  // blocks per lcm(256, C) threads
+  size_t want = std::min<size_t>(2048, (sh.n + 255) / 256);
+  want = std::max<size_t>(want, base);
+  return (int)((want + base - 1) / base * base);
+}
+
+constexpr int BD_ROWS = 8;
+
+// rows [BD_ROWS*y, ...): <V_j, w>_r for j < k, |w|^2_r for j == k
+template <typename R>
+__global__ __launch_bounds__(256) void k_bdot_part(const cplx<R>* __restrict__ V, size_t ldv,
+                                                   int k, int rows,
+                                                   const cplx<R>* __restrict__ w, size_t n,
+                                                   int C, int dof, int nrhs,
+                                                   double* __restrict__ part) {
+  __shared__ double sm[BD_ROWS][256][2];
+  const int j0 = blockIdx.y * BD_ROWS;
+  const int nj = min(BD_ROWS, rows - j0);
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double re[BD_ROWS], im[BD_ROWS];
+#pragma unroll
+  for (int jj = 0; jj < BD_ROWS; ++jj) re[jj] = im[jj] = 0.0;
+  for (size_t i = g; i < n; i += T) {
+    const double br = w[i].re, bi = w[i].im;
+#pragma unroll
+    for (int jj = 0; jj < BD_ROWS; ++jj) {
+      if (jj < nj) {
+        const int j = j0 + jj;
+        if (j == k) {
+          re[jj] += br * br + bi * bi;
+        } else {
+          const cplx<R> a = V[(size_t)j * ldv + i];
+          const double ar = a.re, ai = a.im;
+          re[jj] += ar * br + ai * bi;
+          im[jj] += ar * bi - ai * br;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < BD_ROWS; ++jj) {
+    sm[jj][threadIdx.x][0] = re[jj];
+    sm[jj][threadIdx.x][1] = im[jj];
+  }
+  __syncthreads();
+  // per-rhs sums of this block, threads in ascending order
+  for (int q = threadIdx.x; q < nj * nrhs; q += blockDim.x) {
+    const int jj = q / nrhs, rr = q % nrhs;
+    double sr = 0.0, si = 0.0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
+      if ((int)((gt % C) / dof) == rr) {
+        sr += sm[jj][t][0];
+        si += sm[jj][t][1];
+      }
+    }
+    double* o = part + ((((size_t)(j0 + jj) * nrhs + rr) * gridDim.x) + blockIdx.x) * 2;
+    o[0] = sr;
+    o[1] = si;
+  }
+}
+
+// res[q] = sum over nblk partials of part[q][*], q = row*nrhs + r
+__global__ __launch_bounds__(256) void k_bdot_final(const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ res) {
+  __shared__ double sh[2][256];
+  const int q = blockIdx.x;
+  double re = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) {
+    re += part[((size_t)q * nblk + i) * 2];
+    im += part[((size_t)q * nblk + i) * 2 + 1];
+  }
+  sh[0][threadIdx.x] = re;
+  sh[1][threadIdx.x] = im;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      sh[0][threadIdx.x] += sh[0][threadIdx.x + off];
+      sh[1][threadIdx.x] += sh[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[2 * q] = sh[0][0];
+    res[2 * q + 1] = sh[1][0];
+  }
+}
+
+template <typename R>
+__global__ void k_bmultisub(const cplx<double>* __restrict__ c, const cplx<R>* __restrict__ V,
+                            size_t ldv, int k, cplx<R>* __restrict__ w, size_t n, int C, int dof,
+                            int nrhs, double sign) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)((i % C) / dof);
+    double re = w[i].re, im = w[i].im;
+    for (int j = 0; j < k; ++j) {
+      const cplx<double> cj = c[(size_t)j * nrhs + r];
+      const cplx<R> v = V[(size_t)j * ldv + i];
+      re += sign * (cj.re * v.re - cj.im * v.im);
+      im += sign * (cj.re * v.im + cj.im * v.re);
+    }
+    w[i] = {(R)re, (R)im};
+  }
+}
+
+template <typename R>
+__global__ void k_bsubnorm(cplx<double>* h2, const cplx<R>* __restrict__ V, size_t ldv, int k,
+                           const cplx<R>* __restrict__ w, cplx<R>* __restrict__ v, size_t n,
+                           int C, int dof, int nrhs) {
+  __shared__ double s_inv[BBlas::RMAX];
+  for (int r = threadIdx.x; r < nrhs; r += blockDim.x) {
+    double q = h2[(size_t)k * nrhs + r].re;
+    for (int j = 0; j < k; ++j) {
+      const cplx<double> h = h2[(size_t)j * nrhs + r];
+      q -= h.re * h.re + h.im * h.im;
+    }
+    const double hn = q > 0.0 ? sqrt(q) : 0.0;
+    s_inv[r] = hn > 0.0 ? 1.0 / hn : 0.0;
+    if (blockIdx.x == 0) h2[(size_t)k * nrhs + r].im = hn;  // other blocks read only .re
+  }
+  __syncthreads();
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const int r = (int)((i % C) / dof);
+    double re = w[i].re, im = w[i].im;
+    for (int j = 0; j < k; ++j) {
+      const cplx<double> cj = h2[(size_t)j * nrhs + r];
+      const cplx<R> b = V[(size_t)j * ldv + i];
+      re -= cj.re * b.re - cj.im * b.im;
+      im -= cj.re * b.im + cj.im * b.re;
+    }
+    const double a = s_inv[r];
+    v[i] = {(R)(a * re), (R)(a * im)};
+  }
+}
+
+template <typename Rs, typename Rd>
+__global__ void k_bconvert(const cplx<Rs>* __restrict__ x, cplx<Rd>* __restrict__ y, size_t n,
+                           int C, int dof, const cplx<double>* __restrict__ a) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double s = a[(i % C) / dof].re;
+    y[i] = {(Rd)((double)x[i].re * s), (Rd)((double)x[i].im * s)};
+  }
+}
+
+static void check_shape(const BShape& sh) {
+  MGC_CHECK(sh.nrhs >= 1 && sh.nrhs <= BBlas::RMAX, MGC_ERR_ARG, "batch of 1..64 rhs");
+  MGC_CHECK(sh.dof >= 1 && sh.n % (size_t)sh.cols() == 0, MGC_ERR_SHAPE,
+            "batched vector length is not sites x nrhs x dof");
+}
+
+template <typename R>
+void bdots_dev(BBlas& b, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
+               const BShape& sh, bool with_norm, cplx<double>* d_out) {
+  check_shape(sh);
+  MGC_CHECK(k >= 0 && k <= BBlas::KMAX && (k > 0 || with_norm), MGC_ERR_ARG,
+            "batched reduction: bad row count");
+  const int rows = k + (with_norm ? 1 : 0);
+  const int nblk = bred_blocks(sh);
+  const size_t need = (size_t)rows * sh.nrhs * nblk * 2;
+  if (need > b.part_cap) {
+    if (b.d_part) MGC_HIP(hipFree(b.d_part));
+    b.d_part = nullptr;
+    b.part_cap = 0;
+    const size_t cap = std::max(need, (size_t)(BBlas::KMAX + 1) * sh.nrhs * nblk * 2);
+    MGC_HIP(hipMalloc(&b.d_part, cap * sizeof(double)));
+    b.part_cap = cap;
+  }
+  k_bdot_part<R><<<dim3(nblk, (rows + BD_ROWS - 1) / BD_ROWS), 256, 0, b.s>>>(
+      V, ldv, k, rows, w, sh.n, sh.cols(), sh.dof, sh.nrhs, b.d_part);
+  MGC_LAUNCH_CHECK();
+  k_bdot_final<<<rows * sh.nrhs, 256, 0, b.s>>>(b.d_part, nblk, reinterpret_cast<double*>(d_out));
+  MGC_LAUNCH_CHECK();
+}
+
+void bfetch(BBlas& b, const double* d_src, int count) {
+  MGC_CHECK(count >= 0 && count <= (2 * BBlas::KMAX + 3) * BBlas::RMAX * 2, MGC_ERR_ARG,
+            "fetch too large");
+  MGC_HIP(hipMemcpyAsync(b.h_res, d_src, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
+                         b.s));
+  MGC_HIP(hipStreamSynchronize(b.s));
+}
+
+template <typename R> void bnorm2(BBlas& b, const cplx<R>* x, const BShape& sh, double* out) {
+  bdots_dev<R>(b, x, 0, 0, x, sh, true, b.d_coef + (size_t)BBlas::KMAX * BBlas::RMAX);
+  bfetch(b, reinterpret_cast<const double*>(b.d_coef + (size_t)BBlas::KMAX * BBlas::RMAX),
+         2 * sh.nrhs);
+  for (int r = 0; r < sh.nrhs; ++r) out[r] = b.h_res[2 * r];
+}
+
+template <typename R>
+void bmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k, cplx<R>* w,
+                   const BShape& sh, hipStream_t s) {
+  check_shape(sh);
+  if (k == 0) return;
+  k_bmultisub<R><<<bew_grid(sh.n), 256, 0, s>>>(d_c, V, ldv, k, w, sh.n, sh.cols(), sh.dof,
+                                                sh.nrhs, -1.0);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bmultiaxpy(BBlas& b, const cplx<double>* c, const cplx<R>* V, size_t ldv, int k,
+                cplx<R>* w, const BShape& sh) {
+  check_shape(sh);
+  MGC_CHECK(k >= 0 && k <= BBlas::KMAX, MGC_ERR_ARG, "too many vectors");
+  if (k == 0) return;
+  MGC_HIP(hipEventSynchronize(b.coef_done));
+  std::copy(c, c + (size_t)k * sh.nrhs, b.h_coef);
+  MGC_HIP(hipMemcpyAsync(b.d_coef, b.h_coef, (size_t)k * sh.nrhs * sizeof(cplx<double>),
+                         hipMemcpyHostToDevice, b.s));
+  MGC_HIP(hipEventRecord(b.coef_done, b.s));
+  k_bmultisub<R><<<bew_grid(sh.n), 256, 0, b.s>>>(b.d_coef, V, ldv, k, w, sh.n, sh.cols(),
+                                                  sh.dof, sh.nrhs, 1.0);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
+                  cplx<R>* v, const BShape& sh, hipStream_t s) {
+  check_shape(sh);
+  k_bsubnorm<R><<<bew_grid(sh.n), 256, 0, s>>>(d_h2, V, ldv, k, w, v, sh.n, sh.cols(), sh.dof,
+                                               sh.nrhs);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename Rs, typename Rd>
+void bconvert(BBlas& b, const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const double* a) {
+  check_shape(sh);
+  MGC_HIP(hipEventSynchronize(b.coef_done));
+  for (int r = 0; r < sh.nrhs; ++r) b.h_coef[r] = {a[r], 0.0};
+  MGC_HIP(hipMemcpyAsync(b.d_coef, b.h_coef, (size_t)sh.nrhs * sizeof(cplx<double>),
+                         hipMemcpyHostToDevice, b.s));
+  MGC_HIP(hipEventRecord(b.coef_done, b.s));
+  k_bconvert<Rs, Rd><<<bew_grid(sh.n), 256, 0, b.s>>>(x, y, sh.n, sh.cols(), sh.dof, b.d_coef);
+  MGC_LAUNCH_CHECK();
+}
+
+#define INST(R)                                                                               \
+  template void bdots_dev<R>(BBlas&, const cplx<R>*, size_t, int, const cplx<R>*,             \
+                             const BShape&, bool, cplx<double>*);                             \
+  template void bnorm2<R>(BBlas&, const cplx<R>*, const BShape&, double*);                    \
+  template void bmultisub_dev<R>(const cplx<double>*, const cplx<R>*, size_t, int, cplx<R>*,  \
+                                 const BShape&, hipStream_t);                                 \
+  template void bmultiaxpy<R>(BBlas&, const cplx<double>*, const cplx<R>*, size_t, int,       \
+                              cplx<R>*, const BShape&);                                       \
+  template void bsubnorm_dev<R>(cplx<double>*, const cplx<R>*, size_t, int, const cplx<R>*,   \
+                                cplx<R>*, const BShape&, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+template void bconvert<double, float>(BBlas&, const cplx<double>*, cplx<float>*, const BShape&,
+                                      const double*);
+template void bconvert<float, double>(BBlas&, const cplx<float>*, cplx<double>*, const BShape&,
+                                      const double*);
+template void bconvert<double, double>(BBlas&, const cplx<double>*, cplx<double>*,
+                                       const BShape&, const double*);
+template void bconvert<float, float>(BBlas&, const cplx<float>*, cplx<float>*, const BShape&,
+                                     const double*);
+
+}  // namespace mgc

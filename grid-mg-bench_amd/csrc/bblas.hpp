@@ -1,0 +1,64 @@
+// Batched complex BLAS-1 for nrhs independent systems solved in lockstep.
+// A batched vector is [site][rhs][dof] (the multi-rhs layout of the ABI):
+// element i belongs to right-hand side r(i) = (i % (nrhs*dof)) / dof.  Every
+// reduction returns one FP64 result per rhs; coefficient arrays hold one
+// entry per rhs ([j][nrhs] for the multi-vector forms).
+//
+// The reduction kernels launch a multiple of nrhs*dof threads, so each thread
+// only ever touches elements of one rhs; a workgroup's partial sums are added
+// per rhs in LDS in a fixed order, and the per-workgroup partials in a second
+// fixed-order pass, so results are reproducible run to run.
+#pragma once
+#include "blas.hpp"
+
+namespace mgc {
+
+struct BShape {
+  size_t n = 0;  // complex elements of one batched vector (sites * nrhs * dof)
+  int nrhs = 1;
+  int dof = 1;
+  int cols() const { return nrhs * dof; }
+};
+
+struct BBlas {
+  hipStream_t s = nullptr;
+  double* d_part = nullptr;  // [rows][nrhs][nblk][2]
+  size_t part_cap = 0;       // doubles
+  double* h_res = nullptr;   // pinned, [rows][nrhs][2]
+  cplx<double>* d_coef = nullptr;  // device coefficients [KMAX+1][nrhs]
+  cplx<double>* h_coef = nullptr;  // pinned staging of host coefficients
+  hipEvent_t coef_done = nullptr;
+  static constexpr int KMAX = 64;
+  static constexpr int RMAX = 64;  // rhs per batch
+  BBlas(hipStream_t st);
+  ~BBlas();
+  BBlas(const BBlas&) = delete;
+  BBlas& operator=(const BBlas&) = delete;
+};
+
+// per-rhs reductions (synchronise the stream; results [row][r] in h_res)
+template <typename R>
+void bdots_dev(BBlas& b, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
+               const BShape& sh, bool with_norm, cplx<double>* d_out);
+// copy count doubles to b.h_res and wait
+void bfetch(BBlas& b, const double* d_src, int count);
+template <typename R> void bnorm2(BBlas& b, const cplx<R>* x, const BShape& sh, double* out);
+
+// w -= sum_j c[j][r] V_j (c on the device, [k][nrhs])
+template <typename R>
+void bmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k, cplx<R>* w,
+                   const BShape& sh, hipStream_t s);
+// w += sum_j c[j][r] V_j (c on the host, [k][nrhs], uploaded through pinned memory)
+template <typename R>
+void bmultiaxpy(BBlas& b, const cplx<double>* c, const cplx<R>* V, size_t ldv, int k,
+                cplx<R>* w, const BShape& sh);
+// v = (w - sum_j h2[j][r] V_j) / hn_r with hn_r = sqrt(h2[k][r].re - sum_j |h2[j][r]|^2)
+// (0 if hn_r <= 0); hn_r stored in h2[k][r].im
+template <typename R>
+void bsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
+                  cplx<R>* v, const BShape& sh, hipStream_t s);
+// y = x * a[r] with precision change (a on the host, nrhs reals)
+template <typename Rs, typename Rd>
+void bconvert(BBlas& b, const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const double* a);
+
+}  // namespace mgc

@@ -1,0 +1,228 @@
+// Batched restarted (F)GMRES: nrhs independent systems A x_r = b_r advanced in
+// lockstep, so that every operator application and every BLAS pass serves
+// all right-hand sides at once.  Each rhs keeps its own Hessenberg matrix,
+// complex Givens rotations and convergence test: once rhs r has converged
+// (|g_{j+1}| <= tol |b_r|) or broken down, the Krylov columns computed
+// afterwards are not used for its update, exactly as if it had been solved
+// alone and stopped there.  CGS2 with device-resident coefficients and one
+// host synchronisation per Arnoldi step, as the single-rhs solver.
+#pragma once
+#include <cmath>
+#include <complex>
+#include <vector>
+
+#include "bblas.hpp"
+
+namespace mgc {
+
+template <typename R>
+struct BLinOp {
+  BShape sh;
+  virtual ~BLinOp() {}
+  virtual void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) = 0;
+};
+
+template <typename R>
+struct BPrec {
+  virtual ~BPrec() {}
+  virtual void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) = 0;
+};
+
+struct BKrylovStats {
+  int iters = 0;    // lockstep iterations
+  int applies = 0;  // batched operator applications
+  std::vector<double> rel_res;
+  std::vector<int> converged;
+  std::vector<int> iters_rhs;  // iteration at which each rhs converged (or iters)
+};
+
+template <typename R>
+class BGmres {
+ public:
+  BGmres(const BShape& sh, int m, bool flexible) : sh_(sh), m_(m), flexible_(flexible) {
+    MGC_CHECK(m >= 1 && 2 * m + 3 <= 2 * BBlas::KMAX + 3, MGC_ERR_ARG,
+              "restart length out of range");
+    MGC_CHECK(sh.nrhs >= 1 && sh.nrhs <= BBlas::RMAX, MGC_ERR_ARG, "batch of 1..64 rhs");
+    const size_t n = sh.n;
+    MGC_HIP(hipMalloc(&V_, (size_t)(m + 1) * n * sizeof(cplx<R>)));
+    if (flexible) MGC_HIP(hipMalloc(&Z_, (size_t)m * n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&w_, n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&r_, n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&d_h_, (size_t)(2 * BBlas::KMAX + 3) * sh.nrhs * sizeof(cplx<double>)));
+  }
+  ~BGmres() {
+    void* b[] = {V_, Z_, w_, r_, d_h_};
+    for (void* p : b)
+      if (p) (void)hipFree(p);
+  }
+  BGmres(const BGmres&) = delete;
+  BGmres& operator=(const BGmres&) = delete;
+
+  BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>* P, const cplx<R>* b, cplx<R>* x,
+                     bool x_zero, double tol, int maxit) {
+    typedef std::complex<double> C;
+    hipStream_t s = bl.s;
+    const size_t n = sh_.n;
+    const int nr = sh_.nrhs;
+    const int m = m_;
+    BKrylovStats st;
+    st.rel_res.assign(nr, 0.0);
+    st.converged.assign(nr, 0);
+    st.iters_rhs.assign(nr, 0);
+    std::vector<double> bnorm(nr), beta(nr), tmp(nr);
+    bnorm2<R>(bl, b, sh_, tmp.data());
+    for (int r = 0; r < nr; ++r) bnorm[r] = std::sqrt(tmp[r]);
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, r_, n, s);
+    } else {
+      A.apply(x, w_, s);
+      ++st.applies;
+      vsub<R>(b, w_, r_, n, s);
+    }
+    bnorm2<R>(bl, r_, sh_, tmp.data());
+    for (int r = 0; r < nr; ++r) beta[r] = std::sqrt(tmp[r]);
+    std::vector<C> H((size_t)nr * (m + 1) * m), g((size_t)nr * (m + 1)), sn((size_t)nr * m);
+    std::vector<double> cs((size_t)nr * m);
+    std::vector<int> jr(nr);      // Krylov columns used by rhs r in this cycle
+    std::vector<char> live(nr);   // rhs still iterating in this cycle
+    std::vector<cplx<double>> y((size_t)m * nr);
+    std::vector<double> scale(nr);
+    auto Hc = [&](int r, int i, int j) -> C& { return H[((size_t)r * (m + 1) + i) * m + j]; };
+    while (true) {
+      bool all_done = true;
+      for (int r = 0; r < nr; ++r) {
+        st.rel_res[r] = bnorm[r] > 0.0 ? beta[r] / bnorm[r] : 0.0;
+        const bool done = bnorm[r] == 0.0 || st.rel_res[r] <= tol;
+        if (done && !st.converged[r]) {
+          st.converged[r] = 1;
+          st.iters_rhs[r] = st.iters;
+        }
+        live[r] = !done;
+        all_done = all_done && done;
+        jr[r] = 0;
+      }
+      if (all_done || st.iters >= maxit) break;
+      for (int r = 0; r < nr; ++r) scale[r] = live[r] && beta[r] > 0.0 ? 1.0 / beta[r] : 0.0;
+      bconvert<R, R>(bl, r_, V_, sh_, scale.data());
+      std::fill(g.begin(), g.end(), C(0.0));
+      for (int r = 0; r < nr; ++r) g[(size_t)r * (m + 1)] = live[r] ? beta[r] : 0.0;
+      int j = 0;
+      for (; j < m;) {
+        cplx<R>* vj = V_ + (size_t)j * n;
+        const cplx<R>* z = vj;
+        if (flexible_) {
+          cplx<R>* zj = Z_ + (size_t)j * n;
+          if (P) P->apply(vj, zj, s);
+          else vcopy<R>(vj, zj, n, s);
+          z = zj;
+        }
+        A.apply(z, w_, s);
+        ++st.applies;
+        cplx<double>* dh1 = d_h_;
+        cplx<double>* dh2 = d_h_ + (size_t)(j + 1) * nr;
+        cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
+        bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, false, dh1);
+        bmultisub_dev<R>(dh1, V_, n, j + 1, w_, sh_, s);
+        bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, true, dh2);
+        bsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, sh_, s);
+        bfetch(bl, reinterpret_cast<const double*>(d_h_), 2 * (2 * j + 3) * nr);
+        const double* hr = bl.h_res;
+        // per rhs: h = h1 + h2, the new norm, cancellation check
+        bool redo = false;
+        std::vector<double> hn(nr), wn2(nr), sum2(nr);
+        for (int r = 0; r < nr; ++r) {
+          sum2[r] = 0.0;
+          for (int i = 0; i <= j; ++i) {
+            const double* a1 = hr + 2 * ((size_t)i * nr + r);
+            const double* a2 = hr + 2 * ((size_t)(j + 1 + i) * nr + r);
+            Hc(r, i, j) = C(a1[0] + a2[0], a1[1] + a2[1]);
+            sum2[r] += a2[0] * a2[0] + a2[1] * a2[1];
+          }
+          const double* nrm = hr + 2 * ((size_t)(2 * j + 2) * nr + r);
+          wn2[r] = nrm[0];
+          hn[r] = nrm[1];
+          if (live[r] && wn2[r] > 0.0 && (!(hn[r] > 0.0) || sum2[r] > 0.25 * wn2[r])) redo = true;
+        }
+        if (redo) {
+          // strong cancellation for some rhs: form w'' and take its norms directly
+          bmultisub_dev<R>(dh2, V_, n, j + 1, w_, sh_, s);
+          bnorm2<R>(bl, w_, sh_, tmp.data());
+          for (int r = 0; r < nr; ++r) {
+            hn[r] = std::sqrt(tmp[r]);
+            scale[r] = hn[r] > 0.0 ? 1.0 / hn[r] : 0.0;
+          }
+          bconvert<R, R>(bl, w_, vnext, sh_, scale.data());
+        }
+        ++st.iters;
+        ++j;
+        bool any_live = false;
+        for (int r = 0; r < nr; ++r) {
+          if (!live[r]) continue;
+          const int jj = j - 1;
+          Hc(r, jj + 1, jj) = hn[r];
+          for (int i = 0; i < jj; ++i) {
+            const C a = Hc(r, i, jj), bb = Hc(r, i + 1, jj);
+            Hc(r, i, jj) = cs[(size_t)r * m + i] * a + sn[(size_t)r * m + i] * bb;
+            Hc(r, i + 1, jj) = -std::conj(sn[(size_t)r * m + i]) * a + cs[(size_t)r * m + i] * bb;
+          }
+          const C a = Hc(r, jj, jj);
+          const double bb = std::abs(Hc(r, jj + 1, jj));
+          const double t = std::sqrt(std::norm(a) + bb * bb);
+          if (std::abs(a) == 0.0) {
+            cs[(size_t)r * m + jj] = 0.0;
+            sn[(size_t)r * m + jj] = 1.0;
+          } else {
+            cs[(size_t)r * m + jj] = std::abs(a) / t;
+            sn[(size_t)r * m + jj] = (a / std::abs(a)) * (bb / t);
+          }
+          Hc(r, jj, jj) = cs[(size_t)r * m + jj] * a + sn[(size_t)r * m + jj] * bb;
+          Hc(r, jj + 1, jj) = 0.0;
+          C* gr = &g[(size_t)r * (m + 1)];
+          gr[jj + 1] = -std::conj(sn[(size_t)r * m + jj]) * gr[jj];
+          gr[jj] = cs[(size_t)r * m + jj] * gr[jj];
+          jr[r] = j;
+          if (std::abs(gr[j]) <= tol * bnorm[r] || hn[r] == 0.0 || Hc(r, jj, jj) == 0.0)
+            live[r] = 0;  // converged or broken down: freeze its columns
+          any_live = any_live || live[r];
+        }
+        if (!any_live || st.iters >= maxit) break;
+      }
+      // per-rhs back substitution and one batched update of every x_r
+      std::fill(y.begin(), y.end(), cplx<double>{0.0, 0.0});
+      for (int r = 0; r < nr; ++r) {
+        const int k = jr[r];
+        std::vector<C> yy(k);
+        for (int i = k - 1; i >= 0; --i) {
+          C acc = g[(size_t)r * (m + 1) + i];
+          for (int q = i + 1; q < k; ++q) acc -= Hc(r, i, q) * yy[q];
+          yy[i] = Hc(r, i, i) != 0.0 ? acc / Hc(r, i, i) : C(0.0);
+        }
+        for (int i = 0; i < k; ++i) y[(size_t)i * nr + r] = {yy[i].real(), yy[i].imag()};
+      }
+      bmultiaxpy<R>(bl, y.data(), flexible_ ? Z_ : V_, n, j, x, sh_);
+      A.apply(x, w_, s);
+      ++st.applies;
+      vsub<R>(b, w_, r_, n, s);
+      bnorm2<R>(bl, r_, sh_, tmp.data());
+      for (int r = 0; r < nr; ++r) beta[r] = std::sqrt(tmp[r]);
+    }
+    for (int r = 0; r < nr; ++r)
+      if (!st.converged[r]) st.iters_rhs[r] = st.iters;
+    return st;
+  }
+
+  const BShape& shape() const { return sh_; }
+
+ private:
+  BShape sh_;
+  int m_;
+  bool flexible_;
+  cplx<R>* V_ = nullptr;
+  cplx<R>* Z_ = nullptr;
+  cplx<R>* w_ = nullptr;
+  cplx<R>* r_ = nullptr;
+  cplx<double>* d_h_ = nullptr;  // [2*KMAX+3][nrhs] CGS2 coefficients
+};
+
+}  // namespace mgc

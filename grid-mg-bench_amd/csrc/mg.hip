@@ -21,9 +21,11 @@
 // always built in FP64 and rounded for the FP32 hierarchy.
 #include <chrono>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "aggregation.hpp"
+#include "bkrylov.hpp"
 #include "coarse_op.hpp"
 #include "fine_op.hpp"
 #include "krylov.hpp"
@@ -80,6 +82,35 @@ struct Level {
   }
 };
 
+// batched operators: nrhs right-hand sides per apply, vectors [site][rhs][dof]
+template <typename R>
+struct BFineLin : BLinOp<R> {
+  mgc_fine_op* op;
+  BFineLin(mgc_fine_op* o, int nrhs) : op(o) {
+    this->sh.n = (size_t)o->V * 12 * nrhs;
+    this->sh.nrhs = nrhs;
+    this->sh.dof = 12;
+  }
+  void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) override {
+    ok(mgc_fine_op_M(op, in, out, prec_of<R>(), this->sh.nrhs, (void*)s));
+  }
+};
+
+// coarse levels of the batched solve: the multi-rhs MFMA stencil with FP32
+// vectors, in FP32 or in FP16/BF16 operands (quantised on the fly)
+struct BCoarseLin : BLinOp<float> {
+  mgc_coarse_op* op;
+  int mprec;
+  BCoarseLin(mgc_coarse_op* o, int nrhs, int mp) : op(o), mprec(mp) {
+    sh.n = (size_t)o->V * o->N * nrhs;
+    sh.nrhs = nrhs;
+    sh.dof = o->N;
+  }
+  void apply(const cplx<float>* in, cplx<float>* out, hipStream_t s) override {
+    ok(mgc_coarse_op_M_mrhs(op, in, nullptr, MGC_F32, out, sh.nrhs, mprec, 0, (void*)s));
+  }
+};
+
 struct MGBase {
   virtual ~MGBase() {}
   virtual void setup(hipStream_t s) = 0;
@@ -89,6 +120,9 @@ struct MGBase {
                         hipStream_t s) = 0;
   virtual mgc_coarse_op* coarse_op(int level) = 0;
   virtual mgc_aggregation* aggregation(int level) = 0;
+  // batched preconditioner for nrhs right-hand sides (FP32 hierarchy only)
+  virtual void prepare_batch(int nrhs, int coarse_prec, hipStream_t s) = 0;
+  virtual void precondition64_batch(const cplx<double>* in, cplx<double>* out, hipStream_t s) = 0;
   int precond_applies = 0;
 };
 
@@ -113,6 +147,7 @@ struct MG : MGBase {
 
   MG(mgc_fine_op* f, const mgc_mg_params& pp, hipStream_t s) : fop(f), p(pp), bl(s) {}
   ~MG() override {
+    free_batch();
     L.clear();
     if (in_r) (void)hipFree(in_r);
     if (out_r) (void)hipFree(out_r);
@@ -302,6 +337,108 @@ struct MG : MGBase {
     vconvert<R, double>(x, out, n, nin, s);
   }
 
+  // ---- batched K-cycle (nrhs right-hand sides in lockstep) ----------------
+  struct BLevelState {
+    std::unique_ptr<BLinOp<R>> op;
+    std::unique_ptr<BGmres<R>> smoother, wrapper;
+    cplx<R>* rc = nullptr;  // next-level residual / correction, batched
+    cplx<R>* ec = nullptr;
+    ~BLevelState() {
+      if (rc) (void)hipFree(rc);
+      if (ec) (void)hipFree(ec);
+    }
+  };
+  struct BLevelPrec : BPrec<R> {
+    MG* mg;
+    int l;
+    BLevelPrec(MG* m, int lv) : mg(m), l(lv) {}
+    void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) override {
+      mg->kcycle_b(l, in, out, s);
+    }
+  };
+  int bn = 0, bprec = -1;
+  std::vector<std::unique_ptr<BLevelState>> BL;
+  std::vector<std::unique_ptr<BLevelPrec>> bprecs;
+  std::unique_ptr<BBlas> bbl;
+  cplx<R>* bin = nullptr;
+  cplx<R>* bout = nullptr;
+
+  void free_batch() {
+    BL.clear();
+    bprecs.clear();
+    if (bin) (void)hipFree(bin);
+    if (bout) (void)hipFree(bout);
+    bin = bout = nullptr;
+    bn = 0;
+  }
+
+  void prepare_batch(int nrhs, int coarse_prec, hipStream_t s) override {
+    MGC_CHECK(ready, MGC_ERR_STATE, "multigrid setup not run");
+    MGC_CHECK(prec_of<R>() == MGC_F32, MGC_ERR_PRECISION,
+              "the batched solve needs the FP32 hierarchy (multi-rhs MFMA coarse levels)");
+    MGC_CHECK(coarse_prec == MGC_F32 || coarse_prec == MGC_F16 || coarse_prec == MGC_BF16,
+              MGC_ERR_PRECISION, "coarse precision must be F32, F16 or BF16");
+    if (bn == nrhs && bprec == coarse_prec) return;
+    free_batch();
+    if (!bbl) bbl = std::make_unique<BBlas>(s);
+    for (int l = 0; l < p.nlevels; ++l) {
+      auto st = std::make_unique<BLevelState>();
+      if (l == 0) {
+        st->op = std::make_unique<BFineLin<R>>(fop, nrhs);
+      } else {
+        if constexpr (std::is_same<R, float>::value)
+          st->op = std::make_unique<BCoarseLin>(L[l - 1]->copR, nrhs, coarse_prec);
+      }
+      const BShape& sh = st->op->sh;
+      if (l + 1 < p.nlevels) {
+        st->smoother = std::make_unique<BGmres<R>>(sh, p.smoother_restart, false);
+        const size_t nc = L[l + 1]->n * (size_t)nrhs;
+        MGC_HIP(hipMalloc(&st->rc, nc * sizeof(cplx<R>)));
+        MGC_HIP(hipMalloc(&st->ec, nc * sizeof(cplx<R>)));
+      } else {
+        st->smoother = std::make_unique<BGmres<R>>(sh, p.coarsest_restart, false);
+      }
+      BL.push_back(std::move(st));
+    }
+    for (int l = 0; l + 2 < p.nlevels; ++l)
+      BL[l]->wrapper = std::make_unique<BGmres<R>>(BL[l + 1]->op->sh, p.wrapper_restart, true);
+    for (int l = 0; l < p.nlevels; ++l) bprecs.push_back(std::make_unique<BLevelPrec>(this, l));
+    MGC_HIP(hipMalloc(&bin, BL[0]->op->sh.n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&bout, BL[0]->op->sh.n * sizeof(cplx<R>)));
+    bn = nrhs;
+    bprec = coarse_prec;
+  }
+
+  // MultiGridPreconditioner::operator() for nrhs right-hand sides at once
+  void kcycle_b(int l, const cplx<R>* in, cplx<R>* out, hipStream_t s) {
+    BLevelState& lv = *BL[l];
+    BLevelState& nx = *BL[l + 1];
+    agg_transfer(L[l]->agg, in, lv.rc, prec_of<R>(), bn, true, 0, 8, s);
+    if (l + 1 == p.nlevels - 1)
+      nx.smoother->solve(*bbl, *nx.op, nullptr, lv.rc, lv.ec, true, p.coarsest_tol,
+                         p.coarsest_iters);
+    else
+      lv.wrapper->solve(*bbl, *nx.op, bprecs[l + 1].get(), lv.rc, lv.ec, true, p.wrapper_tol,
+                        p.wrapper_iters);
+    agg_transfer(L[l]->agg, lv.ec, out, prec_of<R>(), bn, false, 0, 8, s);
+    lv.smoother->solve(*bbl, *lv.op, nullptr, in, out, false, p.smoother_tol, p.smoother_iters);
+  }
+
+  void precondition64_batch(const cplx<double>* in, cplx<double>* out, hipStream_t s) override {
+    MGC_CHECK(bn > 0, MGC_ERR_STATE, "batched preconditioner not prepared");
+    BShape sh64 = BL[0]->op->sh;
+    std::vector<double> nin(bn), inv(bn);
+    bnorm2<double>(*bbl, in, sh64, nin.data());
+    for (int r = 0; r < bn; ++r) {
+      nin[r] = std::sqrt(nin[r]);
+      inv[r] = nin[r] > 0.0 ? 1.0 / nin[r] : 0.0;
+    }
+    bconvert<double, R>(*bbl, in, bin, sh64, inv.data());
+    kcycle_b(0, bin, bout, s);
+    bconvert<R, double>(*bbl, bout, out, sh64, nin.data());
+    ++precond_applies;
+  }
+
   mgc_coarse_op* coarse_op(int level) override {
     MGC_CHECK(ready && level >= 1 && level < p.nlevels, MGC_ERR_ARG, "no such coarse level");
     return L[level - 1]->cop64;
@@ -317,6 +454,14 @@ struct PrecAdapter : Prec<double> {
   explicit PrecAdapter(MGBase* m) : mg(m) {}
   void apply(const cplx<double>* in, cplx<double>* out, hipStream_t s) override {
     mg->precondition64(in, out, s);
+  }
+};
+
+struct BPrecAdapter : BPrec<double> {
+  MGBase* mg;
+  explicit BPrecAdapter(MGBase* m) : mg(m) {}
+  void apply(const cplx<double>* in, cplx<double>* out, hipStream_t s) override {
+    mg->precondition64_batch(in, out, s);
   }
 };
 
@@ -432,6 +577,36 @@ int mgc_mg_solve(mgc_mg* m, const void* b, void* x, double tol, int maxit, int r
       stats->precond_applies = m->impl->precond_applies - pa0;
       stats->seconds = std::chrono::duration<double>(t1 - t0).count();
     }
+  });
+}
+
+int mgc_mg_solve_mrhs(mgc_mg* m, const void* b, void* x, int nrhs, int coarse_prec, double tol,
+                      int maxit, int restart, mgc_solve_stats* stats, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(m && m->impl && b && x, MGC_ERR_STATE, "setup not run or null vector");
+    MGC_CHECK(as_stream(stream) == m->stream, MGC_ERR_ARG, "use the stream given to setup");
+    MGC_CHECK(nrhs >= 1 && nrhs <= BBlas::RMAX, MGC_ERR_ARG, "nrhs must be 1..64");
+    hipStream_t s = m->stream;
+    m->impl->prepare_batch(nrhs, coarse_prec, s);
+    BBlas bl(s);
+    BFineLin<double> A(m->fop, nrhs);
+    BPrecAdapter P(m->impl.get());
+    BGmres<double> outer(A.sh, restart, true);
+    MGC_HIP(hipStreamSynchronize(s));
+    const int pa0 = m->impl->precond_applies;
+    const auto t0 = std::chrono::steady_clock::now();
+    const BKrylovStats st =
+        outer.solve(bl, A, &P, (const cplx<double>*)b, (cplx<double>*)x, true, tol, maxit);
+    MGC_HIP(hipStreamSynchronize(s));
+    const auto t1 = std::chrono::steady_clock::now();
+    if (stats)
+      for (int r = 0; r < nrhs; ++r) {
+        stats[r].iterations = st.iters_rhs[r];
+        stats[r].converged = st.converged[r];
+        stats[r].true_residual = st.rel_res[r];
+        stats[r].precond_applies = m->impl->precond_applies - pa0;
+        stats[r].seconds = std::chrono::duration<double>(t1 - t0).count();
+      }
   });
 }
 

@@ -301,6 +301,16 @@ class MultiGridPreconditioner {
     check(mgc_mg_solve(h_, b.data(), x.data(), tol, maxit, restart, &st, stream_));
     return st;
   }
+  // the same for nrhs right-hand sides in lockstep, b and x [site][rhs][12];
+  // coarse levels on the multi-rhs MFMA stencil in coarse_prec
+  std::vector<mgc_solve_stats> SolveMRHS(const DeviceField& b, DeviceField& x, int nrhs,
+                                         double tol, int coarse_prec = MGC_F16,
+                                         int maxit = 1000, int restart = 20) {
+    std::vector<mgc_solve_stats> st((size_t)nrhs);
+    check(mgc_mg_solve_mrhs(h_, b.data(), x.data(), nrhs, coarse_prec, tol, maxit, restart,
+                            st.data(), stream_));
+    return st;
+  }
 
  private:
   mgc_mg* h_ = nullptr;

@@ -114,6 +114,7 @@ _SIGS = {
     "mgc_mg_precondition": ([_vp, _vp, _vp, _vp], _i),
     "mgc_mg_vcycle": ([_vp, _vp, _vp, _i, _c.c_double, _vp], _i),
     "mgc_mg_solve": ([_vp, _vp, _vp, _c.c_double, _i, _i, _c.POINTER(SolveStats), _vp], _i),
+    "mgc_mg_solve_mrhs": ([_vp, _vp, _vp, _i, _i, _c.c_double, _i, _i, _vp, _vp], _i),
     "mgc_mg_coarse_op": ([_vp, _i, _c.POINTER(_vp)], _i),
     "mgc_mg_aggregation": ([_vp, _i, _c.POINTER(_vp)], _i),
     "mgc_fine_gmres_solve": ([_vp, _vp, _vp, _c.c_double, _i, _i, _c.POINTER(SolveStats),
@@ -637,6 +638,13 @@ class Multigrid:
         call("mgc_mg_solve", self.handle, _p(b), _p(x), float(tol), int(maxit), int(restart),
              _c.byref(st), self._s())
         return st.as_dict()
+
+    def solve_mrhs(self, b, x, nrhs, coarse_prec=F32, tol=1e-10, maxit=200, restart=20) -> list:
+        """Batched solve of nrhs systems; b, x are [site][rhs][12] FP64."""
+        st = (SolveStats * int(nrhs))()
+        call("mgc_mg_solve_mrhs", self.handle, _p(b), _p(x), int(nrhs), int(coarse_prec),
+             float(tol), int(maxit), int(restart), st, self._s())
+        return [s.as_dict() for s in st]
 
     def coarse_op_handle(self, level: int) -> int:
         h = _vp()

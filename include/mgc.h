@@ -375,6 +375,21 @@ int mgc_mg_vcycle(mgc_mg* mg, const void* in, void* out, int max_iter, double to
  * FP64 outer iteration, zero initial guess, stop at ||b - D x|| <= tol ||b||. */
 int mgc_mg_solve(mgc_mg* mg, const void* b, void* x, double tol, int maxit, int restart,
                  mgc_solve_stats* stats, void* stream);
+/* The same solve for nrhs right-hand sides at once (1 <= nrhs <= 64): b and x
+ * are [site][rhs][12] FP64.  The outer FGMRES runs the nrhs systems in
+ * lockstep (one batched operator application and one batched BLAS pass per
+ * step for all of them, each rhs with its own Hessenberg matrix, rotations
+ * and stopping point), and the K-cycle applies the whole batch level by
+ * level: multi-rhs fine operator and transfers, and the coarse levels
+ * through the multi-rhs MFMA stencil (mgc_coarse_op_M_mrhs) with operands in
+ * coarse_prec (MGC_F32, MGC_F16 or MGC_BF16; FP32 accumulation).  Needs an
+ * FP32 hierarchy whose coarse nbasis is 32 or 64.  stats holds nrhs entries:
+ * per rhs its outer iteration count at convergence and its true residual;
+ * precond_applies and seconds are those of the whole batch.  Replaces one
+ * solve per rhs with the reference's solver on the multi-rhs (5d) grid
+ * (src/Benchmark_mrhs_coarsening.cc, include/CoarsenedMatrixUpstreamImprovedDirsaveLutMRHS.h). */
+int mgc_mg_solve_mrhs(mgc_mg* mg, const void* b, void* x, int nrhs, int coarse_prec, double tol,
+                      int maxit, int restart, mgc_solve_stats* stats, void* stream);
 /* the FP64 coarse operator of level >= 1 and the aggregation of level >= 0 (owned by mg) */
 int mgc_mg_coarse_op(mgc_mg* mg, int level, mgc_coarse_op** cop);
 int mgc_mg_aggregation(mgc_mg* mg, int level, mgc_aggregation** agg);

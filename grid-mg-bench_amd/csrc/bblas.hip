@@ -2,6 +2,7 @@
 #include "bblas.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 namespace mgc {
@@ -33,12 +34,14 @@ static size_t gcd_sz(size_t a, size_t b) {
   return a;
 }
 
-// reduction grid: a multiple of cols/gcd(256, cols) blocks, so that the total
-// thread count is a multiple of cols and each thread keeps one rhs
-static int bred_blocks(const BShape& sh) {
-  const size_t C = (size_t)sh.cols();
+// reduction grid: a multiple of C/gcd(256, C) blocks (C = columns per site,
+// or half of them when a thread takes element pairs), so that the total
+// thread count is a multiple of C and each thread keeps one rhs
+static int bred_blocks(const BShape& sh, bool pairs) {
+  const size_t C = (size_t)sh.cols() / (pairs ? 2 : 1);
+  const size_t n = sh.n / (pairs ? 2 : 1);
   const size_t base = C / gcd_sz(256, C);  // blocks per lcm(256, C) threads
-  size_t want = std::min<size_t>(2048, (sh.n + 255) / 256);
+  size_t want = std::min<size_t>(2048, (n + 255) / 256);
   want = std::max<size_t>(want, base);
   return (int)((want + base - 1) / base * base);
 }
@@ -90,6 +93,64 @@ __global__ __launch_bounds__(256) void k_bdot_part(const cplx<R>* __restrict__ V
     for (int t = 0; t < (int)blockDim.x; ++t) {
       const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
       if ((int)((gt % C) / dof) == rr) {
+        sr += sm[jj][t][0];
+        si += sm[jj][t][1];
+      }
+    }
+    double* o = part + ((((size_t)(j0 + jj) * nrhs + rr) * gridDim.x) + blockIdx.x) * 2;
+    o[0] = sr;
+    o[1] = si;
+  }
+}
+
+// FP32 form taking two consecutive elements (one 16-byte load; dof is even,
+// so a pair never straddles two rhs): thread t keeps rhs ((2t) % C) / dof
+__global__ __launch_bounds__(256) void k_bdot_part_f2(const cplx<float>* __restrict__ V,
+                                                      size_t ldv, int k, int rows,
+                                                      const cplx<float>* __restrict__ w,
+                                                      size_t n, int C, int dof, int nrhs,
+                                                      double* __restrict__ part) {
+  __shared__ double sm[BD_ROWS][256][2];
+  const int j0 = blockIdx.y * BD_ROWS;
+  const int nj = min(BD_ROWS, rows - j0);
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n2 = n / 2;
+  double re[BD_ROWS], im[BD_ROWS];
+#pragma unroll
+  for (int jj = 0; jj < BD_ROWS; ++jj) re[jj] = im[jj] = 0.0;
+  for (size_t i2 = g; i2 < n2; i2 += T) {
+    const float4 b = reinterpret_cast<const float4*>(w)[i2];
+#pragma unroll
+    for (int jj = 0; jj < BD_ROWS; ++jj) {
+      if (jj < nj) {
+        const int j = j0 + jj;
+        if (j == k) {
+          re[jj] += (double)b.x * b.x + (double)b.y * b.y;
+          re[jj] += (double)b.z * b.z + (double)b.w * b.w;
+        } else {
+          const float4 a = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          re[jj] += (double)a.x * b.x + (double)a.y * b.y;
+          im[jj] += (double)a.x * b.y - (double)a.y * b.x;
+          re[jj] += (double)a.z * b.z + (double)a.w * b.w;
+          im[jj] += (double)a.z * b.w - (double)a.w * b.z;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int jj = 0; jj < BD_ROWS; ++jj) {
+    sm[jj][threadIdx.x][0] = re[jj];
+    sm[jj][threadIdx.x][1] = im[jj];
+  }
+  __syncthreads();
+  const int C2 = C / 2;
+  for (int q = threadIdx.x; q < nj * nrhs; q += blockDim.x) {
+    const int jj = q / nrhs, rr = q % nrhs;
+    double sr = 0.0, si = 0.0;
+    for (int t = 0; t < (int)blockDim.x; ++t) {
+      const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
+      if ((int)((2 * (gt % C2)) / dof) == rr) {
         sr += sm[jj][t][0];
         si += sm[jj][t][1];
       }
@@ -198,7 +259,11 @@ void bdots_dev(BBlas& b, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
   MGC_CHECK(k >= 0 && k <= BBlas::KMAX && (k > 0 || with_norm), MGC_ERR_ARG,
             "batched reduction: bad row count");
   const int rows = k + (with_norm ? 1 : 0);
-  const int nblk = bred_blocks(sh);
+  bool pairs = false;
+  if constexpr (std::is_same<R, float>::value)
+    pairs = sh.dof % 2 == 0 && ldv % 2 == 0 && ((uintptr_t)V % 16) == 0 &&
+            ((uintptr_t)w % 16) == 0;
+  const int nblk = bred_blocks(sh, pairs);
   const size_t need = (size_t)rows * sh.nrhs * nblk * 2;
   if (need > b.part_cap) {
     if (b.d_part) MGC_HIP(hipFree(b.d_part));
@@ -208,8 +273,18 @@ void bdots_dev(BBlas& b, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
     MGC_HIP(hipMalloc(&b.d_part, cap * sizeof(double)));
     b.part_cap = cap;
   }
-  k_bdot_part<R><<<dim3(nblk, (rows + BD_ROWS - 1) / BD_ROWS), 256, 0, b.s>>>(
-      V, ldv, k, rows, w, sh.n, sh.cols(), sh.dof, sh.nrhs, b.d_part);
+  const dim3 grid(nblk, (rows + BD_ROWS - 1) / BD_ROWS);
+  if constexpr (std::is_same<R, float>::value) {
+    if (pairs)
+      k_bdot_part_f2<<<grid, 256, 0, b.s>>>(V, ldv, k, rows, w, sh.n, sh.cols(), sh.dof,
+                                            sh.nrhs, b.d_part);
+    else
+      k_bdot_part<R><<<grid, 256, 0, b.s>>>(V, ldv, k, rows, w, sh.n, sh.cols(), sh.dof,
+                                            sh.nrhs, b.d_part);
+  } else {
+    k_bdot_part<R><<<grid, 256, 0, b.s>>>(V, ldv, k, rows, w, sh.n, sh.cols(), sh.dof, sh.nrhs,
+                                          b.d_part);
+  }
   MGC_LAUNCH_CHECK();
   k_bdot_final<<<rows * sh.nrhs, 256, 0, b.s>>>(b.d_part, nblk, reinterpret_cast<double*>(d_out));
   MGC_LAUNCH_CHECK();

@@ -4,8 +4,10 @@
 // complex Givens rotations and convergence test: once rhs r has converged
 // (|g_{j+1}| <= tol |b_r|) or broken down, the Krylov columns computed
 // afterwards are not used for its update, exactly as if it had been solved
-// alone and stopped there.  CGS2 with device-resident coefficients and one
-// host synchronisation per Arnoldi step, as the single-rhs solver.
+// alone and stopped there.  Classical Gram-Schmidt with selective
+// re-orthogonalisation and device-resident coefficients, as the single-rhs
+// solver (the second projection runs for the whole batch when any live rhs
+// needs it).
 #pragma once
 #include <cmath>
 #include <complex>
@@ -119,40 +121,57 @@ class BGmres {
         }
         A.apply(z, w_, s);
         ++st.applies;
+        // classical Gram-Schmidt with selective re-orthogonalisation (as the
+        // single-rhs solver): h1 and |w|^2 at rows 0 .. j+1 of d_h_; the
+        // second projection at rows j+2 .. 2j+3 when some live rhs has
+        // |w'|^2 = |w|^2 - |h1|^2 < |w|^2 / 2
         cplx<double>* dh1 = d_h_;
-        cplx<double>* dh2 = d_h_ + (size_t)(j + 1) * nr;
+        cplx<double>* dh2 = d_h_ + (size_t)(j + 2) * nr;
         cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
-        bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, false, dh1);
-        bmultisub_dev<R>(dh1, V_, n, j + 1, w_, sh_, s);
-        bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, true, dh2);
-        bsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, sh_, s);
-        bfetch(bl, reinterpret_cast<const double*>(d_h_), 2 * (2 * j + 3) * nr);
-        const double* hr = bl.h_res;
-        // per rhs: h = h1 + h2, the new norm, cancellation check
-        bool redo = false;
-        std::vector<double> hn(nr), wn2(nr), sum2(nr);
+        bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, true, dh1);
+        bfetch(bl, reinterpret_cast<const double*>(dh1), 2 * (j + 2) * nr);
+        std::vector<double> hn(nr), sum1(nr, 0.0), wn2(nr);
+        bool reorth = false;
         for (int r = 0; r < nr; ++r) {
-          sum2[r] = 0.0;
           for (int i = 0; i <= j; ++i) {
-            const double* a1 = hr + 2 * ((size_t)i * nr + r);
-            const double* a2 = hr + 2 * ((size_t)(j + 1 + i) * nr + r);
-            Hc(r, i, j) = C(a1[0] + a2[0], a1[1] + a2[1]);
-            sum2[r] += a2[0] * a2[0] + a2[1] * a2[1];
+            const double* a1 = bl.h_res + 2 * ((size_t)i * nr + r);
+            Hc(r, i, j) = C(a1[0], a1[1]);
+            sum1[r] += a1[0] * a1[0] + a1[1] * a1[1];
           }
-          const double* nrm = hr + 2 * ((size_t)(2 * j + 2) * nr + r);
-          wn2[r] = nrm[0];
-          hn[r] = nrm[1];
-          if (live[r] && wn2[r] > 0.0 && (!(hn[r] > 0.0) || sum2[r] > 0.25 * wn2[r])) redo = true;
+          wn2[r] = bl.h_res[2 * ((size_t)(j + 1) * nr + r)];
+          hn[r] = wn2[r] - sum1[r] > 0.0 ? std::sqrt(wn2[r] - sum1[r]) : 0.0;
+          if (live[r] && sum1[r] > 0.5 * wn2[r]) reorth = true;
         }
-        if (redo) {
-          // strong cancellation for some rhs: form w'' and take its norms directly
-          bmultisub_dev<R>(dh2, V_, n, j + 1, w_, sh_, s);
-          bnorm2<R>(bl, w_, sh_, tmp.data());
+        if (!reorth) {
+          bsubnorm_dev<R>(dh1, V_, n, j + 1, w_, vnext, sh_, s);
+        } else {
+          bmultisub_dev<R>(dh1, V_, n, j + 1, w_, sh_, s);
+          bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, true, dh2);
+          bsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, sh_, s);
+          bfetch(bl, reinterpret_cast<const double*>(dh2), 2 * (j + 2) * nr);
+          bool redo = false;
+          std::vector<double> sum2(nr, 0.0), wn2b(nr);
           for (int r = 0; r < nr; ++r) {
-            hn[r] = std::sqrt(tmp[r]);
-            scale[r] = hn[r] > 0.0 ? 1.0 / hn[r] : 0.0;
+            for (int i = 0; i <= j; ++i) {
+              const double* a2 = bl.h_res + 2 * ((size_t)i * nr + r);
+              Hc(r, i, j) += C(a2[0], a2[1]);
+              sum2[r] += a2[0] * a2[0] + a2[1] * a2[1];
+            }
+            wn2b[r] = bl.h_res[2 * ((size_t)(j + 1) * nr + r)];
+            hn[r] = wn2b[r] - sum2[r] > 0.0 ? std::sqrt(wn2b[r] - sum2[r]) : 0.0;
+            if (live[r] && wn2b[r] > 0.0 && (!(hn[r] > 0.0) || sum2[r] > 0.25 * wn2b[r]))
+              redo = true;
           }
-          bconvert<R, R>(bl, w_, vnext, sh_, scale.data());
+          if (redo) {
+            // strong cancellation for some rhs: form w'' and take its norms directly
+            bmultisub_dev<R>(dh2, V_, n, j + 1, w_, sh_, s);
+            bnorm2<R>(bl, w_, sh_, tmp.data());
+            for (int r = 0; r < nr; ++r) {
+              hn[r] = std::sqrt(tmp[r]);
+              scale[r] = hn[r] > 0.0 ? 1.0 / hn[r] : 0.0;
+            }
+            bconvert<R, R>(bl, w_, vnext, sh_, scale.data());
+          }
         }
         ++st.iters;
         ++j;

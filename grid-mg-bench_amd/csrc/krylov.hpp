@@ -1,15 +1,16 @@
 // Krylov solvers for the multigrid hierarchy: restarted (F)GMRES with
-// classical Gram-Schmidt plus one re-orthogonalisation (CGS2: two fused
-// multi-dot reductions per step), complex Givens rotations, and the true
-// residual recomputed at every restart.  Templated on the vector precision.
+// classical Gram-Schmidt and selective re-orthogonalisation, complex Givens
+// rotations, and the true residual recomputed at every restart.  Templated
+// on the vector precision.
 //
-// The projection coefficients stay in device memory: one Arnoldi step is
-// dot, subtract, dot (+ norm), fused subtract + normalise, all
-// stream-ordered, and the host reads the step's scalars once (one
-// synchronisation per step) for the Hessenberg column and its Givens rotation.  The new vector's norm comes
-// from Pythagoras, |w''|^2 = |w'|^2 - sum |h2_i|^2 (the second projection
-// only removes round-off); when that cancels by more than a quarter the norm
-// is recomputed directly.
+// One Arnoldi step: h1 = V^H w together with |w|^2 in one fused multi-dot,
+// read by the host (one synchronisation).  If |w'|^2 = |w|^2 - |h1|^2 is at
+// least half of |w|^2 one projection is enough and v = (w - V h1) / |w'| is
+// formed in one fused pass (the Daniel-Gragg-Kaufman-Stewart test with
+// threshold 1/sqrt(2)); otherwise the projection is repeated (CGS2) and the
+// new norm taken from Pythagoras, |w''|^2 = |w'|^2 - |h2|^2, or directly when
+// that cancels by more than a quarter.  The coefficients stay in device
+// memory between the passes.
 //
 // Reference use: the FGMRES outer solver and K-cycle wrapper, GMRES smoother
 // and coarsest-level solver of include/MultigridPreconditioner.h:253-309,
@@ -112,30 +113,46 @@ class Gmres {
         }
         A.apply(z, w_, s);
         ++st.applies;
-        // CGS2 with device-resident coefficients: h1 at d_h_[0..j], h2 and
-        // |w'|^2 at d_h_[j+1 .. 2j+2]
+        // Classical Gram-Schmidt with device-resident coefficients and
+        // selective re-orthogonalisation: h1 = V^H w and |w|^2 at
+        // d_h_[0 .. j+1]; the second projection (h2, |w'|^2 at
+        // d_h_[j+2 .. 2j+3]) only when |w'|^2 = |w|^2 - |h1|^2 < |w|^2 / 2.
         cplx<double>* dh1 = d_h_;
-        cplx<double>* dh2 = d_h_ + (j + 1);
+        cplx<double>* dh2 = d_h_ + (j + 2);
         cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
-        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, false, dh1);
-        vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);          // w' = w - V h1
-        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);   // h2 = V^H w', |w'|^2
-        vsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, n, s);    // v = (w' - V h2) / hn
-        fetch(bl, reinterpret_cast<const double*>(d_h_), 2 * (2 * j + 3));
-        const double* hr = bl.h_res;
-        double sum2 = 0.0;
+        vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh1);
+        fetch(bl, reinterpret_cast<const double*>(dh1), 2 * (j + 2));
+        double sum1 = 0.0;
         for (int i = 0; i <= j; ++i) {
-          const double ar = hr[2 * (j + 1 + i)], ai = hr[2 * (j + 1 + i) + 1];
-          h[i] = {hr[2 * i] + ar, hr[2 * i + 1] + ai};
-          sum2 += ar * ar + ai * ai;
+          h[i] = {bl.h_res[2 * i], bl.h_res[2 * i + 1]};
+          sum1 += h[i].re * h[i].re + h[i].im * h[i].im;
         }
-        const double wn2 = hr[2 * (2 * j + 2)];
-        double hn = hr[2 * (2 * j + 2) + 1];
-        if (!(hn > 0.0) || sum2 > 0.25 * wn2) {
-          // strong cancellation: form w'' = w' - V h2 and take its norm directly
-          vmultisub_dev<R>(dh2, V_, n, j + 1, w_, n, s);
-          hn = std::sqrt(vnorm2<R>(bl, w_, n));
-          if (hn > 0.0) vconvert<R, R>(w_, vnext, n, 1.0 / hn, s);
+        const double wn2 = bl.h_res[2 * (j + 1)];
+        double hn;
+        if (sum1 <= 0.5 * wn2) {
+          // one projection suffices: v = (w - V h1) / sqrt(|w|^2 - |h1|^2)
+          vsubnorm_dev<R>(dh1, V_, n, j + 1, w_, vnext, n, s);
+          hn = std::sqrt(wn2 - sum1);
+        } else {
+          vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);         // w' = w - V h1
+          vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);  // h2 = V^H w', |w'|^2
+          vsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, n, s);   // v = (w' - V h2) / hn
+          fetch(bl, reinterpret_cast<const double*>(dh2), 2 * (j + 2));
+          double sum2 = 0.0;
+          for (int i = 0; i <= j; ++i) {
+            const double ar = bl.h_res[2 * i], ai = bl.h_res[2 * i + 1];
+            h[i].re += ar;
+            h[i].im += ai;
+            sum2 += ar * ar + ai * ai;
+          }
+          const double wn2b = bl.h_res[2 * (j + 1)];
+          hn = wn2b - sum2 > 0.0 ? std::sqrt(wn2b - sum2) : 0.0;
+          if (!(hn > 0.0) || sum2 > 0.25 * wn2b) {
+            // strong cancellation: form w'' = w' - V h2 and take its norm directly
+            vmultisub_dev<R>(dh2, V_, n, j + 1, w_, n, s);
+            hn = std::sqrt(vnorm2<R>(bl, w_, n));
+            if (hn > 0.0) vconvert<R, R>(w_, vnext, n, 1.0 / hn, s);
+          }
         }
         for (int i = 0; i <= j; ++i) H[(size_t)i * m_ + j] = C(h[i].re, h[i].im);
         H[(size_t)(j + 1) * m_ + j] = hn;

@@ -224,6 +224,14 @@ __device__ __forceinline__ uint4 bprime(uint4 r, int u, bool neg) {
   return o;
 }
 
+// Order in which a workgroup visits the nine points: self, +x, -x, +y, -y,
+// +z, -z, +t, -t.  Workgroups of one XCD run nearly in step, so consecutive
+// steps read largely the same neighbour inputs (shifted by one lattice
+// spacing) while they are still in L2.
+__device__ __forceinline__ int mrhs_point(int q) {
+  return q == 0 ? 8 : ((q - 1) >> 1) + (((q - 1) & 1) ? 4 : 0);
+}
+
 // LDS-staged form for one site x.  Slot s of the staged B' operand
 // (kc = (s/64)/NT, nt = (s/64)%NT, lane = s%64) holds B'[k][n] for
 // k = kc*KC + (lane/16)*KCL + e (e < KCL), n = nt*16 + lane%16: the KCL/2
@@ -300,10 +308,11 @@ __device__ __forceinline__ void mrhs_lds_site(
   // A is single-buffered: the fragment for point p+1 is loaded into a[kc] as
   // soon as point p's MFMAs have consumed it; B' is staged one point ahead
 #pragma unroll
-  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, kc * 64);
-  issue_b(0);
+  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, (8 * NMT * NKC + kc) * 64);
+  issue_b(mrhs_point(0));
 #pragma unroll 1
-  for (int p = 0; p < 9; ++p) {
+  for (int q9 = 0; q9 < 9; ++q9) {
+    const int p = mrhs_point(q9);
     uint4 b[SPT];
 #pragma unroll
     for (int q = 0; q < SPT; ++q) b[q] = b_n[q];
@@ -312,8 +321,10 @@ __device__ __forceinline__ void mrhs_lds_site(
 #pragma unroll
     for (int q = 0; q < SPT; ++q) sB[threadIdx.x + q * THREADS] = bprime<KIND>(b[q], u, sneg[q]);
     __syncthreads();
-    if (p < 8) issue_b(p + 1);  // next point's input loads overlap this point's MFMAs
-    const int anext = ((p + 1) * NMT * NKC) * 64;
+    const bool more = q9 < 8;
+    const int pn = more ? mrhs_point(q9 + 1) : 0;
+    if (more) issue_b(pn);  // next point's input loads overlap this point's MFMAs
+    const int anext = (pn * NMT * NKC) * 64;
 
     if constexpr (SCALED) {
       floatx4 Tp[NT];
@@ -324,7 +335,7 @@ __device__ __forceinline__ void mrhs_lds_site(
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           Tp[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], Tp[nt]);
-        if (p < 8) a[kc] = src.load(lane_off, anext + kc * 64);
+        if (more) a[kc] = src.load(lane_off, anext + kc * 64);
       }
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
@@ -338,7 +349,7 @@ __device__ __forceinline__ void mrhs_lds_site(
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
-        if (p < 8) a[kc] = src.load(lane_off, anext + kc * 64);
+        if (more) a[kc] = src.load(lane_off, anext + kc * 64);
       }
     }
   }
@@ -422,7 +433,8 @@ __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
     for (int nt = 0; nt < NT; ++nt) C[mt][nt] = {0.f, 0.f, 0.f, 0.f};
 
 #pragma unroll 1
-  for (int p = 0; p < 9; ++p) {
+  for (int q9 = 0; q9 < 9; ++q9) {
+    const int p = mrhs_point(q9);
     const uint4* Ap = A + (((size_t)x * 9 + p) * NMT + w * MPW) * NKC * 64 + l;
     const T* base = in + (size_t)ys[p] * nrhs * N * 2;
     floatx4 Tp[MPW][NT];
@@ -596,9 +608,9 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
   fill_fac(0, cl, cx, ys);
   LinkSrc<LAUX> src(bt.A[cl] + (size_t)cx * SITE_FRAGS, SITE_FRAGS * 16, lane_off);
 #pragma unroll
-  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, kc * 64);
+  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, (8 * NMT * NKC + kc) * 64);
   const T* cin = reinterpret_cast<const T*>(bt.in[cl]);
-  issue_b(cin + (size_t)ys[0] * nrhs * N * 2);
+  issue_b(cin + (size_t)ys[8] * nrhs * N * 2);
 
   floatx4 C[NT];
   int nl = cl, nx = cx;
@@ -611,7 +623,8 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
     for (int nt = 0; nt < NT; ++nt) C[nt] = {0.f, 0.f, 0.f, 0.f};
 
 #pragma unroll 1
-    for (int p = 0; p < 9; ++p) {
+    for (int q9 = 0; q9 < 9; ++q9) {
+      const int p = mrhs_point(q9);
       uint4 b[SPT];
 #pragma unroll
       for (int q = 0; q < SPT; ++q) b[q] = b_n[q];
@@ -620,14 +633,15 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
       for (int q = 0; q < SPT; ++q)
         sB[threadIdx.x + q * THREADS] = bprime<KIND>(b[q], u, cdag && shi[q]);
       __syncthreads();
-      // where the next point's operands come from: this site's point p+1, or
-      // the next site's point 0
+      // where the next step's operands come from: this site's next point, or
+      // the next site's first point (the self point)
       const T* bnext = cin;
-      int ynext = 0, anext = 0;
+      int ynext = 0, anext = 8 * NMT * NKC * 64;
       bool more = true;
-      if (p < 8) {
-        ynext = ys[p + 1];
-        anext = (p + 1) * NMT * NKC * 64;
+      if (q9 < 8) {
+        const int pn = mrhs_point(q9 + 1);
+        ynext = ys[pn];
+        anext = pn * NMT * NKC * 64;
       } else if (has_next) {
         decode(nit, nl, nx);
         load_ys(nl, nx, nys);
@@ -635,7 +649,7 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
         src = LinkSrc<LAUX>(bt.A[nl] + (size_t)nx * SITE_FRAGS, SITE_FRAGS * 16, lane_off);
         cin = reinterpret_cast<const T*>(bt.in[nl]);
         bnext = cin;
-        ynext = nys[0];
+        ynext = nys[8];
       } else {
         more = false;
       }

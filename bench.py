@@ -92,6 +92,106 @@ def single_rhs_leg(mgc, stream, dims, N, prec, steps, warmup):
     return res
 
 
+def transfer_leg(mgc, stream, fine_dims, blk, nb, prec, steps, warmup):
+    """Aggregation project (R) and promote (P), one rhs: the block panel of the
+    basis [block][site][e][nb] is streamed once per call."""
+    Vf = int(np.prod(fine_dims))
+    Vc = Vf // int(np.prod(blk))
+    N = 2 * nb
+    wl = 16 if prec == mgc.F64 else 8
+    agg = mgc.Aggregation(fine_dims, blk, nb)
+    basis = mgc.DeviceBuffer(nb * Vf * 12 * 16)
+    mgc.random_gaussian(basis, nb * Vf * 12, mgc.F64, SEED, 11, stream)
+    agg.set_basis(basis, stream)
+    agg.orthogonalise(1, stream)
+    vf = mgc.DeviceBuffer(Vf * 12 * wl)
+    vc = mgc.DeviceBuffer(Vc * N * wl)
+    mgc.random_gaussian(vf, Vf * 12, prec, SEED, 12, stream)
+    mgc.random_gaussian(vc, Vc * N, prec, SEED, 13, stream)
+    stream.sync()
+    bytes_alg = Vf * 12 * nb * wl + Vf * 12 * wl + Vc * N * wl
+    res = {"workload": f"Aggregation {'x'.join(map(str, fine_dims))} blocks "
+                       f"{'x'.join(map(str, blk))} nbasis={N} ({nb} vectors x 2 chiralities) "
+                       f"{'fp64' if prec == mgc.F64 else 'fp32'}",
+           "bytes_per_call": bytes_alg}
+    for name, fn in (("project", lambda: agg.project(vf, vc, prec, 1, stream)),
+                     ("promote", lambda: agg.promote(vc, vf, prec, 1, stream))):
+        _, t_dev = Timer(mgc, stream).run(fn, steps, warmup)
+        res[name] = {"us": t_dev * 1e6, "GBps": bytes_alg / t_dev / 1e9,
+                     "hbm_frac": bytes_alg / t_dev / 1e9 / HBM_PEAK_GBS}
+    for b in (basis, vf, vc):
+        b.close()
+    agg.close()
+    return res
+
+
+def fine_op_leg(mgc, stream, dims, prec, nrhs, steps, warmup):
+    """Wilson-clover apply: gauge 4 links and packed hermitian clover (2 x 18
+    complex) per site read once, spinors in and out per rhs."""
+    V = int(np.prod(dims))
+    w = 16 if prec == mgc.F64 else 8
+    g = mgc.DeviceBuffer(4 * V * 9 * 16)
+    mgc.random_gauge(g, dims, 0.4, SEED, 21, stream)
+    fop = mgc.FineOp(dims, -0.1, 1.0)
+    fop.set_gauge(g, stream)
+    vi = mgc.DeviceBuffer(V * nrhs * 12 * w)
+    vo = mgc.DeviceBuffer(V * nrhs * 12 * w)
+    mgc.random_gaussian(vi, V * nrhs * 12, prec, SEED, 22, stream)
+    stream.sync()
+    _, t_dev = Timer(mgc, stream).run(lambda: fop.M(vi, vo, prec, nrhs, stream), steps, warmup)
+    bytes_alg = V * (36 * w + 36 * w + 8 * 4 + nrhs * 2 * 12 * w)
+    res = {"workload": f"Wilson-clover M {'x'.join(map(str, dims))} nrhs={nrhs} "
+                       f"{'fp64' if prec == mgc.F64 else 'fp32'}",
+           "kernel_us": t_dev * 1e6, "GBps": bytes_alg / t_dev / 1e9,
+           "hbm_frac": bytes_alg / t_dev / 1e9 / HBM_PEAK_GBS, "bytes_per_launch": bytes_alg}
+    for b in (g, vi, vo):
+        b.close()
+    fop.close()
+    return res
+
+
+def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
+    """CoarsenOperator (FP64, backward points + self, remaining links by
+    gamma5-hermiticity).  Contraction flops per coarse site: for each computed
+    point, every site of its face (the whole block for self) contributes
+    4 nb^2 link entries x 6 spin-colour terms x 8 flops."""
+    Vf = int(np.prod(fine_dims))
+    cd = tuple(f // b for f, b in zip(fine_dims, blk))
+    Vc = int(np.prod(cd))
+    Vb = int(np.prod(blk))
+    N = 2 * nb
+    g = mgc.DeviceBuffer(4 * Vf * 9 * 16)
+    mgc.random_gauge(g, fine_dims, 0.4, SEED, 31, stream)
+    fop = mgc.FineOp(fine_dims, -0.1, 1.0)
+    fop.set_gauge(g, stream)
+    agg = mgc.Aggregation(fine_dims, blk, nb)
+    basis = mgc.DeviceBuffer(nb * Vf * 12 * 16)
+    mgc.random_gaussian(basis, nb * Vf * 12, mgc.F64, SEED, 32, stream)
+    agg.set_basis(basis, stream)
+    agg.orthogonalise(2, stream)
+    cop = mgc.CoarseOp(cd, N, mgc.F64)
+    e0, e1 = mgc.Event(), mgc.Event()
+    mgc.coarsen_operator(cop, fop, agg, mgc.COARSEN_DIRSAVE, stream)  # warm-up
+    e0.record(stream)
+    for _ in range(reps):
+        mgc.coarsen_operator(cop, fop, agg, mgc.COARSEN_DIRSAVE, stream)
+    e1.record(stream)
+    e1.sync()
+    t = e0.elapsed_ms(e1) * 1e-3 / reps
+    face_sites = sum(Vb // b for b in blk) + Vb
+    flops = Vc * face_sites * 4 * nb * nb * 6 * 8
+    res = {"workload": f"CoarsenOperator {'x'.join(map(str, fine_dims))} -> "
+                       f"{'x'.join(map(str, cd))} nbasis={N} fp64 (backward points + self)",
+           "ms": t * 1e3, "flops": flops, "TFLOPs": flops / t / 1e12,
+           "fp64_frac": flops / t / 1e12 / PEAK_TFLOPS["f64"]}
+    for b in (g, basis):
+        b.close()
+    cop.close()
+    agg.close()
+    fop.close()
+    return res
+
+
 # HBM bytes moved per launch by the headline kernel (k_mrhs<F16, N=64, 6 column
 # tiles, non-temporal link loads>, the instantiation behind variants 1 and 4)
 # on the headline workload: two rocprofv3 --pmc runs, FETCH_SIZE then
@@ -302,6 +402,23 @@ def cpu_baseline(dims, N, nrhs, head):
     }
 
 
+def aux_legs(mgc, stream, legs, args):
+    """Single-rhs transfers and fine operator on the 32^4 -> 8^4 drop-in
+    workload, and the CoarsenOperator build on 32^4 with nbasis 64."""
+    f32 = (32, 32, 32, 32)
+    b4 = (4, 4, 4, 4)
+    for prec, nm in ((mgc.F64, "f64"), (mgc.F32, "f32")):
+        log(f"leg: project / promote 32^4 nbasis=32 {nm}")
+        legs[f"transfer_{nm}_n32"] = transfer_leg(mgc, stream, f32, b4, 16, prec, args.steps // 4,
+                                                  args.warmup)
+    for prec, nm, nrhs in ((mgc.F64, "f64", 1), (mgc.F32, "f32", 1), (mgc.F32, "f32", 12)):
+        log(f"leg: Wilson-clover 32^4 {nm} nrhs={nrhs}")
+        legs[f"wilson_{nm}_nrhs{nrhs}"] = fine_op_leg(mgc, stream, f32, prec, nrhs,
+                                                      args.steps // 4, args.warmup)
+    log("leg: CoarsenOperator 32^4 nbasis=64")
+    legs["coarsen_f64_n64"] = coarsen_leg(mgc, stream, f32, b4, 32)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -321,6 +438,8 @@ def main() -> int:
                          "(DIMS:NRHS; empty to skip)")
     ap.add_argument("--only-mg-batch", action="store_true",
                     help="run only the batched multigrid-solve leg and print it (for profiling)")
+    ap.add_argument("--only-aux", action="store_true",
+                    help="run only the transfer, fine-operator and CoarsenOperator legs")
     ap.add_argument("--only-mg", action="store_true",
                     help="run only the multigrid-solve leg and print its result (for profiling)")
     ap.add_argument("--mrhs-variant", type=int, default=-1,
@@ -371,6 +490,10 @@ def main() -> int:
                                              [16, 16])}), flush=True)
         return 0
     legs = {}
+    if args.only_aux:
+        aux_legs(mgc, stream, legs, args)
+        print(json.dumps({"legs": legs}), flush=True)
+        return 0
     dims8 = (8, 8, 8, 8)
     # headline: multi-rhs coarse stencil, 8^4 coarse lattice, nbasis 64, 48 rhs,
     # FP16 MFMA with FP32 accumulation (pre-quantised FP16 inputs, FP32 output)
@@ -395,6 +518,7 @@ def main() -> int:
                     continue
                 log(f"leg: multi-rhs {nm} nbasis=64 nrhs={nrhs}")
                 legs[key] = mrhs_leg(mgc, stream, dims8, 64, nrhs, mp, args.steps, args.warmup)
+        aux_legs(mgc, stream, legs, args)
         log("leg: multi-rhs f16 nbasis=64 nrhs=48, fp32 input quantised in the step")
         legs["M_mrhs48_f16_n64_f32in"] = mrhs_leg(mgc, stream, dims8, 64, 48, mgc.F16,
                                                   args.steps, args.warmup, half_input=False)

@@ -494,6 +494,168 @@ __global__ __launch_bounds__(N / 16 / MPW * 64) void k_mrhs_reg(
     }
 }
 
+// Direct-to-LDS form: two m-tiles per wavefront (N/32 wavefronts), the
+// neighbour's input copied unchanged into LDS by global_load_lds_dwordx4
+// (no registers, no per-slot transform), double-buffered so that one barrier
+// per point suffices, and B' built from the raw 16-byte chunk at read time
+// (lanes l and l^1 read the same chunk).  Each B' fragment read from LDS
+// feeds two MFMAs, so LDS reads per flop are a quarter of the LDS-staged
+// form's.  Raw layout per buffer: row j (input column rhs0 + j) holds the CPR
+// 16-byte chunks of its N complex entries plus one pad chunk, so the eight
+// rows a 16-lane group reads fall in different bank groups.  Columns j >= nr
+// are copied from row 0 of the same site (in bounds) and only ever reach
+// output columns that are not stored.  Arithmetic order per output element
+// is that of the other forms.
+template <int KIND, int N, int NT>
+struct MrhsDmaGeom {
+  using M = MK<KIND>;
+  static constexpr int WAVES = N / 32;
+  static constexpr int THREADS = WAVES * 64;
+  static constexpr int NCOL = NT * 8;
+  static constexpr int CPR = N * 2 * (int)sizeof(typename M::T) / 16;  // chunks per row
+  static constexpr int RSL = CPR + 1;                                    // slots per row
+  static constexpr int SLOTS = NCOL * RSL;
+  static constexpr int IPW = (SLOTS + THREADS - 1) / THREADS;  // copies per wavefront
+  static constexpr int BUF = IPW * THREADS;                    // slots per buffer
+};
+
+template <int KIND, int N, int NT, int LAUX>
+__global__ __launch_bounds__(N / 32 * 64) void k_mrhs_dma(
+    const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
+    const float* __restrict__ in_scale, float* __restrict__ out, const int* __restrict__ nbr,
+    int nrhs, int rhs0, int nr, int dag, int xchunk) {
+  using M = MK<KIND>;
+  using G = MrhsDmaGeom<KIND, N, NT>;
+  typedef typename M::T T;
+  constexpr int NMT = N / 16;
+  constexpr int NKC = 2 * N / M::KC;
+  constexpr bool SCALED = KIND != MGC_F32;
+  static_assert(G::NCOL <= G::THREADS, "one thread per column for the scale table");
+  __shared__ uint4 sraw[2][G::BUF];
+  __shared__ float s_fac[SCALED ? 9 : 1][G::NCOL];
+
+  const int x = site_of_block((int)blockIdx.x, xchunk);
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+  const int l = threadIdx.x & 63;
+  const int u = l & 1;
+  const char* in = reinterpret_cast<const char*>(in_);
+  constexpr size_t ROWB = (size_t)N * 2 * sizeof(T);  // bytes of one input column
+
+  int ys[9];
+#pragma unroll
+  for (int p = 0; p < 8; ++p) ys[p] = nbr[(size_t)x * 8 + p];
+  ys[8] = x;
+  if constexpr (SCALED) {
+    if ((int)threadIdx.x < G::NCOL) {
+      const int j = threadIdx.x;
+#pragma unroll
+      for (int p = 0; p < 9; ++p)
+        s_fac[p][j] =
+            j < nr ? ascale[(size_t)x * 9 + p] * in_scale[(size_t)ys[p] * nrhs + rhs0 + j] : 0.f;
+    }
+  }
+  // byte offset, within a site's input block, of the chunk this lane copies
+  // in its i-th copy (rows >= nr and pad slots copy row 0, chunk 0)
+  auto src_off = [&](int i) -> int {
+    const int slot = (i * G::WAVES + w) * 64 + l;
+    const int j = slot / G::RSL, c16 = slot - j * G::RSL;
+    return (j < nr && c16 < G::CPR) ? (int)((rhs0 + j) * ROWB) + c16 * 16 : (int)(rhs0 * ROWB);
+  };
+  auto issue = [&](int p, int bf) {
+    const char* site = in + (size_t)ys[p] * nrhs * ROWB;
+#pragma unroll
+    for (int i = 0; i < G::IPW; ++i)
+      __builtin_amdgcn_global_load_lds(
+          (const void*)(site + src_off(i)),
+          (__attribute__((address_space(3))) void*)&sraw[bf][(i * G::WAVES + w) * 64], 16, 0, 0);
+  };
+
+  // one link source per m-tile of this wavefront (lane positions lo0, lo1)
+  const int lo0 = (2 * w) * NKC * 64 + l, lo1 = (2 * w + 1) * NKC * 64 + l;
+  const uint4* site_links = A + (size_t)x * 9 * NMT * NKC * 64;
+  const LinkSrc<LAUX> src0(site_links, 9 * NMT * NKC * 64 * 16, lo0);
+  const LinkSrc<LAUX> src1(site_links, 9 * NMT * NKC * 64 * 16, lo1);
+  uint4 a0[NKC], a1[NKC];
+#pragma unroll
+  for (int kc = 0; kc < NKC; ++kc) {
+    a0[kc] = src0.load(lo0, (8 * NMT * NKC + kc) * 64);
+    a1[kc] = src1.load(lo1, (8 * NMT * NKC + kc) * 64);
+  }
+  issue(mrhs_point(0), 0);
+
+  floatx4 C0[NT], C1[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) C0[nt] = C1[nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+#pragma unroll 1
+  for (int q9 = 0; q9 < 9; ++q9) {
+    const int p = mrhs_point(q9);
+    const int bf = q9 & 1;
+    // this point's copies (and link fragments) have landed; every wavefront
+    // is done reading the other buffer
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    const bool more = q9 < 8;
+    const int pn = more ? mrhs_point(q9 + 1) : 0;
+    if (more) issue(pn, bf ^ 1);
+    const int anext = pn * NMT * NKC * 64;
+    floatx4 T0[NT], T1[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      T0[nt] = SCALED ? floatx4{0.f, 0.f, 0.f, 0.f} : C0[nt];
+      T1[nt] = SCALED ? floatx4{0.f, 0.f, 0.f, 0.f} : C1[nt];
+    }
+#pragma unroll
+    for (int kc = 0; kc < NKC; ++kc) {
+      const int chunk = kc * 4 + (l >> 4);
+      const int c0 = (kc * M::KC + (l >> 4) * M::KCL) >> 1;
+      const bool neg = dag && c0 >= N / 2;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        const int j = nt * 8 + ((l & 15) >> 1);
+        const uint4 b = bprime<KIND>(sraw[bf][j * G::RSL + chunk], u, neg);
+        T0[nt] = mfma_chunk<KIND>(a0[kc], b, T0[nt]);
+        T1[nt] = mfma_chunk<KIND>(a1[kc], b, T1[nt]);
+      }
+      if (more) {
+        a0[kc] = src0.load(lo0, anext + kc * 64);
+        a1[kc] = src1.load(lo1, anext + kc * 64);
+      }
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      if constexpr (SCALED) {
+        const float f = s_fac[p][(nt * 16 + (l & 15)) >> 1];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          C0[nt][i] = fmaf(f, T0[nt][i], C0[nt][i]);
+          C1[nt][i] = fmaf(f, T1[nt][i], C1[nt][i]);
+        }
+      } else {
+        C0[nt] = T0[nt];
+        C1[nt] = T1[nt];
+      }
+    }
+  }
+  // D[m][n]: lane l holds rows 4*(l/16)+i of each m-tile, column l%16
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int j = (nt * 16 + (l & 15)) >> 1;
+    if (j < nr) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int r = (2 * w + h) * 16 + 4 * (l >> 4) + i;
+          float v = h ? C1[nt][i] : C0[nt][i];
+          if (dag && r >= N / 2) v = -v;
+          out[(((size_t)x * nrhs + rhs0 + j) * N + r) * 2 + u] = v;
+        }
+      }
+    }
+  }
+}
+
 // Several coarse lattices of (possibly) different volume and one nbasis in a
 // single launch: block b belongs to the lattice whose site range
 // [first[l], first[l+1]) contains b.
@@ -712,11 +874,13 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
 // link loads, 2 = register B' with two m-tiles per wavefront, 3 and 4 =
 // variants 0 and 1 with the XCD-contiguous site order (identity order when
 // the site count is not a multiple of 8), 5 = persistent LDS-staged form,
-// 6 = persistent with non-temporal link loads and the XCD-contiguous order
-constexpr int MRHS_VARIANTS = 7;
-static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6) ? 2 : 0; }
+// 6 = persistent with non-temporal link loads and the XCD-contiguous order,
+// 7 = direct-to-LDS input with two m-tiles per wavefront, 8 = variant 7 with
+// non-temporal link loads and the XCD-contiguous order
+constexpr int MRHS_VARIANTS = 9;
+static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6 || v == 8) ? 2 : 0; }
 static int variant_xchunk(int v, int blocks) {
-  return ((v == 3 || v == 4 || v == 6) && blocks % 8 == 0) ? blocks / 8 : 0;
+  return ((v == 3 || v == 4 || v == 6 || v == 8) && blocks % 8 == 0) ? blocks / 8 : 0;
 }
 
 // persistent grid: every resident workgroup slot of the device, a multiple of
@@ -775,6 +939,15 @@ static void launch_variant(int variant, int V, const uint4* A, const float* as, 
   if (variant == 2) {
     k_mrhs_reg<KIND, N, NT, 2><<<g, N / 32 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr,
                                                         dag);
+    return;
+  }
+  if (variant == 7 || variant == 8) {
+    if (variant_laux(variant) == 2)
+      k_mrhs_dma<KIND, N, NT, 2><<<g, N / 32 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0,
+                                                          nr, dag, xc);
+    else
+      k_mrhs_dma<KIND, N, NT, 0><<<g, N / 32 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0,
+                                                          nr, dag, xc);
     return;
   }
   if (variant_laux(variant) == 2)
@@ -918,7 +1091,7 @@ static int g_batch_choice[4][2][9];  // [prec][N==64][NT] -> variant + 1 (0 = un
 
 static int tuned_batched(int mp, int N, int NT, int blocks, const MrhsBatch& bt, int nrhs,
                          int rhs0, int nr, hipStream_t s) {
-  if (g_mrhs_forced >= 0 && g_mrhs_forced != 2) return g_mrhs_forced;
+  if (g_mrhs_forced >= 0 && g_mrhs_forced != 2 && g_mrhs_forced < 7) return g_mrhs_forced;
   int& slot = g_batch_choice[mp][N == 64 ? 1 : 0][NT];
   if (slot) return slot - 1;
   const int v = BATCH_VARIANTS[time_variants(

@@ -19,6 +19,8 @@
 // workgroup builds W for both chiralities in LDS (items (l, c', colour)), stages
 // the block-panel rows of u, and every thread accumulates a TC x TC register
 // tile of one chirality block (k, l) of the N x N link in FP64.
+#include <type_traits>
+
 #include "aggregation.hpp"
 #include "coarse_op.hpp"
 #include "fine_op.hpp"
@@ -58,29 +60,35 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
   const int mu = p & 3;
   const int disp = p < 4 ? 1 : -1;
   const int nsite = (p == 8) ? Vb : Vb / blk.L[mu];
+  // face site f: the coordinates other than mu run lexicographically over the
+  // face, coordinate mu is the block's last (+mu) or first (-mu) layer
   auto face_site = [&](int f) {
     if (p == 8) return f;
-    int c[4];
-    int r = f;
+    int r = f, idx = 0, stride = 1;
+#pragma unroll
     for (int d = 0; d < 4; ++d) {
-      if (d == mu) continue;
-      c[d] = r % blk.L[d];
-      r /= blk.L[d];
+      int cd;
+      if (d == mu) {
+        cd = disp > 0 ? blk.L[d] - 1 : 0;
+      } else {
+        cd = r % blk.L[d];
+        r /= blk.L[d];
+      }
+      idx += cd * stride;
+      stride *= blk.L[d];
     }
-    c[mu] = disp > 0 ? blk.L[mu] - 1 : 0;
-    return blk.index(c);
+    return idx;
   };
 
   for (int s0 = 0; s0 < nsite; s0 += CO_SB) {
     // ---- stage u rows of the SB sites
     for (int idx = tid; idx < CO_SB * ROW; idx += THREADS) {
       const int s = idx / ROW, r = idx - s * ROW;
-      cplx<double> v = {0.0, 0.0};
-      if (s0 + s < nsite) {
-        const int xb = face_site(s0 + s);
-        v = basis[((size_t)xc * Vb + xb) * ROW + r];
-      }
-      (&sU[s][0][0])[r] = v;
+      // past the face end: load block site 0 (in bounds) and stage zero
+      const bool ok = s0 + s < nsite;
+      const int xb = ok ? face_site(s0 + s) : 0;
+      const cplx<double> v = basis[((size_t)xc * Vb + xb) * ROW + r];
+      (&sU[s][0][0])[r] = ok ? v : cplx<double>{0.0, 0.0};
     }
     // ---- W items (s, l', c', colour i)
     for (int it = tid; it < CO_SB * 2 * NB * 3; it += THREADS) {
@@ -90,18 +98,25 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
       r -= lw * NB * 3;
       const int cp = r / 3, ci = r - cp * 3;
       cplx<double> w[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};  // spins 0..3 at colour ci
-      if (s0 + s < nsite) {
+      // the body is instantiated per chirality LW so that every spin index is
+      // a compile-time constant (no indexed private arrays)
+      auto build_w = [&](auto lw_c) {
+        constexpr int LW = decltype(lw_c)::value;
         const int xb = face_site(s0 + s);
         const int x = blk_fine[(size_t)xc * Vb + xb];
-        int bc[4];
-        blk.coords(xb, bc);
         // hopping contributions
+#pragma unroll
         for (int q = 0; q < 8; ++q) {
           if (p != 8 && q != p) continue;
           const int qm = q & 3;
           // self: inner hops only, by the reference's coordinate mask
           // (x_mu mod b != b-1 for +mu, != 0 for -mu)
-          if (p == 8 && bc[qm] == (q < 4 ? blk.L[qm] - 1 : 0)) continue;
+          if (p == 8) {
+            int r = xb;
+#pragma unroll
+            for (int d = 0; d < qm; ++d) r /= blk.L[d];
+            if (r % blk.L[qm] == (q < 4 ? blk.L[qm] - 1 : 0)) continue;
+          }
           const int y = nbr[(size_t)x * 8 + q];
           const int ry = fine_row[y];
           const double sg = q < 4 ? 1.0 : -1.0;
@@ -118,7 +133,7 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
           const cplx<double>* uy = basis + (size_t)ry * ROW;
 #pragma unroll
           for (int sl = 0; sl < 2; ++sl) {
-            const int sp = 2 * lw + sl;
+            const int sp = 2 * LW + sl;
             cplx<double> chi = {0.0, 0.0};
 #pragma unroll
             for (int j = 0; j < 3; ++j) cmac(chi, L[j], uy[(sp * 3 + j) * NB + cp]);
@@ -134,17 +149,21 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
           }
         }
         if (p == 8) {  // diagonal (mass + clover) term, block-diagonal in chirality
-          const cplx<double>* D = diag + ((size_t)x * 2 + lw) * 36;
+          const cplx<double>* D = diag + ((size_t)x * 2 + LW) * 36;
           const cplx<double>* ux = basis + ((size_t)xc * Vb + xb) * ROW;
 #pragma unroll
           for (int sl = 0; sl < 2; ++sl) {
             cplx<double> a = {0.0, 0.0};
 #pragma unroll
-            for (int b = 0; b < 6; ++b) cmac(a, D[(sl * 3 + ci) * 6 + b], ux[(6 * lw + b) * NB + cp]);
-            w[2 * lw + sl].re += a.re;
-            w[2 * lw + sl].im += a.im;
+            for (int b = 0; b < 6; ++b) cmac(a, D[(sl * 3 + ci) * 6 + b], ux[(6 * LW + b) * NB + cp]);
+            w[2 * LW + sl].re += a.re;
+            w[2 * LW + sl].im += a.im;
           }
         }
+      };
+      if (s0 + s < nsite) {
+        if (lw == 0) build_w(std::integral_constant<int, 0>{});
+        else build_w(std::integral_constant<int, 1>{});
       }
 #pragma unroll
       for (int sp = 0; sp < 4; ++sp) sW[s][lw][sp * 3 + ci][cp] = w[sp];

@@ -191,16 +191,45 @@ __global__ void k_antiperiodic(cplx<double>* __restrict__ U, Lat4 lat) {
 // fastest, so a wavefront's link and clover loads are contiguous.  mask bit p < 8 selects
 // hopping point p, bit 8 the diagonal; dag applies g5 D g5 (= D^dag)
 // ---------------------------------------------------------------------------
+// Workgroup -> sites.  A workgroup covers a tile of S consecutive sites (all
+// nrhs of each).  Consecutive workgroups are dispatched to the eight XCDs in
+// turn, each with its own L2, so with xcd = 1 XCD k receives the time slices
+// [k*TL, (k+1)*TL) and walks them t fastest, then z within strips of ZS, then
+// the S-site groups g of an xy plane, then the z strips.  The t, z and y
+// neighbours a tile reads were then fetched into the same L2 by a workgroup
+// a few steps earlier (1, TL and TL*ZS steps).  With xcd = 0 the tiles are
+// simply consecutive (any lattice).
+struct WilsonTiles {
+  int S;       // sites per workgroup
+  int xcd;     // 1: XCD-local ordering below, 0: tile b = blockIdx
+  int G, TL, ZS, L2, P;  // groups per xy plane, t slices per XCD, z strip, L_z, L_x*L_y
+};
+
+__device__ __forceinline__ int wilson_tile_site(const WilsonTiles& w, int b) {
+  if (!w.xcd) return b * w.S;
+  const int i = b >> 3;
+  const int tl = i % w.TL;
+  int r = i / w.TL;
+  const int zi = r % w.ZS;
+  r /= w.ZS;
+  const int g = r % w.G;
+  const int zs = r / w.G;
+  const int t = (b & 7) * w.TL + tl;
+  const int z = zs * w.ZS + zi;
+  return g * w.S + w.P * (z + w.L2 * t);
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
                                                 const cplx<R>* __restrict__ diag,
                                                 const cplx<R>* __restrict__ in,
                                                 cplx<R>* __restrict__ out,
-                                                const int* __restrict__ nbr, int V, int nrhs,
-                                                unsigned mask, int dag) {
-  const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (t >= (size_t)V * nrhs) return;
-  const int x = (int)(t / nrhs), rr = (int)(t % nrhs);
+                                                const int* __restrict__ nbr, int V, int sv,
+                                                int nrhs, unsigned mask, int dag,
+                                                WilsonTiles wt) {
+  const int x = wilson_tile_site(wt, (int)blockIdx.x) + (int)threadIdx.x / nrhs;
+  const int rr = (int)threadIdx.x % nrhs;
+  if (x >= V) return;
   const GammaTab G = gamma_table();
   cplx<R> acc[12];
 #pragma unroll
@@ -213,13 +242,13 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
   if (mask & 0x100u) {
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
-      const cplx<R>* D = diag + (size_t)l * 18 * V + x;  // packed slot s at D[s * V]
+      const cplx<R>* D = diag + (size_t)l * 18 * sv + x;  // packed slot s at D[s * sv]
       cplx<R> v[6];
 #pragma unroll
       for (int b = 0; b < 6; ++b) v[b] = ld(x, 6 * l + b);
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
-        const cplx<R> dd = D[(size_t)a * V];  // (d_{2a}, d_{2a+1})
+        const cplx<R> dd = D[(size_t)a * sv];  // (d_{2a}, d_{2a+1})
         acc[6 * l + 2 * a].re += dd.re * v[2 * a].re;
         acc[6 * l + 2 * a].im += dd.re * v[2 * a].im;
         acc[6 * l + 2 * a + 1].re += dd.im * v[2 * a + 1].re;
@@ -230,7 +259,7 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
       for (int a = 0; a < 6; ++a)
 #pragma unroll
         for (int b = a + 1; b < 6; ++b, ++slot) {
-          const cplx<R> u = D[(size_t)slot * V];
+          const cplx<R> u = D[(size_t)slot * sv];
           cmac(acc[6 * l + a], u, v[b]);       // upper: D_ab v_b
           cmac_conj(acc[6 * l + b], u, v[a]);  // lower: conj(D_ab) v_a
         }
@@ -259,15 +288,15 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
         h[s][j] = {a.re - t.re, a.im - t.im};
       }
     }
-    const cplx<R>* u = U + (size_t)mu * 9 * V + (p < 4 ? x : y);  // element k at u[k * V]
+    const cplx<R>* u = U + (size_t)mu * 9 * sv + (p < 4 ? x : y);  // element k at u[k * sv]
     cplx<R> chi[2][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       cplx<R> L[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j)
-        L[j] = p < 4 ? u[(size_t)(i * 3 + j) * V]
-                     : cplx<R>{u[(size_t)(j * 3 + i) * V].re, -u[(size_t)(j * 3 + i) * V].im};
+        L[j] = p < 4 ? u[(size_t)(i * 3 + j) * sv]
+                     : cplx<R>{u[(size_t)(j * 3 + i) * sv].re, -u[(size_t)(j * 3 + i) * sv].im};
 #pragma unroll
       for (int s = 0; s < 2; ++s) {
         cplx<R> a = {R(0), R(0)};
@@ -299,22 +328,25 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
   }
 }
 
-// [4][V][9] -> [4][9][V] and [V][2][36] -> [2][36][V] (with precision change)
+// [4][V][9] -> [4][9][sv] and [V][2][36] -> [2][18][sv] (with precision change);
+// the site stride sv is V plus a pad
 template <typename Rd>
-__global__ void k_gauge_soa(const cplx<double>* __restrict__ g, cplx<Rd>* __restrict__ o, int V) {
+__global__ void k_gauge_soa(const cplx<double>* __restrict__ g, cplx<Rd>* __restrict__ o, int V,
+                            int sv) {
   const size_t n = (size_t)36 * V;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
        t += (size_t)gridDim.x * blockDim.x) {
     const size_t x = t % V, mk = t / V, mu = mk / 9, k = mk % 9;
     const cplx<double> v = g[(mu * V + x) * 9 + k];
-    o[t] = {(Rd)v.re, (Rd)v.im};
+    o[mk * sv + x] = {(Rd)v.re, (Rd)v.im};
   }
 }
 // packed hermitian blocks: [V][2][6][6] -> [2][18][V] (see fine_op.hpp); the
 // upper triangle is kept, the lower one is its conjugate by construction
 // (F = (Q - Q^dag)/(8i) and sigma_{mu nu} are hermitian)
 template <typename Rd>
-__global__ void k_diag_soa(const cplx<double>* __restrict__ d, cplx<Rd>* __restrict__ o, int V) {
+__global__ void k_diag_soa(const cplx<double>* __restrict__ d, cplx<Rd>* __restrict__ o, int V,
+                           int sv) {
   const size_t n = (size_t)36 * V;
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n;
        t += (size_t)gridDim.x * blockDim.x) {
@@ -332,7 +364,7 @@ __global__ void k_diag_soa(const cplx<double>* __restrict__ d, cplx<Rd>* __restr
       }
       v = D[a * 6 + a + 1 + q];
     }
-    o[t] = {(Rd)v.re, (Rd)v.im};
+    o[ls * sv + x] = {(Rd)v.re, (Rd)v.im};
   }
 }
 
@@ -340,8 +372,9 @@ __global__ void k_diag_soa(const cplx<double>* __restrict__ d, cplx<Rd>* __restr
 template <typename Rd>
 static void build_soa(mgc_fine_op* op, cplx<Rd>* gs, cplx<Rd>* ds, hipStream_t s) {
   k_gauge_soa<Rd><<<grid_for((size_t)36 * op->V, 256, 16384), 256, 0, s>>>(op->gauge64, gs,
-                                                                           op->V);
-  k_diag_soa<Rd><<<grid_for((size_t)36 * op->V, 256, 16384), 256, 0, s>>>(op->diag64, ds, op->V);
+                                                                           op->V, op->sv);
+  k_diag_soa<Rd><<<grid_for((size_t)36 * op->V, 256, 16384), 256, 0, s>>>(op->diag64, ds, op->V,
+                                                                          op->sv);
   MGC_LAUNCH_CHECK();
 }
 
@@ -354,25 +387,53 @@ static void free_fine(mgc_fine_op* op) {
   delete op;
 }
 
+// Tile geometry of the apply: S sites of nrhs threads each (at most 256
+// threads), XCD-local ordering where the lattice allows it.
+static WilsonTiles wilson_tiles(const Lat4& L, int nrhs) {
+  WilsonTiles w{};
+  const int P = L.L[0] * L.L[1];
+  int S = 1;
+  while (2 * S * nrhs <= 256 && P % (2 * S) == 0) S *= 2;
+  w.S = S;
+  w.P = P;
+  w.L2 = L.L[2];
+  w.xcd = (L.L[3] % 8 == 0 && P % S == 0) ? 1 : 0;
+  if (!w.xcd) {
+    // consecutive tiles of as many whole sites as fit 256 threads
+    w.S = 256 / nrhs;
+    return w;
+  }
+  w.G = P / S;
+  w.TL = L.L[3] / 8;
+  w.ZS = (L.L[2] % 4 == 0) ? 4 : (L.L[2] % 2 == 0 ? 2 : 1);
+  return w;
+}
+
 static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
                        unsigned mask, int dag, hipStream_t s) {
   MGC_CHECK(op, MGC_ERR_ARG, "null fine operator");
   MGC_CHECK(op->gauge_set, MGC_ERR_STATE, "gauge field not set");
   MGC_CHECK(in && out && in != out, MGC_ERR_ARG, "null or aliased vector");
   MGC_CHECK(nrhs >= 1, MGC_ERR_ARG, "nrhs < 1");
-  const size_t n = (size_t)op->V * nrhs;
-  const unsigned g = (unsigned)((n + 255) / 256);
+  MGC_CHECK(nrhs <= 256, MGC_ERR_ARG, "fine operator: nrhs > 256");
+  const WilsonTiles wt = wilson_tiles(op->lat, nrhs);
+  const unsigned g = (unsigned)((op->V + wt.S - 1) / wt.S);
+  const unsigned threads = (unsigned)(wt.S * nrhs);
   if (prec == MGC_F64) {
-    k_wilson<double><<<g, 256, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in,
-                                       (cplx<double>*)out, op->nbr, op->V, nrhs, mask, dag);
+    k_wilson<double><<<g, threads, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in,
+                                           (cplx<double>*)out, op->nbr, op->V, op->sv, nrhs,
+                                           mask, dag,
+                                           wt);
   } else if (prec == MGC_F32) {
     if (!op->gsoa32) {
-      MGC_HIP(hipMalloc(&op->gsoa32, (size_t)36 * op->V * sizeof(cplx<float>)));
-      MGC_HIP(hipMalloc(&op->dsoa32, (size_t)36 * op->V * sizeof(cplx<float>)));
+      MGC_HIP(hipMalloc(&op->gsoa32, (size_t)36 * op->sv * sizeof(cplx<float>)));
+      MGC_HIP(hipMalloc(&op->dsoa32, (size_t)36 * op->sv * sizeof(cplx<float>)));
       build_soa<float>(op, op->gsoa32, op->dsoa32, s);
     }
-    k_wilson<float><<<g, 256, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in,
-                                      (cplx<float>*)out, op->nbr, op->V, nrhs, mask, dag);
+    k_wilson<float><<<g, threads, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in,
+                                          (cplx<float>*)out, op->nbr, op->V, op->sv, nrhs,
+                                          mask, dag,
+                                          wt);
   } else {
     throw Error(MGC_ERR_PRECISION, "fine operator: F64 or F32");
   }
@@ -410,6 +471,7 @@ int mgc_fine_op_create(mgc_fine_op** out, const int dims[4], double mass, double
       op->lat.L[d] = dims[d];
     }
     op->V = op->lat.volume();
+    op->sv = op->V + 32;
     op->mass = mass;
     op->csw = csw;
     op->antiperiodic_t = antiperiodic_t ? 1 : 0;
@@ -420,8 +482,8 @@ int mgc_fine_op_create(mgc_fine_op** out, const int dims[4], double mass, double
     MGC_HIP(hipMemcpy(op->nbr, nb.data(), nb.size() * sizeof(int), hipMemcpyHostToDevice));
     MGC_HIP(hipMalloc(&op->gauge64, (size_t)4 * op->V * 9 * sizeof(cplx<double>)));
     MGC_HIP(hipMalloc(&op->diag64, (size_t)op->V * 72 * sizeof(cplx<double>)));
-    MGC_HIP(hipMalloc(&op->gsoa64, (size_t)36 * op->V * sizeof(cplx<double>)));
-    MGC_HIP(hipMalloc(&op->dsoa64, (size_t)36 * op->V * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&op->gsoa64, (size_t)36 * op->sv * sizeof(cplx<double>)));
+    MGC_HIP(hipMalloc(&op->dsoa64, (size_t)36 * op->sv * sizeof(cplx<double>)));
     *out = op;
   });
   if (st != MGC_OK) free_fine(op);

@@ -18,8 +18,10 @@ struct mgc_fine_op {
   mgc::cplx<double>* diag64 = nullptr;   // [V][2][6][6]
   // site-fastest copies read by the apply kernel, so that consecutive lanes
   // (consecutive sites) read consecutive addresses:
-  mgc::cplx<double>* gsoa64 = nullptr;   // [4][9][V]   (mu, row*3+col, site)
-  mgc::cplx<double>* dsoa64 = nullptr;   // [2][18][V]  (chirality, packed slot, site):
+  int sv = 0;  // site stride of the site-fastest copies: V + 32, so that the 36
+               // streams of one site do not all start in the same L2 set
+  mgc::cplx<double>* gsoa64 = nullptr;   // [4][9][sv]  (mu, row*3+col, site)
+  mgc::cplx<double>* dsoa64 = nullptr;   // [2][18][sv]  (chirality, packed slot, site):
                                          // each 6x6 diagonal block is hermitian; slots
                                          // 0..2 hold the real diagonal as pairs
                                          // (d0,d1) (d2,d3) (d4,d5), slots 3..17 the

@@ -202,9 +202,12 @@ int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const voi
  * link loads, 2 = register operand with two m-tiles per wavefront, 3 and 4 =
  * variants 0 and 1 with sites dealt to the eight XCDs in contiguous ranges,
  * 5 = persistent workgroups walking the sites, 6 = persistent with
+ * non-temporal link loads and the XCD-contiguous order, 7 = input copied
+ * straight to LDS with two m-tiles per wavefront, 8 = variant 7 with
  * non-temporal link loads and the XCD-contiguous order; the batched apply
- * uses every variant but 2); -1 restores autotuning.  get_variant reports the
- * variant in use (-1 = not tuned yet). */
+ * uses variants 0, 1 and 3 to 6 and tunes itself when another one is
+ * forced); -1 restores autotuning.  get_variant reports the variant in use
+ * (-1 = not tuned yet). */
 int mgc_mrhs_set_variant(int variant);
 int mgc_mrhs_get_variant(int mrhs_prec, int nbasis, int nrhs, int* variant);
 /* rows x n complex FP32 -> rows x n complex F16/BF16 with scales[row] =
@@ -279,6 +282,7 @@ int mgc_fine_op_destroy(mgc_fine_op* op);
 int mgc_fine_op_set_gauge(mgc_fine_op* op, const void* gauge_f64, void* stream);
 /* (4+m) + C(x) as [site][2 chiralities][6][6] complex double (device) */
 int mgc_fine_op_get_diag(const mgc_fine_op* op, void* out_f64, void* stream);
+/* vectors [site][rhs][spin 4][colour 3], prec MGC_F64 or MGC_F32, 1 <= nrhs <= 256 */
 int mgc_fine_op_M(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs, void* stream);
 int mgc_fine_op_Mdag(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
                      void* stream);

@@ -56,7 +56,7 @@ def test_mrhs_matches_oracle(gpu, dims, N, nrhs, mp):
         assert max(rel(ref[:, k], got_h[:, k]) for k in range(nrhs)) < TOL[mp](N)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 def test_mrhs_every_kernel_variant(gpu, variant, mp):
     """Each kernel variant the autotuner may pick is checked on its own."""
@@ -76,6 +76,32 @@ def test_mrhs_every_kernel_variant(gpu, variant, mp):
                 r = ref[:, k] if not dag else oracle.coarse_apply(
                     dims, N, links, np.ascontiguousarray(vq[:, k]), dag=1)
                 assert rel(r, got[:, k]) < TOL[mp](N), (dag, k)
+    finally:
+        gpu.call("mgc_mrhs_set_variant", -1)
+
+
+@pytest.mark.parametrize("variant", [7, 8])
+@pytest.mark.parametrize("mp", [F32, F16, BF16])
+@pytest.mark.parametrize("dims,N,nrhs", [((2, 2, 2, 2), 32, 70), ((2, 4, 2, 2), 64, 48),
+                                         ((4, 2, 2, 2), 64, 5)])
+def test_mrhs_direct_lds_variants_bit_equal(gpu, dims, N, nrhs, mp, variant):
+    """The direct-to-LDS forms keep the arithmetic order of the LDS-staged
+    form: outputs equal variant 0's bit for bit (FP16/BF16 inputs quantised,
+    column passes with rhs0 > 0 and partly filled tiles included)."""
+    print(f"[case] mrhs direct-lds dims={dims} N={N} nrhs={nrhs} prec={mp} v={variant}",
+          flush=True)
+    links = rand_links(dims, N, 651)
+    op = gpu.CoarseOp(dims, N, gpu.F64)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 652, nrhs=nrhs)
+    half = mp != F32
+    try:
+        for dag in (0, 1):
+            gpu.call("mgc_mrhs_set_variant", 0)
+            base = run_mrhs(gpu, op, v, nrhs, mp, dag=dag, half_input=half)
+            gpu.call("mgc_mrhs_set_variant", variant)
+            got = run_mrhs(gpu, op, v, nrhs, mp, dag=dag, half_input=half)
+            assert np.array_equal(base, got), (dag, float(np.abs(base - got).max()))
     finally:
         gpu.call("mgc_mrhs_set_variant", -1)
 

@@ -109,7 +109,8 @@ __device__ inline bool site_in_mask(const Lat4& blk, int xb, int kind, int p) {
 // ---------------------------------------------------------------------------
 constexpr int RC = 4;  // right-hand sides per workgroup
 
-template <typename R>
+// RCT = right-hand sides a workgroup actually holds (1 for a single rhs)
+template <typename R, int RCT>
 __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ f,
                                                  cplx<R>* __restrict__ c,
@@ -125,22 +126,54 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
   const int G = blockDim.x / nb;
   const int i = tid % nb, g = tid / nb;
   const int half = Ef / 2;
-  cplx<R> acc[RC][2];
+  cplx<R> acc[RCT][2];
 #pragma unroll
-  for (int r = 0; r < RC; ++r) acc[r][0] = acc[r][1] = {R(0), R(0)};
+  for (int r = 0; r < RCT; ++r) acc[r][0] = acc[r][1] = {R(0), R(0)};
   if (g < G) {
     const int rows = Vb * Ef;
     const cplx<R>* Ub = U + (size_t)xc * rows * nb;
     const int* bf = blk_fine + (size_t)xc * Vb;
     const int qbeg = blockIdx.z * rows_per, qend = min(rows, qbeg + rows_per);
-    for (int q = qbeg + g; q < qend; q += G) {
+    int q = qbeg + g;
+    if (kind == 0) {
+      // unmasked: PU rows per step, their loads issued together (rows are
+      // still accumulated in the order q, q + G, q + 2G, ...)
+      constexpr int PU = (sizeof(R) == 4 ? 8 : 4) * (RCT == 1 ? 2 : 1);
+      for (; q + (PU - 1) * G < qend; q += PU * G) {
+        cplx<R> uu[PU];
+        const cplx<R>* fp[PU];
+        bool lo[PU];
+#pragma unroll
+        for (int v = 0; v < PU; ++v) {
+          const int qq = q + v * G;
+          const int xb = qq / Ef, e = qq - xb * Ef;
+          lo[v] = e < half;
+          uu[v] = Ub[(size_t)qq * nb + i];
+          fp[v] = f + ((size_t)bf[xb] * nrhs + r0) * Ef + e;
+        }
+        cplx<R> xv[PU][RCT];
+#pragma unroll
+        for (int v = 0; v < PU; ++v)
+#pragma unroll
+          for (int r = 0; r < RCT; ++r) xv[v][r] = r < nr ? fp[v][(size_t)r * Ef] : cplx<R>{R(0), R(0)};
+#pragma unroll
+        for (int v = 0; v < PU; ++v)
+#pragma unroll
+          for (int r = 0; r < RCT; ++r)
+            if (r < nr) {
+              if (lo[v]) cmac_conj(acc[r][0], uu[v], xv[v][r]);
+              else cmac_conj(acc[r][1], uu[v], xv[v][r]);
+            }
+      }
+    }
+    for (; q < qend; q += G) {
       const int xb = q / Ef, e = q - xb * Ef;
       if (kind != 0 && !site_in_mask(blk, xb, kind, point)) continue;
       const int k = e >= half;
       const cplx<R> u = Ub[(size_t)q * nb + i];
       const cplx<R>* fr = f + ((size_t)bf[xb] * nrhs + r0) * Ef + e;
 #pragma unroll
-      for (int r = 0; r < RC; ++r)
+      for (int r = 0; r < RCT; ++r)
         if (r < nr) {
           const cplx<R> x = fr[(size_t)r * Ef];
           if (k == 0) cmac_conj(acc[r][0], u, x);
@@ -148,7 +181,7 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
         }
     }
 #pragma unroll
-    for (int r = 0; r < RC; ++r)
+    for (int r = 0; r < RCT; ++r)
       if (r < nr)
 #pragma unroll
         for (int k = 0; k < 2; ++k) s_red[(((size_t)g * RC + r) * 2 + k) * nb + i] = acc[r][k];
@@ -211,7 +244,7 @@ __global__ __launch_bounds__(256) void k_project_reduce(const cplx<R>* __restric
 // LPR adjacent lanes then sums one row, nb/LPR entries each, joined by lane
 // exchanges.  LPR = 2, or 4 where a 128-row tile would not fit in 64 KB.
 // ---------------------------------------------------------------------------
-template <typename R, int LPR>
+template <typename R, int LPR, int TPT>
 __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ c,
                                                  cplx<R>* __restrict__ f,
@@ -239,14 +272,33 @@ __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
   const int step_r = 256 / nb, step_i = 256 % nb;
   // this workgroup's row range (blockIdx.z splits the panel on small lattices)
   const int qbeg = blockIdx.z * rows_per, qend = min(rows, qbeg + rows_per);
+  // the tile's entries are loaded into registers one tile ahead, so the
+  // next tile's loads are in flight while the current one is computed;
+  // TPT >= ceil(PR * nb / 256) entries per thread
+  // (real and imaginary parts in separate arrays: kept in registers)
+  R pre_re[TPT], pre_im[TPT];
+  auto fetch = [&](int qt) __attribute__((always_inline)) {
+    const int nrow_t = min(PR, qend - qt);
+    const cplx<R>* src = Ub + (size_t)qt * nb;
+    // past the tile end the last entry is loaded again (never stored)
+    const int last = nrow_t * nb - 1;
+#pragma unroll
+    for (int k2 = 0; k2 < TPT; ++k2) {
+      const cplx<R> v = src[min(tid + 256 * k2, last)];
+      pre_re[k2] = v.re;
+      pre_im[k2] = v.im;
+    }
+  };
+  if (qbeg < qend) fetch(qbeg);
   for (int q0 = qbeg; q0 < qend; q0 += PR) {
     const int nrow = min(PR, qend - q0);
     __syncthreads();  // the previous tile has been consumed (and s_c is visible)
     {
-      const cplx<R>* src = Ub + (size_t)q0 * nb;  // rows q0 .. q0+nrow-1
       int rl = tid / nb, i = tid - rl * nb;
-      for (int t = tid; t < nrow * nb; t += 256) {
-        s_u[i * LD + rl] = src[t];
+#pragma unroll
+      for (int k2 = 0; k2 < TPT; ++k2) {
+        const int t = tid + 256 * k2;
+        if (t < nrow * nb) s_u[i * LD + rl] = cplx<R>{pre_re[k2], pre_im[k2]};
         rl += step_r;
         i += step_i;
         if (i >= nb) {
@@ -256,6 +308,9 @@ __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
       }
     }
     __syncthreads();
+    // the next tile (after the last one, the current tile again: L2 hits,
+    // never stored; keeps the prefetch unconditional)
+    fetch(q0 + PR < qend ? q0 + PR : q0);
     cplx<R> acc[RC];
 #pragma unroll
     for (int r = 0; r < RC; ++r) acc[r] = {R(0), R(0)};
@@ -441,19 +496,32 @@ void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int n
     const int nred = a->Vc * nchunk;  // reduction workgroups
     const size_t red_sh = (size_t)256 * es;  // ZG * (RC*2*nb) <= 256 entries
     if (prec == MGC_F64) {
-      k_project<double><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
-                                                (cplx<double>*)out, a->blk_fine, nb, a->Vb, a->Ef,
-                                                nrhs, a->blk, kind, point, rows_per,
-                                                (cplx<double>*)part);
+      if (nrhs == 1)
+        k_project<double, 1><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
+                                                     (cplx<double>*)out, a->blk_fine, nb, a->Vb,
+                                                     a->Ef, nrhs, a->blk, kind, point, rows_per,
+                                                     (cplx<double>*)part);
+      else
+        k_project<double, RC><<<grid, block, sh, s>>>(a->basis64, (const cplx<double>*)in,
+                                                      (cplx<double>*)out, a->blk_fine, nb, a->Vb,
+                                                      a->Ef, nrhs, a->blk, kind, point, rows_per,
+                                                      (cplx<double>*)part);
       if (S > 1)
         k_project_reduce<double><<<nred, 256, red_sh, s>>>((const cplx<double>*)part,
                                                             (cplx<double>*)out, nchunk, S, nrhs,
                                                             nb);
     } else {
-      k_project<float><<<grid, block, sh, s>>>(agg_basis32(a, s), (const cplx<float>*)in,
-                                               (cplx<float>*)out, a->blk_fine, nb, a->Vb, a->Ef,
-                                               nrhs, a->blk, kind, point, rows_per,
-                                               (cplx<float>*)part);
+      const cplx<float>* b32 = agg_basis32(a, s);
+      if (nrhs == 1)
+        k_project<float, 1><<<grid, block, sh, s>>>(b32, (const cplx<float>*)in,
+                                                    (cplx<float>*)out, a->blk_fine, nb, a->Vb,
+                                                    a->Ef, nrhs, a->blk, kind, point, rows_per,
+                                                    (cplx<float>*)part);
+      else
+        k_project<float, RC><<<grid, block, sh, s>>>(b32, (const cplx<float>*)in,
+                                                     (cplx<float>*)out, a->blk_fine, nb, a->Vb,
+                                                     a->Ef, nrhs, a->blk, kind, point, rows_per,
+                                                     (cplx<float>*)part);
       if (S > 1)
         k_project_reduce<float><<<nred, 256, red_sh, s>>>((const cplx<float>*)part,
                                                            (cplx<float>*)out, nchunk, S, nrhs,
@@ -464,19 +532,37 @@ void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int n
     const int lpr = lds(2) <= 65536 ? 2 : 4;
     const size_t sh = lds(lpr);
     MGC_CHECK(sh <= 65536, MGC_ERR_SHAPE, "promote: nbasis too large for the LDS tile");
-#define PROMOTE(R, LPR, basis)                                                                \
-  k_promote<R, LPR><<<grid, 256, sh, s>>>(basis, (const cplx<R>*)in, (cplx<R>*)out, a->blk_fine, \
-                                          nb, a->Vb, a->Ef, nrhs, rows_per)
+    // register prefetch depth: entries per thread of one tile, rounded up to a power of 2
+    const int need = (256 / lpr * nb + 255) / 256;
+    const int tpt = need <= 2 ? 2 : need <= 4 ? 4 : need <= 8 ? 8 : need <= 16 ? 16 : 32;
+#define PROMOTE_T(R, LPR, T, basis)                                                            \
+  k_promote<R, LPR, T><<<grid, 256, sh, s>>>(basis, (const cplx<R>*)in, (cplx<R>*)out,          \
+                                             a->blk_fine, nb, a->Vb, a->Ef, nrhs, rows_per)
+#define PROMOTE(R, LPR, basis)                           \
+  switch (tpt) {                                         \
+    case 2: PROMOTE_T(R, LPR, 2, basis); break;          \
+    case 4: PROMOTE_T(R, LPR, 4, basis); break;          \
+    case 8: PROMOTE_T(R, LPR, 8, basis); break;          \
+    case 16: PROMOTE_T(R, LPR, 16, basis); break;        \
+    default: PROMOTE_T(R, LPR, 32, basis); break;        \
+  }
     if (prec == MGC_F64) {
-      if (lpr == 2) PROMOTE(double, 2, a->basis64);
-      else PROMOTE(double, 4, a->basis64);
+      if (lpr == 2) {
+        PROMOTE(double, 2, a->basis64)
+      } else {
+        PROMOTE(double, 4, a->basis64)
+      }
     } else if (prec == MGC_F32) {
-      if (lpr == 2) PROMOTE(float, 2, agg_basis32(a, s));
-      else PROMOTE(float, 4, agg_basis32(a, s));
+      if (lpr == 2) {
+        PROMOTE(float, 2, agg_basis32(a, s))
+      } else {
+        PROMOTE(float, 4, agg_basis32(a, s))
+      }
     } else {
       throw Error(MGC_ERR_PRECISION, "promote: F64 or F32 vectors");
     }
 #undef PROMOTE
+#undef PROMOTE_T
   }
   MGC_LAUNCH_CHECK();
 }

@@ -29,12 +29,26 @@ namespace mgc {
 
 constexpr int CO_SB = 2;  // fine sites per stage
 
+// Workgroup b -> (coarse site, point).  The points of one coarse site are
+// consecutive in the dispatch sequence of one XCD (b % 8 is the XCD), so the
+// workgroups that read one block panel run together on one L2; with a coarse
+// volume that is not a multiple of 8 they are simply consecutive.
+__device__ __forceinline__ void coarsen_work(int b, int np, int Vc, int& xc, int& pi) {
+  if (Vc % 8 == 0) {
+    const int i = b >> 3;
+    xc = (i / np) * 8 + (b & 7);
+    pi = i % np;
+  } else {
+    xc = b / np;
+    pi = b % np;
+  }
+}
+
 template <int NB, int TC>
 __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
-    const cplx<double>* __restrict__ basis, const int* __restrict__ blk_fine,
-    const int* __restrict__ fine_row, const cplx<double>* __restrict__ gauge,
-    const cplx<double>* __restrict__ diag, const int* __restrict__ nbr, Lat4 blk, int Vf, int Vb,
-    const int* __restrict__ plist, int Vc, cplx<double>* __restrict__ links) {
+    const cplx<double>* __restrict__ basis, const cplx<double>* __restrict__ gauge,
+    const cplx<double>* __restrict__ diag, Lat4 fine, Lat4 coarse, Lat4 blk, int Vf, int Vb,
+    const int* __restrict__ plist, int np, int Vc, cplx<double>* __restrict__ links) {
   constexpr int NT1 = NB / TC;
   constexpr int THREADS = 4 * NT1 * NT1;
   constexpr int ROW = 12 * NB;  // complex per panel row group (one fine site)
@@ -42,8 +56,16 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
   __shared__ cplx<double> sW[CO_SB][2][12][NB];
   const GammaTab G = gamma_table();
 
-  const int xc = blockIdx.x;
-  const int p = plist[blockIdx.y];
+  int xc, pi;
+  coarsen_work((int)blockIdx.x, np, Vc, xc, pi);
+  const int p = plist[pi];
+  // coarse coordinates of this block, and lexicographic strides
+  int cc[4];
+  coarse.coords(xc, cc);
+  const int bs[4] = {1, blk.L[0], blk.L[0] * blk.L[1], blk.L[0] * blk.L[1] * blk.L[2]};
+  const int fs[4] = {1, fine.L[0], fine.L[0] * fine.L[1], fine.L[0] * fine.L[1] * fine.L[2]};
+  const int cs[4] = {1, coarse.L[0], coarse.L[0] * coarse.L[1],
+                     coarse.L[0] * coarse.L[1] * coarse.L[2]};
   const int tid = threadIdx.x;
   const int kl = tid / (NT1 * NT1);
   const int k = kl >> 1, l = kl & 1;
@@ -103,22 +125,42 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
       auto build_w = [&](auto lw_c) {
         constexpr int LW = decltype(lw_c)::value;
         const int xb = face_site(s0 + s);
-        const int x = blk_fine[(size_t)xc * Vb + xb];
-        // hopping contributions
+        // block and fine coordinates of this site
+        int bc[4], x = 0;
+        {
+          int r = xb;
+#pragma unroll
+          for (int d = 0; d < 4; ++d) {
+            bc[d] = r % blk.L[d];
+            r /= blk.L[d];
+            x += (cc[d] * blk.L[d] + bc[d]) * fs[d];
+          }
+        }
+        // hopping contributions; unrolled so that the spin indices below are
+        // compile-time constants (no indexed private arrays)
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           if (p != 8 && q != p) continue;
           const int qm = q & 3;
+          const int disp = q < 4 ? 1 : -1;
+          const bool inner = (unsigned)(bc[qm] + disp) < (unsigned)blk.L[qm];
           // self: inner hops only, by the reference's coordinate mask
           // (x_mu mod b != b-1 for +mu, != 0 for -mu)
-          if (p == 8) {
-            int r = xb;
-#pragma unroll
-            for (int d = 0; d < qm; ++d) r /= blk.L[d];
-            if (r % blk.L[qm] == (q < 4 ? blk.L[qm] - 1 : 0)) continue;
+          if (p == 8 && !inner) continue;
+          // fine neighbour y = x + disp*mu (periodic) and its panel row
+          const int fq = cc[qm] * blk.L[qm] + bc[qm] + disp;
+          const int y = x + disp * fs[qm] -
+                        (fq < 0 ? -fine.L[qm] * fs[qm] : fq >= fine.L[qm] ? fine.L[qm] * fs[qm] : 0);
+          int ry;
+          if (inner) {
+            ry = xc * Vb + xb + disp * bs[qm];
+          } else {
+            const int cq = cc[qm] + disp;
+            const int xn = xc + disp * cs[qm] -
+                           (cq < 0 ? -coarse.L[qm] * cs[qm]
+                                   : cq >= coarse.L[qm] ? coarse.L[qm] * cs[qm] : 0);
+            ry = xn * Vb + xb - disp * (blk.L[qm] - 1) * bs[qm];
           }
-          const int y = nbr[(size_t)x * 8 + q];
-          const int ry = fine_row[y];
           const double sg = q < 4 ? 1.0 : -1.0;
           cplx<double> L[3];  // row ci of the link matrix
           if (q < 4) {
@@ -227,10 +269,9 @@ template <int NB, int TC>
 static void launch_coarsen_t(const mgc_aggregation* agg, const mgc_fine_op* fop, const int* plist,
                              int np, cplx<double>* links, hipStream_t s) {
   constexpr int THREADS = 4 * (NB / TC) * (NB / TC);
-  const dim3 g(agg->Vc, np);
-  k_coarsen<NB, TC><<<g, THREADS, 0, s>>>(agg->basis64, agg->blk_fine, agg->fine_row,
-                                          fop->gauge64, fop->diag64, fop->nbr, agg->blk, agg->Vf,
-                                          agg->Vb, plist, agg->Vc, links);
+  k_coarsen<NB, TC><<<agg->Vc * np, THREADS, 0, s>>>(agg->basis64, fop->gauge64, fop->diag64,
+                                                      agg->fine, agg->coarse, agg->blk, agg->Vf,
+                                                      agg->Vb, plist, np, agg->Vc, links);
   MGC_LAUNCH_CHECK();
 }
 

@@ -259,6 +259,96 @@ static bool pairable(const void* a, const void* b, size_t ldv, size_t n) {
   return n % 2 == 0 && ldv % 2 == 0 && ((uintptr_t)a % 16) == 0 && ((uintptr_t)b % 16) == 0;
 }
 
+// Fused re-orthogonalisation pass: w <- w - sum_j c[j] V_j (stored in the
+// vector precision), and the partial sums of <V_j, w> for j < k and of |w|^2
+// (row k) of the updated w, as k_vmultisub followed by k_mdot_part /
+// k_mdot_part_f2 would produce them (same grid, same per-thread element
+// order and arithmetic, so the same bits), reading V once from HBM (the
+// second read of V_j by the same thread is an L2 hit).  KT >= k.
+template <typename R, int KT, bool PAIRED>
+__global__ __launch_bounds__(256) void k_msub_mdot(const cplx<double>* __restrict__ c,
+                                                   const cplx<R>* V, size_t ldv, int k,
+                                                   cplx<R>* w, size_t n, double* part) {
+  __shared__ cplx<double> s_c[KT];
+  for (int j = threadIdx.x; j < k; j += blockDim.x) s_c[j] = c[j];
+  __syncthreads();
+  double re[KT + 1], im[KT];
+#pragma unroll
+  for (int j = 0; j <= KT; ++j) re[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < KT; ++j) im[j] = 0.0;
+  if constexpr (!PAIRED) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x) {
+      double xr = w[i].re, xi = w[i].im;
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const cplx<R> v = V[(size_t)j * ldv + i];
+          xr -= s_c[j].re * v.re - s_c[j].im * v.im;
+          xi -= s_c[j].re * v.im + s_c[j].im * v.re;
+        }
+      const cplx<R> b = {(R)xr, (R)xi};
+      w[i] = b;
+      // re-read V_j below instead of holding k loaded values in registers
+      asm volatile("" ::: "memory");
+      const double br = b.re, bi = b.im;
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const cplx<R> a = V[(size_t)j * ldv + i];
+          const double ar = a.re, ai = a.im;
+          re[j] += ar * br + ai * bi;
+          im[j] += ar * bi - ai * br;
+        }
+      re[KT] += br * br + bi * bi;
+    }
+  } else {
+    // FP32 pairs: elements 2 i2 and 2 i2 + 1 (one 16-byte load per row)
+    const size_t n2 = n / 2;
+    for (size_t i2 = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i2 < n2;
+         i2 += (size_t)gridDim.x * blockDim.x) {
+      const float4 wv = reinterpret_cast<const float4*>(w)[i2];
+      double x0r = wv.x, x0i = wv.y, x1r = wv.z, x1i = wv.w;
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const float4 v = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          x0r -= s_c[j].re * v.x - s_c[j].im * v.y;
+          x0i -= s_c[j].re * v.y + s_c[j].im * v.x;
+          x1r -= s_c[j].re * v.z - s_c[j].im * v.w;
+          x1i -= s_c[j].re * v.w + s_c[j].im * v.z;
+        }
+      const float4 b = make_float4((float)x0r, (float)x0i, (float)x1r, (float)x1i);
+      reinterpret_cast<float4*>(w)[i2] = b;
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const float4 a = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          re[j] += (double)a.x * b.x + (double)a.y * b.y;
+          im[j] += (double)a.x * b.y - (double)a.y * b.x;
+          re[j] += (double)a.z * b.z + (double)a.w * b.w;
+          im[j] += (double)a.z * b.w - (double)a.w * b.z;
+        }
+      re[KT] += (double)b.x * b.x + (double)b.y * b.y;
+      re[KT] += (double)b.z * b.z + (double)b.w * b.w;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j <= KT; ++j) {
+    if (j < k || j == KT) {  // uniform over the block
+      const int row = j < k ? j : k;
+      const double2 t = block_sum2(re[j], j < KT ? im[j] : 0.0);
+      if (threadIdx.x == 0) {
+        part[((size_t)row * gridDim.x + blockIdx.x) * 2] = t.x;
+        part[((size_t)row * gridDim.x + blockIdx.x) * 2 + 1] = t.y;
+      }
+      __syncthreads();  // block_sum2's LDS is reused by the next row
+    }
+  }
+}
+
 // v = (w - sum_j h2[j] V_j) / hn, hn = sqrt(h2[k].re - sum_j |h2[j]|^2)
 template <typename R>
 __global__ __launch_bounds__(256) void k_subnorm(cplx<double>* h2, const cplx<R>* __restrict__ V,
@@ -420,6 +510,35 @@ template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, si
   k_subnorm<R><<<ew_grid(n), 256, 0, s>>>(d_h2, V, ldv, k, w, v, n);
   MGC_LAUNCH_CHECK();
 }
+template <typename R, int KT>
+static void launch_msub_mdot(Blas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv,
+                             int k, cplx<R>* w, size_t n, int np) {
+  const dim3 g(np);
+  if constexpr (std::is_same<R, float>::value) {
+    if (pairable(V, w, ldv, n)) {
+      k_msub_mdot<R, KT, true><<<g, 256, 0, b.s>>>(d_c, V, ldv, k, w, n, b.d_part);
+      return;
+    }
+  }
+  k_msub_mdot<R, KT, false><<<g, 256, 0, b.s>>>(d_c, V, ldv, k, w, n, b.d_part);
+}
+
+template <typename R> bool vmultisub_dot_dev(Blas& b, const cplx<double>* d_c, const cplx<R>* V,
+                                             size_t ldv, int k, cplx<R>* w, size_t n,
+                                             cplx<double>* d_h) {
+  if (k < 1 || k > 24) return false;
+  // the grid of vmultidot_dev, so that the partial sums are the same
+  const bool paired = std::is_same<R, float>::value && pairable(V, w, ldv, n);
+  const int np = nparts_for(paired ? n / 2 : n);
+  if (k <= 8) launch_msub_mdot<R, 8>(b, d_c, V, ldv, k, w, n, np);
+  else if (k <= 16) launch_msub_mdot<R, 16>(b, d_c, V, ldv, k, w, n, np);
+  else launch_msub_mdot<R, 24>(b, d_c, V, ldv, k, w, n, np);
+  MGC_LAUNCH_CHECK();
+  k_dot_final<<<k + 1, 256, 0, b.s>>>(b.d_part, np, reinterpret_cast<double*>(d_h));
+  MGC_LAUNCH_CHECK();
+  return true;
+}
+
 void fetch(Blas& b, const double* d_src, int count) {
   MGC_CHECK(count >= 0 && count <= 4 * (Blas::KMAX + 1) * 2, MGC_ERR_ARG, "fetch too large");
   MGC_HIP(hipMemcpyAsync(b.h_res, d_src, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
@@ -447,7 +566,9 @@ void fetch(Blas& b, const double* d_src, int count) {
   template void vnormalize_dev<R>(const cplx<R>*, cplx<R>*, size_t, cplx<double>*, int,      \
                                   hipStream_t);                                              \
   template void vsubnorm_dev<R>(cplx<double>*, const cplx<R>*, size_t, int, const cplx<R>*,   \
-                                cplx<R>*, size_t, hipStream_t);
+                                cplx<R>*, size_t, hipStream_t);                              \
+  template bool vmultisub_dot_dev<R>(Blas&, const cplx<double>*, const cplx<R>*, size_t, int,  \
+                                     cplx<R>*, size_t, cplx<double>*);
 INST(double)
 INST(float)
 #undef INST

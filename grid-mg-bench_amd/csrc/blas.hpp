@@ -66,6 +66,13 @@ template <typename R> void vnormalize_dev(const cplx<R>* w, cplx<R>* v, size_t n
 // w is left unchanged
 template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, size_t ldv, int k,
                                         const cplx<R>* w, cplx<R>* v, size_t n, hipStream_t s);
+// Fused re-orthogonalisation pass: w -= sum_j d_c[j] V_j, then d_h[j] =
+// <V_j, w> for j < k and d_h[k].re = |w|^2 of the updated w, with the same
+// results as vmultisub_dev followed by vmultidot_dev(..., true, d_h) but one
+// read of V from memory.  Returns false (and does nothing) for k > 24.
+template <typename R> bool vmultisub_dot_dev(Blas& b, const cplx<double>* d_c, const cplx<R>* V,
+                                             size_t ldv, int k, cplx<R>* w, size_t n,
+                                             cplx<double>* d_h);
 // copy count doubles from device to the pinned h_res and wait for them
 void fetch(Blas& b, const double* d_src, int count);
 

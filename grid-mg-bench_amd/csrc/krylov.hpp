@@ -134,8 +134,11 @@ class Gmres {
           vsubnorm_dev<R>(dh1, V_, n, j + 1, w_, vnext, n, s);
           hn = std::sqrt(wn2 - sum1);
         } else {
-          vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);         // w' = w - V h1
-          vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);  // h2 = V^H w', |w'|^2
+          // w' = w - V h1, then h2 = V^H w' and |w'|^2 (one fused pass when j < 24)
+          if (!vmultisub_dot_dev<R>(bl, dh1, V_, n, j + 1, w_, n, dh2)) {
+            vmultisub_dev<R>(dh1, V_, n, j + 1, w_, n, s);
+            vmultidot_dev<R>(bl, V_, n, j + 1, w_, n, true, dh2);
+          }
           vsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, n, s);   // v = (w' - V h2) / hn
           fetch(bl, reinterpret_cast<const double*>(dh2), 2 * (j + 2));
           double sum2 = 0.0;

@@ -344,42 +344,49 @@ __global__ __launch_bounds__(256) void k_promote(const cplx<R>* __restrict__ U,
 }
 
 // ---------------------------------------------------------------------------
-// block modified Gram-Schmidt, FP64.  One workgroup per (coarse site,
-// chirality); thread t owns panel rows t, t+T, ... (at most RPT rows) and
-// only ever touches its own rows, so the only communication is the
-// workgroup-wide reductions of the inner products and norms.
+// block Gram-Schmidt, FP64.  One workgroup per (coarse site, chirality);
+// thread t owns panel rows t, t+T, ... (at most RPT rows) and only ever
+// touches its own rows, so the only communication is workgroup-wide
+// reductions.  Each pass sweeps the columns v = 0 .. nb-1: all inner
+// products <u, v> with the finished columns u < v are reduced together
+// (classical Gram-Schmidt, one reduction of v values instead of v scalar
+// reductions), v -= sum_u <u, v> u, and the projection is repeated when the
+// norm fell below 1/sqrt(2) of its value before it (the
+// Daniel-Gragg-Kaufman-Stewart test); then v /= |v|.  For a panel of full
+// rank this gives the Q factor of the modified Gram-Schmidt sweep of the
+// reference (the unique one with a positive real R diagonal) to rounding,
+// with at least its orthogonality.
 // ---------------------------------------------------------------------------
 constexpr int MGS_THREADS = 512;
 constexpr int MGS_RPT = 8;
+constexpr int MGS_NBMAX = 64;
 
-__device__ inline double2 wg_sum2(double a, double b, double* s_buf) {
+// per-wavefront sums of (a, b) for each of n values, lane 0 of wave w
+// writing s_part[w][i]; the caller syncs and adds the wavefronts
+__device__ inline void wave_sum2(double a, double b, double* out) {
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off);
     b += __shfl_xor(b, off);
   }
-  const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    s_buf[2 * w] = a;
-    s_buf[2 * w + 1] = b;
+    out[0] = a;
+    out[1] = b;
   }
-  __syncthreads();
-  double ra = 0, rb = 0;
-  for (int k = 0; k < nw; ++k) {
-    ra += s_buf[2 * k];
-    rb += s_buf[2 * k + 1];
-  }
-  __syncthreads();
-  return make_double2(ra, rb);
 }
 
 __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __restrict__ U,
                                                                 int nb, int Vb, int Ef,
                                                                 int passes) {
-  __shared__ double s_buf[2 * (MGS_THREADS / 64)];
+  constexpr int NW = MGS_THREADS / 64;
+  // s_part[w][u][2]: wavefront partial sums; row MGS_NBMAX carries |v|^2
+  __shared__ double s_part[NW][MGS_NBMAX + 1][2];
+  __shared__ cplx<double> s_alpha[MGS_NBMAX];
+  __shared__ double s_n2;
   const int xc = blockIdx.x >> 1;
   const int k = blockIdx.x & 1;
   const int half = Ef / 2;
   const int Rk = Vb * half;
+  const int w = threadIdx.x >> 6;
   size_t rowoff[MGS_RPT];
   int nrow = 0;
 #pragma unroll
@@ -390,37 +397,91 @@ __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __r
     const int xb = rr / half, e = half * k + rr % half;
     rowoff[j] = agg_row(nb, Vb, Ef, xc, xb, e);
   }
+  // sum of this workgroup's s_part rows [0, cnt) into s_alpha (and row
+  // MGS_NBMAX into s_n2); ends with a barrier
+  auto finish = [&](int cnt, bool with_norm) {
+    __syncthreads();
+    for (int u = threadIdx.x; u < cnt; u += MGS_THREADS) {
+      double re = 0.0, im = 0.0;
+      for (int ww = 0; ww < NW; ++ww) {
+        re += s_part[ww][u][0];
+        im += s_part[ww][u][1];
+      }
+      s_alpha[u] = {re, im};
+    }
+    if (with_norm && threadIdx.x == MGS_THREADS - 1) {
+      double n2 = 0.0;
+      for (int ww = 0; ww < NW; ++ww) n2 += s_part[ww][MGS_NBMAX][0];
+      s_n2 = n2;
+    }
+    __syncthreads();
+  };
   for (int pass = 0; pass < passes; ++pass) {
     for (int v = 0; v < nb; ++v) {
       cplx<double> vr[MGS_RPT];
 #pragma unroll
       for (int j = 0; j < MGS_RPT; ++j)
         vr[j] = j < nrow ? U[rowoff[j] + v] : cplx<double>{0.0, 0.0};
-      for (int u = 0; u < v; ++u) {
-        cplx<double> d = {0.0, 0.0};
-        cplx<double> ur[MGS_RPT];
+      double n2v = 0.0;  // |v|^2 before the projection
+      {
+        double a = 0.0;
 #pragma unroll
-        for (int j = 0; j < MGS_RPT; ++j) {
-          ur[j] = j < nrow ? U[rowoff[j] + u] : cplx<double>{0.0, 0.0};
-          cmac_conj(d, ur[j], vr[j]);
-        }
-        const double2 a = wg_sum2(d.re, d.im, s_buf);
-        const cplx<double> alpha = {a.x, a.y};
-#pragma unroll
-        for (int j = 0; j < MGS_RPT; ++j) {
-          const cplx<double> t = cmul(alpha, ur[j]);
-          vr[j].re -= t.re;
-          vr[j].im -= t.im;
-        }
+        for (int j = 0; j < MGS_RPT; ++j) a += vr[j].re * vr[j].re + vr[j].im * vr[j].im;
+        wave_sum2(a, 0.0, s_part[w][MGS_NBMAX]);
       }
-      double n2 = 0.0;
+      for (int rep = 0; rep < 2 && v > 0; ++rep) {
+        // <u, v> for every finished column u < v, four columns' loads at a time
+        for (int u0 = 0; u0 < v; u0 += 4) {
+          cplx<double> d[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
 #pragma unroll
-      for (int j = 0; j < MGS_RPT; ++j) n2 += vr[j].re * vr[j].re + vr[j].im * vr[j].im;
-      const double2 nn = wg_sum2(n2, 0.0, s_buf);
-      const double sc = 1.0 / sqrt(nn.x);
+          for (int j = 0; j < MGS_RPT; ++j)
+            if (j < nrow) {
+              cplx<double> uu[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) uu[q] = U[rowoff[j] + min(u0 + q, v - 1)];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) cmac_conj(d[q], uu[q], vr[j]);
+            }
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (u0 + q < v) wave_sum2(d[q].re, d[q].im, s_part[w][u0 + q]);
+        }
+        finish(v, rep == 0);
+        if (rep == 0) n2v = s_n2;
+        // v -= sum_u <u, v> u, and |v|^2 after it
+        double a = 0.0;
+#pragma unroll
+        for (int j = 0; j < MGS_RPT; ++j) {
+          if (j < nrow) {
+            double xr = vr[j].re, xi = vr[j].im;
+            for (int u0 = 0; u0 < v; u0 += 4) {
+              cplx<double> uu[4];
+#pragma unroll
+              for (int q = 0; q < 4; ++q) uu[q] = U[rowoff[j] + min(u0 + q, v - 1)];
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                if (u0 + q < v) {
+                  const cplx<double> al = s_alpha[u0 + q];
+                  xr -= al.re * uu[q].re - al.im * uu[q].im;
+                  xi -= al.re * uu[q].im + al.im * uu[q].re;
+                }
+            }
+            vr[j] = {xr, xi};
+          }
+          a += vr[j].re * vr[j].re + vr[j].im * vr[j].im;
+        }
+        wave_sum2(a, 0.0, s_part[w][MGS_NBMAX]);
+        finish(0, true);
+        // one projection suffices unless the norm dropped below 1/sqrt(2)
+        if (s_n2 >= 0.5 * n2v) break;
+        n2v = s_n2;
+      }
+      if (v == 0) finish(0, true);
+      const double sc = 1.0 / sqrt(s_n2);
 #pragma unroll
       for (int j = 0; j < MGS_RPT; ++j)
         if (j < nrow) U[rowoff[j] + v] = {vr[j].re * sc, vr[j].im * sc};
+      __syncthreads();  // column v is final before the next column reads it
     }
   }
 }

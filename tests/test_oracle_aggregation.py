@@ -106,6 +106,54 @@ def test_orthogonalise_is_phase_fixed_qr():
             assert np.abs(q - got).max() < 1e-12
 
 
+def _cgs_dgks_sweeps(A, passes):
+    """The device kernel's column sweep on one panel A (rows x columns): all
+    <u, v> with the finished columns at once, v -= U <U, v>, repeated when
+    |v| fell below 1/sqrt(2) of its value before the projection, v /= |v|."""
+    Q = A.copy()
+    for _ in range(passes):
+        for v in range(Q.shape[1]):
+            x = Q[:, v].copy()
+            n2 = np.vdot(x, x).real
+            for _rep in range(2):
+                if v == 0:
+                    break
+                x = x - Q[:, :v] @ (Q[:, :v].conj().T @ x)
+                m2 = np.vdot(x, x).real
+                if m2 >= 0.5 * n2:
+                    break
+                n2 = m2
+            Q[:, v] = x / np.sqrt(np.vdot(x, x).real)
+    return Q
+
+
+@pytest.mark.parametrize("passes", [1, 2])
+def test_classical_sweep_with_reorthogonalisation_equals_mgs(passes):
+    """The GPU orthonormalisation sweeps columns with classical projections
+    (all inner products with the finished columns reduced together) and a
+    second projection when the norm drops below 1/sqrt(2); on full-rank
+    panels this gives the modified Gram-Schmidt result of the oracle
+    (include/AggregationKernelsRework.h:266-341) to rounding, also for a
+    panel with nearly dependent columns, where the second projection acts."""
+    nb = 6
+    B = rand_basis(FD, nb, 441)
+    bi = block_index(FD, BLK)
+    sites = np.nonzero(bi == 3)[0]
+    # columns 4 and 5 nearly along columns 0 and 1 (cancellation by 1e-6)
+    B[4] = B[0] + 1e-6 * B[4]
+    B[5] = B[1] - 1e-6 * B[5]
+    Q = oracle.orthogonalise(FD, BLK, nb, B, passes=passes)
+    for k in range(2):
+        sl = slice(6 * k, 6 * k + 6)
+        A = B[:, sites, sl].reshape(nb, -1).T
+        got = _cgs_dgks_sweeps(A, passes)
+        ref = Q[:, sites, sl].reshape(nb, -1).T
+        assert np.abs(got.conj().T @ got - np.eye(nb)).max() < 1e-12
+        # the nearly dependent columns are determined only to ~1e-16 / 1e-6
+        assert np.abs(got[:, :4] - ref[:, :4]).max() < 1e-12
+        assert np.abs(got - ref).max() < 1e-8
+
+
 def test_transfer_identities():
     """(1 - P R) v_i = 0 and (1 - R P) v_c = 0 (MultigridPreconditioner.h:352-396)."""
     nb = 3

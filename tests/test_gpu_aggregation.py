@@ -42,6 +42,32 @@ def test_orthogonalise_matches_oracle(gpu, fd, blk, nb):
         assert errs.max() < 1e-20, errs.max()
 
 
+@pytest.mark.parametrize("passes", [1, 2])
+def test_orthogonalise_nearly_dependent_columns(gpu, passes):
+    """A basis whose vectors 4 and 5 are nearly along 0 and 1 (cancellation
+    1e-6), where the sweep's second projection acts.  The well-determined
+    columns match the oracle's modified Gram-Schmidt to 1e-12, the nearly
+    dependent ones to 1e-8 (they carry ~1e-16 / 1e-6 rounding), and every
+    panel is orthonormal."""
+    fd, blk, nb = (4, 4, 4, 8), (2, 2, 2, 4), 6
+    print(f"[case] ortho nearly dependent passes={passes}", flush=True)
+    Vf = int(np.prod(fd))
+    B = gauss(nb * Vf * 12, 561).reshape(nb, Vf, 12)
+    B[4] = B[0] + 1e-6 * B[4]
+    B[5] = B[1] - 1e-6 * B[5]
+    agg = gpu.Aggregation(fd, blk, nb)
+    agg.set_basis(gpu.DeviceBuffer.from_numpy(B))
+    agg.orthogonalise(passes)
+    d = gpu.DeviceBuffer(B.nbytes)
+    agg.get_basis(d)
+    gpu.synchronize()
+    got = d.to_numpy(np.complex128, B.shape)
+    ref = oracle.orthogonalise(fd, blk, nb, B, passes)
+    assert rel(ref[:4], got[:4]) < 1e-12
+    assert rel(ref, got) < 1e-8
+    assert agg.check_orthogonal().max() < 1e-20
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("fd,blk,nb", CASES)
 @pytest.mark.parametrize("nrhs", [1, 5])

@@ -161,6 +161,129 @@ __global__ __launch_bounds__(256) void k_bdot_part_f2(const cplx<float>* __restr
   }
 }
 
+// Fused re-orthogonalisation pass: w -= sum_j c[j][r] V_j (stored in the
+// vector precision), then the per-rhs partials of <V_j, w>_r for j < k and of
+// |w|^2_r (row k) for the updated w, on the grid and with the per-thread
+// element order of k_bdot_part / k_bdot_part_f2 (so the same partials), with
+// V read once from memory (the second read of V_j by the same thread is an
+// L2 hit).  KT >= k.
+template <typename R, int KT, bool PAIRED>
+__global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restrict__ c,
+                                                   const cplx<R>* V, size_t ldv, int k,
+                                                   cplx<R>* w, size_t n, int C, int dof, int nrhs,
+                                                   double* __restrict__ part) {
+  __shared__ double sm[BD_ROWS][256][2];
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double re[KT + 1], im[KT];
+#pragma unroll
+  for (int j = 0; j <= KT; ++j) re[j] = 0.0;
+#pragma unroll
+  for (int j = 0; j < KT; ++j) im[j] = 0.0;
+  if constexpr (!PAIRED) {
+    for (size_t i = g; i < n; i += T) {
+      const int r = (int)((i % C) / dof);
+      double xr = w[i].re, xi = w[i].im;
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const cplx<double> cj = c[(size_t)j * nrhs + r];
+          const cplx<R> v = V[(size_t)j * ldv + i];
+          xr += -1.0 * (cj.re * v.re - cj.im * v.im);
+          xi += -1.0 * (cj.re * v.im + cj.im * v.re);
+        }
+      const cplx<R> b = {(R)xr, (R)xi};
+      w[i] = b;
+      asm volatile("" ::: "memory");  // re-read V_j below instead of holding it
+      const double br = b.re, bi = b.im;
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const cplx<R> a = V[(size_t)j * ldv + i];
+          const double ar = a.re, ai = a.im;
+          re[j] += ar * br + ai * bi;
+          im[j] += ar * bi - ai * br;
+        }
+      re[KT] += br * br + bi * bi;
+    }
+  } else {
+    const size_t n2 = n / 2;
+    const int C2 = C / 2;
+    for (size_t i2 = g; i2 < n2; i2 += T) {
+      const int r = (int)((2 * (i2 % C2)) / dof);
+      const float4 wv = reinterpret_cast<const float4*>(w)[i2];
+      double x0r = wv.x, x0i = wv.y, x1r = wv.z, x1i = wv.w;
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const cplx<double> cj = c[(size_t)j * nrhs + r];
+          const float4 v = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          x0r += -1.0 * (cj.re * v.x - cj.im * v.y);
+          x0i += -1.0 * (cj.re * v.y + cj.im * v.x);
+          x1r += -1.0 * (cj.re * v.z - cj.im * v.w);
+          x1i += -1.0 * (cj.re * v.w + cj.im * v.z);
+        }
+      const float4 b = make_float4((float)x0r, (float)x0i, (float)x1r, (float)x1i);
+      reinterpret_cast<float4*>(w)[i2] = b;
+      asm volatile("" ::: "memory");
+#pragma unroll
+      for (int j = 0; j < KT; ++j)
+        if (j < k) {
+          const float4 a = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          re[j] += (double)a.x * b.x + (double)a.y * b.y;
+          im[j] += (double)a.x * b.y - (double)a.y * b.x;
+          re[j] += (double)a.z * b.z + (double)a.w * b.w;
+          im[j] += (double)a.z * b.w - (double)a.w * b.z;
+        }
+      re[KT] += (double)b.x * b.x + (double)b.y * b.y;
+      re[KT] += (double)b.z * b.z + (double)b.w * b.w;
+    }
+  }
+  // per-rhs sums of this block in groups of BD_ROWS rows (row k, |w|^2, is
+  // the accumulator re[KT]), threads in ascending order as in k_bdot_part
+  const int rows = k + 1;
+#pragma unroll
+  for (int j0 = 0; j0 <= KT; j0 += BD_ROWS) {
+    if (j0 < rows) {  // uniform over the block
+      __syncthreads();
+#pragma unroll
+      for (int jj = 0; jj < BD_ROWS; ++jj) {
+        const int j = j0 + jj;  // group row; row k takes re[KT]
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int q = 0; q <= KT; ++q)
+          if (q == j) {
+            a = q < k ? re[q] : re[KT];
+            b = q < k && q < KT ? im[q] : 0.0;
+          }
+        if (j == k) {
+          a = re[KT];
+          b = 0.0;
+        }
+        sm[jj][threadIdx.x][0] = a;
+        sm[jj][threadIdx.x][1] = b;
+      }
+      __syncthreads();
+      const int nj = min(BD_ROWS, rows - j0);
+      for (int q = threadIdx.x; q < nj * nrhs; q += blockDim.x) {
+        const int jj = q / nrhs, rr = q % nrhs;
+        double sr = 0.0, si = 0.0;
+        for (int t = 0; t < (int)blockDim.x; ++t) {
+          const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
+          const int rt = PAIRED ? (int)((2 * (gt % (C / 2))) / dof) : (int)((gt % C) / dof);
+          if (rt == rr) {
+            sr += sm[jj][t][0];
+            si += sm[jj][t][1];
+          }
+        }
+        double* o = part + ((((size_t)(j0 + jj) * nrhs + rr) * gridDim.x) + blockIdx.x) * 2;
+        o[0] = sr;
+        o[1] = si;
+      }
+    }
+  }
+}
+
 // res[q] = sum over nblk partials of part[q][*], q = row*nrhs + r
 __global__ __launch_bounds__(256) void k_bdot_final(const double* __restrict__ part, int nblk,
                                                     double* __restrict__ res) {
@@ -305,6 +428,49 @@ template <typename R> void bnorm2(BBlas& b, const cplx<R>* x, const BShape& sh, 
   for (int r = 0; r < sh.nrhs; ++r) out[r] = b.h_res[2 * r];
 }
 
+template <typename R, int KT>
+static void launch_bmsub_dot(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv,
+                             int k, cplx<R>* w, const BShape& sh, bool pairs, int nblk) {
+  if constexpr (std::is_same<R, float>::value) {
+    if (pairs) {
+      k_bmsub_dot<R, KT, true><<<nblk, 256, 0, b.s>>>(d_c, V, ldv, k, w, sh.n, sh.cols(), sh.dof,
+                                                        sh.nrhs, b.d_part);
+      return;
+    }
+  }
+  k_bmsub_dot<R, KT, false><<<nblk, 256, 0, b.s>>>(d_c, V, ldv, k, w, sh.n, sh.cols(), sh.dof,
+                                                     sh.nrhs, b.d_part);
+}
+
+template <typename R>
+bool bmultisub_dots_dev(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k,
+                        cplx<R>* w, const BShape& sh, cplx<double>* d_out) {
+  check_shape(sh);
+  if (k < 1 || k > 24) return false;
+  bool pairs = false;
+  if constexpr (std::is_same<R, float>::value)
+    pairs = sh.dof % 2 == 0 && ldv % 2 == 0 && ((uintptr_t)V % 16) == 0 &&
+            ((uintptr_t)w % 16) == 0;
+  const int nblk = bred_blocks(sh, pairs);
+  const int rows = k + 1;
+  const size_t need = (size_t)rows * sh.nrhs * nblk * 2;
+  if (need > b.part_cap) {
+    if (b.d_part) MGC_HIP(hipFree(b.d_part));
+    b.d_part = nullptr;
+    b.part_cap = 0;
+    const size_t cap = std::max(need, (size_t)(BBlas::KMAX + 1) * sh.nrhs * nblk * 2);
+    MGC_HIP(hipMalloc(&b.d_part, cap * sizeof(double)));
+    b.part_cap = cap;
+  }
+  if (k <= 8) launch_bmsub_dot<R, 8>(b, d_c, V, ldv, k, w, sh, pairs, nblk);
+  else if (k <= 16) launch_bmsub_dot<R, 16>(b, d_c, V, ldv, k, w, sh, pairs, nblk);
+  else launch_bmsub_dot<R, 24>(b, d_c, V, ldv, k, w, sh, pairs, nblk);
+  MGC_LAUNCH_CHECK();
+  k_bdot_final<<<rows * sh.nrhs, 256, 0, b.s>>>(b.d_part, nblk, reinterpret_cast<double*>(d_out));
+  MGC_LAUNCH_CHECK();
+  return true;
+}
+
 template <typename R>
 void bmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k, cplx<R>* w,
                    const BShape& sh, hipStream_t s) {
@@ -356,6 +522,8 @@ void bconvert(BBlas& b, const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const 
   template void bdots_dev<R>(BBlas&, const cplx<R>*, size_t, int, const cplx<R>*,             \
                              const BShape&, bool, cplx<double>*);                             \
   template void bnorm2<R>(BBlas&, const cplx<R>*, const BShape&, double*);                    \
+  template bool bmultisub_dots_dev<R>(BBlas&, const cplx<double>*, const cplx<R>*, size_t, int, \
+                                      cplx<R>*, const BShape&, cplx<double>*);                 \
   template void bmultisub_dev<R>(const cplx<double>*, const cplx<R>*, size_t, int, cplx<R>*,  \
                                  const BShape&, hipStream_t);                                 \
   template void bmultiaxpy<R>(BBlas&, const cplx<double>*, const cplx<R>*, size_t, int,       \

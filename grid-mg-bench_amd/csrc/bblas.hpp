@@ -44,6 +44,13 @@ void bdots_dev(BBlas& b, const cplx<R>* V, size_t ldv, int k, const cplx<R>* w,
 void bfetch(BBlas& b, const double* d_src, int count);
 template <typename R> void bnorm2(BBlas& b, const cplx<R>* x, const BShape& sh, double* out);
 
+// Fused re-orthogonalisation pass: w -= sum_j c[j][r] V_j, then d_out[j][r] =
+// <V_j, w>_r for j < k and d_out[k][r].re = |w|^2_r, as bmultisub_dev followed
+// by bdots_dev(..., true, d_out) but with one read of V; false (nothing done)
+// for k > 24
+template <typename R>
+bool bmultisub_dots_dev(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k,
+                        cplx<R>* w, const BShape& sh, cplx<double>* d_out);
 // w -= sum_j c[j][r] V_j (c on the device, [k][nrhs])
 template <typename R>
 void bmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k, cplx<R>* w,

@@ -145,8 +145,11 @@ class BGmres {
         if (!reorth) {
           bsubnorm_dev<R>(dh1, V_, n, j + 1, w_, vnext, sh_, s);
         } else {
-          bmultisub_dev<R>(dh1, V_, n, j + 1, w_, sh_, s);
-          bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, true, dh2);
+          // w' = w - V h1, then h2 = V^H w' and |w'|^2 (one fused pass when j < 24)
+          if (!bmultisub_dots_dev<R>(bl, dh1, V_, n, j + 1, w_, sh_, dh2)) {
+            bmultisub_dev<R>(dh1, V_, n, j + 1, w_, sh_, s);
+            bdots_dev<R>(bl, V_, n, j + 1, w_, sh_, true, dh2);
+          }
           bsubnorm_dev<R>(dh2, V_, n, j + 1, w_, vnext, sh_, s);
           bfetch(bl, reinterpret_cast<const double*>(dh2), 2 * (j + 2) * nr);
           bool redo = false;

@@ -240,7 +240,10 @@ int mgc_aggregation_info(const mgc_aggregation* agg, int coarse_dims[4], int* nb
  * writable Subspace() (:75). */
 int mgc_aggregation_set_basis(mgc_aggregation* agg, const void* basis_f64, void* stream);
 int mgc_aggregation_get_basis(const mgc_aggregation* agg, void* basis_f64, void* stream);
-/* Block modified Gram-Schmidt per (block, chirality), FP64, `passes` sweeps.
+/* Block Gram-Schmidt per (block, chirality), FP64, `passes` sweeps over the
+ * columns (each column's inner products reduced together, projected again
+ * when its norm drops below 1/sqrt(2)); for a basis of full rank the result is
+ * the modified Gram-Schmidt one of the reference to rounding.
  * Orthogonalise (include/AggregationRework.h:80-89) -> aggregateOrthogonalise
  * (include/AggregationKernelsRework.h:266-341). */
 int mgc_aggregation_orthogonalise(mgc_aggregation* agg, int passes, void* stream);

@@ -468,8 +468,8 @@ def main() -> int:
         print(json.dumps({"copy_calibration_bytes": nbytes, "launches": max(1, args.steps)}))
         return 0
     if args.gpus != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(the coarse operator shards by 4d domain decomposition; see DESIGN.md)",
+        print("bench.py measures one GPU only: the multi-GPU form (4d domain decomposition "
+              "of the coarse lattice with halo exchange) is not built",
               file=sys.stderr)
         return 2
 

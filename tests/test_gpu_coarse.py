@@ -1,8 +1,9 @@
 """GPU parity of the single-rhs coarse operator against the FP64 oracle.
 
 Tolerances (relative L2, ||ref-res||/||ref||):
-  F64 operator: 1e-13 (the north-star bound is 1e-12; the reference's own
-                bound 1e-15 on the squared ratio equals 3.2e-8 here)
+  F64 operator: 1e-13 (the FP64 parity bound of this path is 1e-12; the
+                reference's own bound 1e-15 on the squared ratio equals
+                3.2e-8 here)
   F32 operator: 2e-6 * sqrt(N/8) (one FP32 rounding per multiply-add over
                 9*N terms of random sign)
 """

@@ -25,6 +25,8 @@
 // FP32 scale (block max -> 1) and each (site, rhs) input column an FP32
 // scale; a point's product is accumulated in its own FP32 tile and folded
 // into the result as C += s_link(x,p) * s_in(x+d_p, j) * T_p.
+#include <type_traits>
+
 #include "coarse_op.hpp"
 
 namespace mgc {
@@ -232,13 +234,31 @@ __device__ __forceinline__ int mrhs_point(int q) {
   return q == 0 ? 8 : ((q - 1) >> 1) + (((q - 1) & 1) ? 4 : 0);
 }
 
+// B' words of one staged 16-byte chunk for FP16/BF16 with a per-lane byte
+// selector and sign mask: sel = identity (u = 0) or half swap (u = 1), and
+// m = (u = 0 ? sign of the high half : 0) ^ (both signs for the G5C sign of
+// Mdag).  The result is bit-identical to bprime().
+__device__ __forceinline__ uint4 bprime_perm(uint4 r, unsigned sel, unsigned m) {
+  uint4 o;
+  o.x = __builtin_amdgcn_perm(r.x, r.x, sel) ^ m;
+  o.y = __builtin_amdgcn_perm(r.y, r.y, sel) ^ m;
+  o.z = __builtin_amdgcn_perm(r.z, r.z, sel) ^ m;
+  o.w = __builtin_amdgcn_perm(r.w, r.w, sel) ^ m;
+  return o;
+}
+
 // LDS-staged form for one site x.  Slot s of the staged B' operand
 // (kc = (s/64)/NT, nt = (s/64)%NT, lane = s%64) holds B'[k][n] for
 // k = kc*KC + (lane/16)*KCL + e (e < KCL), n = nt*16 + lane%16: the KCL/2
-// complex entries c0.. of input column j = n/2.  Each point's loads are
-// issued before the previous point's MFMAs.  For FP16/BF16 the point's
-// product T_p is accumulated from zero and folded into the FP32 result as
-// C += s_link(x,p) * s_in(x+d_p, j) * T_p.
+// complex entries c0.. of input column j = n/2.  The nine points are
+// unrolled, so neighbour indices, scale rows and memory counters are
+// compile-time per point.  Neighbour inputs are read with unconditional
+// buffer loads over the neighbour's site block; a slot of a column j >= nr
+// reads column 0 instead (in bounds), and its product only reaches output
+// columns that are not stored.  The next point's input loads are issued as
+// soon as this point's chunks are staged, before the barrier that releases
+// the MFMAs.  For FP16/BF16 the point's product T_p is accumulated from zero
+// and folded into the FP32 result as C += s_link(x,p) * s_in(x+d_p, j) * T_p.
 template <int KIND, int N, int NT, int LAUX>
 __device__ __forceinline__ void mrhs_lds_site(
     const int x, const uint4* __restrict__ A, const float* __restrict__ ascale,
@@ -257,15 +277,22 @@ __device__ __forceinline__ void mrhs_lds_site(
   __shared__ uint4 sB[NSLOT];
   __shared__ float s_fac[SCALED ? 9 : 1][NCOL];
 
-  const int w = threadIdx.x >> 6;
+  __shared__ int s_ys[NMT][8];  // each wavefront's copy of the neighbour sites
+
+  const int w = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
   const int l = threadIdx.x & 63;
   const int u = l & 1;  // every slot of this thread has lane l, so column parity l%2
-  const T* in = reinterpret_cast<const T*>(in_);  // [site][nrhs][N] complex (T pairs)
+  const char* in = reinterpret_cast<const char*>(in_);  // [site][nrhs][N] complex (T pairs)
+  const int site_bytes = nrhs * N * 2 * (int)sizeof(T);
 
+  // neighbour sites: scalar for the scale table, and a per-wavefront LDS
+  // copy from which the point loop reads the next neighbour (written and
+  // read by the same wavefront, so no barrier is involved)
   int ys[9];
 #pragma unroll
   for (int p = 0; p < 8; ++p) ys[p] = nbr[(size_t)x * 8 + p];
   ys[8] = x;
+  if (l < 8) s_ys[w][l] = nbr[(size_t)x * 8 + l];
 
   if constexpr (SCALED) {
     if (threadIdx.x < NCOL) {
@@ -277,53 +304,57 @@ __device__ __forceinline__ void mrhs_lds_site(
     }
   }
 
-  // per-thread staging geometry (independent of the point)
-  int soff[SPT];   // element offset of the 16-byte source chunk within a site, -1 if padding
-  bool sneg[SPT];  // chunk lies in the lower chirality half (negated by Mdag)
-#pragma unroll
-  for (int q = 0; q < SPT; ++q) {
-    const int s = threadIdx.x + q * THREADS;
-    const int nt = (s >> 6) % NT, kc = (s >> 6) / NT;
-    const int j = (nt * 16 + (l & 15)) >> 1;
-    const int c0 = (kc * M::KC + (l >> 4) * M::KCL) >> 1;
-    soff[q] = j < nr ? ((rhs0 + j) * N + c0) * 2 : -1;
-    sneg[q] = dag && c0 >= N / 2;
-  }
+  // Staging geometry.  Slot q of this thread reads the 16-byte chunk at
+  // byte lane_b + slot_b(q) of the neighbour's site block: lane_b depends on
+  // the lane only, slot_b(q) and the slot's k-chunk (hence its Mdag sign) on
+  // the wavefront only.  A column j >= nr reads another column of the same
+  // site (rhs0 + j < nrhs) or, past the block, the zeros that the buffer
+  // range check returns; it only reaches output columns that are not stored.
+  constexpr int ROWB = N * 2 * (int)sizeof(T);  // bytes of one input column
+  const int lane_b = ((l & 15) >> 1) * ROWB + (l >> 4) * M::KCL * (int)sizeof(T);
+  auto slot_nt = [&](int q) { return (w + q * NMT) % NT; };
+  auto slot_kc = [&](int q) { return (w + q * NMT) / NT; };
+  auto slot_b = [&](int q) { return (rhs0 + slot_nt(q) * 8) * ROWB + slot_kc(q) * M::KC * (int)sizeof(T); };
+  auto slot_neg = [&](int q) { return dag && slot_kc(q) * M::KC >= N; };  // c0 >= N/2
+  const unsigned ssel = u ? 0x01000302u : 0x03020100u;
+  const unsigned smask0 = u ? 0u : 0x80000000u;
 
   const int lane_off = w * NKC * 64 + l;
   const LinkSrc<LAUX> src(A + (size_t)x * 9 * NMT * NKC * 64, 9 * NMT * NKC * 64 * 16, lane_off);
   uint4 a[NKC], b_n[SPT];
-  auto issue_b = [&](int p) {
-    const T* base = in + (size_t)ys[p] * nrhs * N * 2;
+  auto issue_b = [&](int y) {
+    const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<char*>(in + (size_t)y * site_bytes), (short)0, site_bytes, MRHS_BUF_DWORD3);
 #pragma unroll
-    for (int q = 0; q < SPT; ++q)
-      if (soff[q] >= 0) b_n[q] = *reinterpret_cast<const uint4*>(base + soff[q]);
-      else b_n[q] = make_uint4(0u, 0u, 0u, 0u);
+    for (int q = 0; q < SPT; ++q) {
+      const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, lane_b, slot_b(q), 0);
+      b_n[q] = make_uint4(v.x, v.y, v.z, v.w);
+    }
   };
 
   floatx4 C[NT];
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) C[nt] = {0.f, 0.f, 0.f, 0.f};
 
-  // A is single-buffered: the fragment for point p+1 is loaded into a[kc] as
-  // soon as point p's MFMAs have consumed it; B' is staged one point ahead
-#pragma unroll
-  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, (8 * NMT * NKC + kc) * 64);
-  issue_b(mrhs_point(0));
-#pragma unroll 1
-  for (int q9 = 0; q9 < 9; ++q9) {
+  // One point: stage the landed input chunks as B', issue the next point's
+  // input loads, release the MFMAs; the link fragment for the next point is
+  // loaded into a[kc] as soon as this point's MFMAs have consumed it.
+  auto point = [&](int q9, auto more_c) {
+    constexpr bool more = decltype(more_c)::value;
     const int p = mrhs_point(q9);
-    uint4 b[SPT];
+    const int pn = more ? mrhs_point(q9 + 1) : 0;  // never the self point
+    const int yn = more ? __builtin_amdgcn_readfirstlane(s_ys[w][pn]) : 0;
+    if (q9 > 0) __syncthreads();  // previous point's MFMA readers are done with sB
 #pragma unroll
-    for (int q = 0; q < SPT; ++q) b[q] = b_n[q];
-
-    __syncthreads();  // previous point's MFMA readers are done with sB
-#pragma unroll
-    for (int q = 0; q < SPT; ++q) sB[threadIdx.x + q * THREADS] = bprime<KIND>(b[q], u, sneg[q]);
+    for (int q = 0; q < SPT; ++q) {
+      if constexpr (SCALED)
+        sB[threadIdx.x + q * THREADS] =
+            bprime_perm(b_n[q], ssel, smask0 ^ (slot_neg(q) ? 0x80008000u : 0u));
+      else
+        sB[threadIdx.x + q * THREADS] = bprime<KIND>(b_n[q], u, slot_neg(q));
+    }
+    if constexpr (more) issue_b(yn);  // next point's input loads overlap this point's MFMAs
     __syncthreads();
-    const bool more = q9 < 8;
-    const int pn = more ? mrhs_point(q9 + 1) : 0;
-    if (more) issue_b(pn);  // next point's input loads overlap this point's MFMAs
     const int anext = (pn * NMT * NKC) * 64;
 
     if constexpr (SCALED) {
@@ -335,7 +366,7 @@ __device__ __forceinline__ void mrhs_lds_site(
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           Tp[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], Tp[nt]);
-        if (more) a[kc] = src.load(lane_off, anext + kc * 64);
+        if constexpr (more) a[kc] = src.load(lane_off, anext + kc * 64);
       }
 #pragma unroll
       for (int nt = 0; nt < NT; ++nt) {
@@ -349,10 +380,18 @@ __device__ __forceinline__ void mrhs_lds_site(
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           C[nt] = mfma_chunk<KIND>(a[kc], sB[(kc * NT + nt) * 64 + l], C[nt]);
-        if (more) a[kc] = src.load(lane_off, anext + kc * 64);
+        if constexpr (more) a[kc] = src.load(lane_off, anext + kc * 64);
       }
     }
-  }
+  };
+
+#pragma unroll
+  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, (8 * NMT * NKC + kc) * 64);
+  issue_b(x);  // the self point comes first
+#pragma unroll 1
+  for (int q9 = 0; q9 < 8; ++q9) point(q9, std::integral_constant<bool, true>());
+  point(8, std::integral_constant<bool, false>());
+
   // D[m][n]: lane l holds rows 4*(l/16)+i, column l%16
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) {

@@ -207,7 +207,7 @@ def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
 # WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy kernel
 # (FETCH x 2.000, WRITE x 1.000); the command lines and readings are in the
 # profile summary for this kernel under profiles/.
-HEADLINE_TRAFFIC_BYTES = {1: 930.0e6, 4: 930.0e6}
+HEADLINE_TRAFFIC_BYTES = {1: 938.9e6, 4: 938.9e6}
 
 MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 1: "k_mrhs (LDS-staged B', non-temporal link loads)",

@@ -219,6 +219,95 @@ __device__ __forceinline__ int wilson_tile_site(const WilsonTiles& w, int b) {
   return g * w.S + w.P * (z + w.L2 * t);
 }
 
+// Clover term of one (site, rhs): acc += D(x) v with the packed hermitian
+// blocks of chirality l at diag[l][slot][x]; sp(e) returns element e of the
+// site's spinor.
+template <typename R, typename SP>
+__device__ __forceinline__ void wilson_diag_term(cplx<R>* acc, const cplx<R>* __restrict__ diag,
+                                                 int x, int sv, SP&& sp) {
+#pragma unroll
+  for (int l = 0; l < 2; ++l) {
+    const cplx<R>* D = diag + (size_t)l * 18 * sv + x;  // packed slot s at D[s * sv]
+    cplx<R> v[6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) v[b] = sp(6 * l + b);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const cplx<R> dd = D[(size_t)a * sv];  // (d_{2a}, d_{2a+1})
+      acc[6 * l + 2 * a].re += dd.re * v[2 * a].re;
+      acc[6 * l + 2 * a].im += dd.re * v[2 * a].im;
+      acc[6 * l + 2 * a + 1].re += dd.im * v[2 * a + 1].re;
+      acc[6 * l + 2 * a + 1].im += dd.im * v[2 * a + 1].im;
+    }
+    int slot = 3;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a + 1; b < 6; ++b, ++slot) {
+        const cplx<R> u = D[(size_t)slot * sv];
+        cmac(acc[6 * l + a], u, v[b]);       // upper: D_ab v_b
+        cmac_conj(acc[6 * l + b], u, v[a]);  // lower: conj(D_ab) v_a
+      }
+  }
+}
+
+// Hopping term p with the rank-2 spin projector: for the upper spins s = 0, 1
+// (g_mu maps them to s' = col(s) in {2, 3}),
+//   h_s = psi_s - sg ph(s) psi_s',  (1 - sg g_mu) psi at s' = -sg ph(s') h_s,
+// so only the half spinor h is multiplied by the SU(3) link.  p is a
+// compile-time constant at every call (unrolled loops), so every spin index
+// is too.  sp(e) returns element e of the neighbour's spinor psi(y).
+template <typename R, typename SP>
+__device__ __forceinline__ void wilson_hop_term(cplx<R>* acc, const cplx<R>* __restrict__ U,
+                                                int x, int y, int sv, const GammaTab& G, int p,
+                                                SP&& sp) {
+  const int mu = p & 3;
+  const R sg = p < 4 ? R(1) : R(-1);
+  cplx<R> h[2][3];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int cs = G.col[mu][s];
+    const cplx<R> ph = {(R)(sg * G.re[mu][s]), (R)(sg * G.im[mu][s])};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const cplx<R> a = sp(s * 3 + j), b = sp(cs * 3 + j);
+      const cplx<R> t = cmul(ph, b);
+      h[s][j] = {a.re - t.re, a.im - t.im};
+    }
+  }
+  const cplx<R>* u = U + (size_t)mu * 9 * sv + (p < 4 ? x : y);  // element k at u[k * sv]
+  cplx<R> chi[2][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    cplx<R> L[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      L[j] = p < 4 ? u[(size_t)(i * 3 + j) * sv]
+                   : cplx<R>{u[(size_t)(j * 3 + i) * sv].re, -u[(size_t)(j * 3 + i) * sv].im};
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      cplx<R> a = {R(0), R(0)};
+#pragma unroll
+      for (int j = 0; j < 3; ++j) cmac(a, L[j], h[s][j]);
+      chi[s][i] = a;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int cs = G.col[mu][s];
+    // lower spin cs: -1/2 * (-sg ph(cs)) chi_s = (sg/2) ph(cs) chi_s
+    const cplx<R> ph = {(R)(0.5 * sg * G.re[mu][cs]), (R)(0.5 * sg * G.im[mu][cs])};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      acc[s * 3 + i].re -= R(0.5) * chi[s][i].re;
+      acc[s * 3 + i].im -= R(0.5) * chi[s][i].im;
+      const cplx<R> t = cmul(ph, chi[s][i]);
+      acc[cs * 3 + i].re += t.re;
+      acc[cs * 3 + i].im += t.im;
+    }
+  }
+}
+
 template <typename R>
 __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
                                                 const cplx<R>* __restrict__ diag,
@@ -239,92 +328,140 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
     if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
     return v;
   };
-  if (mask & 0x100u) {
-#pragma unroll
-    for (int l = 0; l < 2; ++l) {
-      const cplx<R>* D = diag + (size_t)l * 18 * sv + x;  // packed slot s at D[s * sv]
-      cplx<R> v[6];
-#pragma unroll
-      for (int b = 0; b < 6; ++b) v[b] = ld(x, 6 * l + b);
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const cplx<R> dd = D[(size_t)a * sv];  // (d_{2a}, d_{2a+1})
-        acc[6 * l + 2 * a].re += dd.re * v[2 * a].re;
-        acc[6 * l + 2 * a].im += dd.re * v[2 * a].im;
-        acc[6 * l + 2 * a + 1].re += dd.im * v[2 * a + 1].re;
-        acc[6 * l + 2 * a + 1].im += dd.im * v[2 * a + 1].im;
-      }
-      int slot = 3;
-#pragma unroll
-      for (int a = 0; a < 6; ++a)
-#pragma unroll
-        for (int b = a + 1; b < 6; ++b, ++slot) {
-          const cplx<R> u = D[(size_t)slot * sv];
-          cmac(acc[6 * l + a], u, v[b]);       // upper: D_ab v_b
-          cmac_conj(acc[6 * l + b], u, v[a]);  // lower: conj(D_ab) v_a
-        }
-    }
-  }
-  // Hopping terms with the rank-2 spin projector: for the upper spins s = 0, 1
-  // (g_mu maps them to s' = col(s) in {2, 3}),
-  //   h_s = psi_s - sg ph(s) psi_s',  (1 - sg g_mu) psi at s' = -sg ph(s') h_s,
-  // so only the half spinor h is multiplied by the SU(3) link.  The loop is
-  // unrolled so that every spin index is a compile-time constant.
+  if (mask & 0x100u) wilson_diag_term<R>(acc, diag, x, sv, [&](int e) { return ld(x, e); });
 #pragma unroll
   for (int p = 0; p < 8; ++p) {
     if (!((mask >> p) & 1u)) continue;
-    const int mu = p & 3;
-    const R sg = p < 4 ? R(1) : R(-1);
     const int y = nbr[(size_t)x * 8 + p];
-    cplx<R> h[2][3];
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int cs = G.col[mu][s];
-      const cplx<R> ph = {(R)(sg * G.re[mu][s]), (R)(sg * G.im[mu][s])};
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        const cplx<R> a = ld(y, s * 3 + j), b = ld(y, cs * 3 + j);
-        const cplx<R> t = cmul(ph, b);
-        h[s][j] = {a.re - t.re, a.im - t.im};
-      }
-    }
-    const cplx<R>* u = U + (size_t)mu * 9 * sv + (p < 4 ? x : y);  // element k at u[k * sv]
-    cplx<R> chi[2][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      cplx<R> L[3];
-#pragma unroll
-      for (int j = 0; j < 3; ++j)
-        L[j] = p < 4 ? u[(size_t)(i * 3 + j) * sv]
-                     : cplx<R>{u[(size_t)(j * 3 + i) * sv].re, -u[(size_t)(j * 3 + i) * sv].im};
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        cplx<R> a = {R(0), R(0)};
-#pragma unroll
-        for (int j = 0; j < 3; ++j) cmac(a, L[j], h[s][j]);
-        chi[s][i] = a;
-      }
-    }
-#pragma unroll
-    for (int s = 0; s < 2; ++s) {
-      const int cs = G.col[mu][s];
-      // lower spin cs: -1/2 * (-sg ph(cs)) chi_s = (sg/2) ph(cs) chi_s
-      const cplx<R> ph = {(R)(0.5 * sg * G.re[mu][cs]), (R)(0.5 * sg * G.im[mu][cs])};
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        acc[s * 3 + i].re -= R(0.5) * chi[s][i].re;
-        acc[s * 3 + i].im -= R(0.5) * chi[s][i].im;
-        const cplx<R> t = cmul(ph, chi[s][i]);
-        acc[cs * 3 + i].re += t.re;
-        acc[cs * 3 + i].im += t.im;
-      }
-    }
+    wilson_hop_term<R>(acc, U, x, y, sv, G, p, [&](int e) { return ld(y, e); });
   }
 #pragma unroll
   for (int e = 0; e < 12; ++e) {
     cplx<R> v = acc[e];
     if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
     out[((size_t)x * nrhs + rr) * 12 + e] = v;
+  }
+}
+
+// Chunk k of thread t in a cooperative copy of T records of CPS 16-byte
+// chunks: chunk c = t + k*T of record q = c / CPS (tile site q / nrhs, rhs
+// q % nrhs), part e = c % CPS.  The copy reads record (sites[q / nrhs], rhs)
+// of the field; the stage stores it at position rec_pos(q, e) in LDS.
+template <int CPS, int RS>
+__device__ __forceinline__ int rec_pos(int q, int e) {
+  return q * RS + (RS == CPS ? (e + q) % CPS : e);
+}
+// Cooperative copy of T records of CPS 16-byte chunks into LDS: chunk
+// c = t + k*T of record q = c / CPS (tile site q / nrhs, rhs q % nrhs), part
+// e = c % CPS, is read from record (sites[q / nrhs], rhs) of the field and
+// stored at position rec_pos(q, e).
+template <int CPS, int RS>
+__device__ __forceinline__ void wilson_copy(uint4* s_rec, const uint4* __restrict__ in4,
+                                            const int* sites, int t, int T, int nrhs) {
+#pragma unroll
+  for (int k = 0; k < CPS; ++k) {
+    const int c = t + k * T, q = c / CPS, e = c - q * CPS, s = q / nrhs, r = q - s * nrhs;
+    s_rec[rec_pos<CPS, RS>(q, e)] = in4[((size_t)sites[s] * nrhs + r) * CPS + e];
+  }
+}
+
+// LDS-gathered form of the apply.  Each (site, rhs) spinor record of 12
+// complex is contiguous in memory, so a workgroup takes a hop's neighbour
+// records for all its threads as one cooperative copy of 16-byte chunks in
+// which consecutive lanes read consecutive chunks (coalesced, unlike a
+// thread reading its own record with a stride of one record per lane).  The
+// chunks go through registers into LDS, each thread then reads its own
+// record there; the copies of the next hop are issued before the arithmetic
+// of the current one.  Hops run self first, then 0..7; the neighbour sites
+// of the next hop are staged in LDS one hop ahead.  FP64 records (12 chunks)
+// are stored with chunk e of record q at position (e + q) mod 12 and FP32
+// records (6 chunks) padded to 7, so that the records that neighbouring
+// lanes read fall in different banks.  The result leaves through LDS in the
+// same coalesced way.  Threads past the lattice copy site 0 and do no
+// arithmetic.  Per-thread arithmetic is that of k_wilson.
+template <typename R>
+__global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ U,
+                                                    const cplx<R>* __restrict__ diag,
+                                                    const cplx<R>* __restrict__ in,
+                                                    cplx<R>* __restrict__ out,
+                                                    const int* __restrict__ nbr, int V, int sv,
+                                                    int nrhs, unsigned mask, int dag,
+                                                    WilsonTiles wt) {
+  constexpr int CPS = 12 * (int)sizeof(cplx<R>) / 16;  // 16-byte chunks per record
+  constexpr int RS = CPS == 12 ? CPS : CPS + 1;  // record stride in chunks
+  __shared__ uint4 s_rec[256 * RS];
+  __shared__ int s_y[2][256];
+  const int T = (int)blockDim.x;  // S sites x nrhs
+  const int S = T / nrhs;
+  const int t = (int)threadIdx.x;
+  const int x0 = wilson_tile_site(wt, (int)blockIdx.x);
+  const int x = x0 + t / nrhs;
+  const int rr = t % nrhs;
+  const bool live = x < V;
+  const GammaTab G = gamma_table();
+
+  auto pos = [&](int q, int e) { return rec_pos<CPS, RS>(q, e); };
+  auto hop_site = [&](int s, int h) {  // h = 8: the site itself
+    const int xs = x0 + s;
+    if (xs >= V) return 0;
+    return h == 8 ? xs : nbr[(size_t)xs * 8 + h];
+  };
+  // next active hop after stage i (stages: 0 = self, 1 + p = hop p), -1 if none
+  auto next_hop = [&](int i) {
+    for (int k = i + 1; k < 9; ++k) {
+      const int h = k == 0 ? 8 : k - 1;
+      if ((mask >> h) & 1u) return h;
+    }
+    return -1;
+  };
+  const uint4* in4 = reinterpret_cast<const uint4*>(in);
+  auto rec = [&](int e) {  // element e of this thread's staged record
+    cplx<R> v = reinterpret_cast<const cplx<R>*>(&s_rec[pos(t, e * CPS / 12)])[(e * CPS) % 12 / 6];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    return v;
+  };
+
+  cplx<R> acc[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) acc[e] = {R(0), R(0)};
+  int cb = 0;  // s_y buffer of the current hop
+  const int h0 = next_hop(-1);
+  if (h0 >= 0) {
+    if (t < S) s_y[0][t] = hop_site(t, h0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int h = i == 0 ? 8 : i - 1;
+    if (!((mask >> h) & 1u)) continue;
+    const int hn = next_hop(i);
+    __syncthreads();  // every thread is done with the previous hop's records
+    wilson_copy<CPS, RS>(s_rec, in4, s_y[cb], t, T, nrhs);
+    if (hn >= 0 && t < S) s_y[cb ^ 1][t] = hop_site(t, hn);
+    __syncthreads();
+    const int y = s_y[cb][t / nrhs];
+    cb ^= 1;
+    if (live) {
+      if (h == 8)
+        wilson_diag_term<R>(acc, diag, x, sv, rec);
+      else
+        wilson_hop_term<R>(acc, U, x, y, sv, G, h, rec);
+    }
+  }
+  // result: records through LDS, then coalesced stores of whole chunks
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    cplx<R> v = acc[e];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    reinterpret_cast<cplx<R>*>(&s_rec[pos(t, e * CPS / 12)])[(e * CPS) % 12 / 6] = v;
+  }
+  __syncthreads();
+  uint4* out4 = reinterpret_cast<uint4*>(out);
+#pragma unroll
+  for (int k = 0; k < CPS; ++k) {
+    const int c = t + k * T, q = c / CPS, e = c - q * CPS, s = q / nrhs, r = q - s * nrhs;
+    if (x0 + s < V) out4[((size_t)(x0 + s) * nrhs + r) * CPS + e] = s_rec[pos(q, e)];
   }
 }
 
@@ -409,6 +546,10 @@ static WilsonTiles wilson_tiles(const Lat4& L, int nrhs) {
   return w;
 }
 
+// apply kernel: 0 = k_wilson (each thread reads its own records), 1 =
+// k_wilson_lds (records gathered through LDS); -1 = the default, the LDS form
+static int g_wilson_variant = -1;
+
 static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
                        unsigned mask, int dag, hipStream_t s) {
   MGC_CHECK(op, MGC_ERR_ARG, "null fine operator");
@@ -419,21 +560,20 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
   const WilsonTiles wt = wilson_tiles(op->lat, nrhs);
   const unsigned g = (unsigned)((op->V + wt.S - 1) / wt.S);
   const unsigned threads = (unsigned)(wt.S * nrhs);
+  const bool lds = g_wilson_variant != 0;
   if (prec == MGC_F64) {
-    k_wilson<double><<<g, threads, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in,
-                                           (cplx<double>*)out, op->nbr, op->V, op->sv, nrhs,
-                                           mask, dag,
-                                           wt);
+    auto k = lds ? k_wilson_lds<double> : k_wilson<double>;
+    k<<<g, threads, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in, (cplx<double>*)out,
+                            op->nbr, op->V, op->sv, nrhs, mask, dag, wt);
   } else if (prec == MGC_F32) {
     if (!op->gsoa32) {
       MGC_HIP(hipMalloc(&op->gsoa32, (size_t)36 * op->sv * sizeof(cplx<float>)));
       MGC_HIP(hipMalloc(&op->dsoa32, (size_t)36 * op->sv * sizeof(cplx<float>)));
       build_soa<float>(op, op->gsoa32, op->dsoa32, s);
     }
-    k_wilson<float><<<g, threads, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in,
-                                          (cplx<float>*)out, op->nbr, op->V, op->sv, nrhs,
-                                          mask, dag,
-                                          wt);
+    auto k = lds ? k_wilson_lds<float> : k_wilson<float>;
+    k<<<g, threads, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in, (cplx<float>*)out,
+                            op->nbr, op->V, op->sv, nrhs, mask, dag, wt);
   } else {
     throw Error(MGC_ERR_PRECISION, "fine operator: F64 or F32");
   }
@@ -445,6 +585,13 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
 using namespace mgc;
 
 extern "C" {
+
+int mgc_fine_op_set_variant(int variant) {
+  return guarded([&] {
+    MGC_CHECK(variant >= -1 && variant <= 1, MGC_ERR_ARG, "unknown fine-operator kernel variant");
+    g_wilson_variant = variant;
+  });
+}
 
 int mgc_random_gauge(void* gauge_f64, const int dims[4], double eps, uint64_t seed,
                      uint64_t stream_id, void* stream) {

@@ -106,6 +106,7 @@ _SIGS = {
     "mgc_fine_op_Mdag": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "mgc_fine_op_Mdiag": ([_vp, _vp, _vp, _i, _i, _vp], _i),
     "mgc_fine_op_Mdir": ([_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i),
+    "mgc_fine_op_set_variant": ([_i], _i),
     "mgc_random_gauge": ([_vp, _ip, _c.c_double, _u64, _u64, _vp], _i),
     "mgc_mg_params_default": ([_c.POINTER(MGParams)], _i),
     "mgc_mg_create": ([_c.POINTER(_vp), _vp, _c.POINTER(MGParams)], _i),

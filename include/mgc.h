@@ -293,6 +293,12 @@ int mgc_fine_op_Mdiag(mgc_fine_op* op, const void* in, void* out, int prec, int 
                       void* stream);
 int mgc_fine_op_Mdir(mgc_fine_op* op, const void* in, void* out, int dir, int disp, int prec,
                      int nrhs, void* stream);
+/* Kernel-variant control of the fine apply (all handles, process-wide):
+ * 0 = each thread reads its own spinor records, 1 = each hop's records
+ * gathered by the workgroup through LDS with coalesced loads; -1 restores
+ * the default (1).  Both give bit-identical results.  No counterpart in the
+ * reference: it serves tests and measurements. */
+int mgc_fine_op_set_variant(int variant);
 /* SU(3) gauge field U = exp(i eps H), H the traceless Hermitian part of a
  * Gaussian 3x3 (counter-based generator, element (mu*V + x)*9 + 3i + j);
  * the synthetic-gauge recipe of the multigrid configurations (eps = 0.4). */

@@ -216,7 +216,10 @@ MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 4: "k_mrhs (LDS-staged B', non-temporal link loads, XCD-contiguous sites)",
                 5: "k_mrhs_persistent (LDS-staged B')",
                 6: "k_mrhs_persistent (LDS-staged B', non-temporal link loads, "
-                   "XCD-contiguous sites)"}
+                   "XCD-contiguous sites)",
+                7: "k_mrhs_dma (input copied to LDS, 2 m-tiles per wavefront)",
+                8: "k_mrhs_dma (input copied to LDS, 2 m-tiles per wavefront, non-temporal "
+                   "link loads, XCD-contiguous sites)"}
 
 
 def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, keep=None):

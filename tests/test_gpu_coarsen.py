@@ -81,12 +81,15 @@ def test_fine_operator_matches_oracle(gpu, prec, nrhs):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("dims,nrhs", [((4, 4, 2, 6), 1), ((4, 4, 2, 6), 3), ((8, 8, 8, 8), 1),
-                                       ((8, 8, 8, 8), 5), ((8, 8, 8, 8), 12)])
+                                       ((8, 8, 8, 8), 5), ((8, 8, 8, 8), 12),
+                                       ((4, 4, 2, 6), 100), ((4, 4, 4, 8), 256)])
 def test_fine_operator_kernel_variants_bit_equal(gpu, prec, dims, nrhs):
     """The LDS-gathered apply keeps the per-thread arithmetic of the apply in
     which each thread reads its own records: every kind (M, Mdag, Mdiag, the
-    eight Mdir) gives the same bits.  (4, 4, 2, 6) has a partly filled tile,
-    8^4 the XCD-ordered tiles."""
+    eight Mdir) gives the same bits.  (4, 4, 2, 6) has partly filled tiles
+    (100 rhs: two sites of 100 threads each), 8^4 and 4x4x4x8 the
+    XCD-ordered tiles (256 rhs, the largest count: one site per
+    workgroup)."""
     print(f"[case] fine variants dims={dims} nrhs={nrhs} prec={prec}", flush=True)
     V = int(np.prod(dims))
     fop, _ = fine_setup(gpu, dims, 43)

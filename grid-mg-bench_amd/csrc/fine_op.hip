@@ -276,7 +276,8 @@ __device__ __forceinline__ void wilson_hop_term(cplx<R>* acc, const cplx<R>* __r
     }
   }
   const cplx<R>* u = U + (size_t)mu * 9 * sv + (p < 4 ? x : y);  // element k at u[k * sv]
-  cplx<R> chi[2][3];
+  // chi_s = L h_s, one colour row i at a time; each chi_s(i) goes into acc
+  // as soon as it is formed (every acc element gets one update per hop)
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
     cplx<R> L[3];
@@ -286,22 +287,15 @@ __device__ __forceinline__ void wilson_hop_term(cplx<R>* acc, const cplx<R>* __r
                    : cplx<R>{u[(size_t)(j * 3 + i) * sv].re, -u[(size_t)(j * 3 + i) * sv].im};
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      cplx<R> a = {R(0), R(0)};
+      cplx<R> chi = {R(0), R(0)};
 #pragma unroll
-      for (int j = 0; j < 3; ++j) cmac(a, L[j], h[s][j]);
-      chi[s][i] = a;
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    const int cs = G.col[mu][s];
-    // lower spin cs: -1/2 * (-sg ph(cs)) chi_s = (sg/2) ph(cs) chi_s
-    const cplx<R> ph = {(R)(0.5 * sg * G.re[mu][cs]), (R)(0.5 * sg * G.im[mu][cs])};
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      acc[s * 3 + i].re -= R(0.5) * chi[s][i].re;
-      acc[s * 3 + i].im -= R(0.5) * chi[s][i].im;
-      const cplx<R> t = cmul(ph, chi[s][i]);
+      for (int j = 0; j < 3; ++j) cmac(chi, L[j], h[s][j]);
+      const int cs = G.col[mu][s];
+      // lower spin cs: -1/2 * (-sg ph(cs)) chi_s = (sg/2) ph(cs) chi_s
+      const cplx<R> ph = {(R)(0.5 * sg * G.re[mu][cs]), (R)(0.5 * sg * G.im[mu][cs])};
+      acc[s * 3 + i].re -= R(0.5) * chi.re;
+      acc[s * 3 + i].im -= R(0.5) * chi.im;
+      const cplx<R> t = cmul(ph, chi);
       acc[cs * 3 + i].re += t.re;
       acc[cs * 3 + i].im += t.im;
     }
@@ -349,7 +343,7 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
 // of the field; the stage stores it at position rec_pos(q, e) in LDS.
 template <int CPS, int RS>
 __device__ __forceinline__ int rec_pos(int q, int e) {
-  return q * RS + (RS == CPS ? (e + q) % CPS : e);
+  return q * RS + e;
 }
 // Cooperative copy of T records of CPS 16-byte chunks into LDS: chunk
 // c = t + k*T of record q = c / CPS (tile site q / nrhs, rhs q % nrhs), part
@@ -373,10 +367,9 @@ __device__ __forceinline__ void wilson_copy(uint4* s_rec, const uint4* __restric
 // chunks go through registers into LDS, each thread then reads its own
 // record there; the copies of the next hop are issued before the arithmetic
 // of the current one.  Hops run self first, then 0..7; the neighbour sites
-// of the next hop are staged in LDS one hop ahead.  FP64 records (12 chunks)
-// are stored with chunk e of record q at position (e + q) mod 12 and FP32
-// records (6 chunks) padded to 7, so that the records that neighbouring
-// lanes read fall in different banks.  The result leaves through LDS in the
+// of the next hop are staged in LDS one hop ahead.  Records are padded by
+// one chunk (13 for FP64, 7 for FP32), so that the records that
+// neighbouring lanes read fall in different banks.  The result leaves through LDS in the
 // same coalesced way.  Threads past the lattice copy site 0 and do no
 // arithmetic.  Per-thread arithmetic is that of k_wilson.
 template <typename R>
@@ -388,7 +381,7 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
                                                     int nrhs, unsigned mask, int dag,
                                                     WilsonTiles wt) {
   constexpr int CPS = 12 * (int)sizeof(cplx<R>) / 16;  // 16-byte chunks per record
-  constexpr int RS = CPS == 12 ? CPS : CPS + 1;  // record stride in chunks
+  constexpr int RS = CPS + 1;  // record stride in chunks
   __shared__ uint4 s_rec[256 * RS];
   __shared__ int s_y[2][256];
   const int T = (int)blockDim.x;  // S sites x nrhs

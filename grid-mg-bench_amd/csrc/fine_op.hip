@@ -337,12 +337,10 @@ __global__ __launch_bounds__(256) void k_wilson(const cplx<R>* __restrict__ U,
   }
 }
 
-// Chunk k of thread t in a cooperative copy of T records of CPS 16-byte
-// chunks: chunk c = t + k*T of record q = c / CPS (tile site q / nrhs, rhs
-// q % nrhs), part e = c % CPS.  The copy reads record (sites[q / nrhs], rhs)
-// of the field; the stage stores it at position rec_pos(q, e) in LDS.
+// LDS position of chunk e of staged record q (records padded to RS chunks)
 template <int CPS, int RS>
 __device__ __forceinline__ int rec_pos(int q, int e) {
+  static_assert(RS > CPS, "padded records");
   return q * RS + e;
 }
 // Cooperative copy of T records of CPS 16-byte chunks into LDS: chunk
@@ -388,8 +386,7 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
   const int S = T / nrhs;
   const int t = (int)threadIdx.x;
   const int x0 = wilson_tile_site(wt, (int)blockIdx.x);
-  const int x = x0 + t / nrhs;
-  const int rr = t % nrhs;
+  const int x = x0 + t / nrhs;  // thread t is record t: site t / nrhs, rhs t % nrhs
   const bool live = x < V;
   const GammaTab G = gamma_table();
 

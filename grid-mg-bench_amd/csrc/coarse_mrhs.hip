@@ -250,15 +250,15 @@ __device__ __forceinline__ uint4 bprime_perm(uint4 r, unsigned sel, unsigned m) 
 // LDS-staged form for one site x.  Slot s of the staged B' operand
 // (kc = (s/64)/NT, nt = (s/64)%NT, lane = s%64) holds B'[k][n] for
 // k = kc*KC + (lane/16)*KCL + e (e < KCL), n = nt*16 + lane%16: the KCL/2
-// complex entries c0.. of input column j = n/2.  The nine points are
-// unrolled, so neighbour indices, scale rows and memory counters are
-// compile-time per point.  Neighbour inputs are read with unconditional
-// buffer loads over the neighbour's site block; a slot of a column j >= nr
-// reads column 0 instead (in bounds), and its product only reaches output
-// columns that are not stored.  The next point's input loads are issued as
-// soon as this point's chunks are staged, before the barrier that releases
-// the MFMAs.  For FP16/BF16 the point's product T_p is accumulated from zero
-// and folded into the FP32 result as C += s_link(x,p) * s_in(x+d_p, j) * T_p.
+// complex entries c0.. of input column j = n/2.  The first eight points run
+// as a rolled loop in which every load is issued and the ninth is peeled, so
+// the memory counters the compiler waits on are static.  Neighbour inputs are
+// read with unconditional buffer loads over the neighbour's site block (a
+// column j >= nr is never stored, see the staging geometry below).  The next
+// point's input loads are issued as soon as this point's chunks are staged,
+// before the barrier that releases the MFMAs.  For FP16/BF16 the point's
+// product T_p is accumulated from zero and folded into the FP32 result as
+// C += s_link(x,p) * s_in(x+d_p, j) * T_p.
 template <int KIND, int N, int NT, int LAUX>
 __device__ __forceinline__ void mrhs_lds_site(
     const int x, const uint4* __restrict__ A, const float* __restrict__ ascale,
@@ -314,7 +314,9 @@ __device__ __forceinline__ void mrhs_lds_site(
   const int lane_b = ((l & 15) >> 1) * ROWB + (l >> 4) * M::KCL * (int)sizeof(T);
   auto slot_nt = [&](int q) { return (w + q * NMT) % NT; };
   auto slot_kc = [&](int q) { return (w + q * NMT) / NT; };
-  auto slot_b = [&](int q) { return (rhs0 + slot_nt(q) * 8) * ROWB + slot_kc(q) * M::KC * (int)sizeof(T); };
+  auto slot_b = [&](int q) {
+    return (rhs0 + slot_nt(q) * 8) * ROWB + slot_kc(q) * M::KC * (int)sizeof(T);
+  };
   auto slot_neg = [&](int q) { return dag && slot_kc(q) * M::KC >= N; };  // c0 >= N/2
   const unsigned ssel = u ? 0x01000302u : 0x03020100u;
   const unsigned smask0 = u ? 0u : 0x80000000u;

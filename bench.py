@@ -219,7 +219,8 @@ MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                    "XCD-contiguous sites)",
                 7: "k_mrhs_dma (input copied to LDS, 2 m-tiles per wavefront)",
                 8: "k_mrhs_dma (input copied to LDS, 2 m-tiles per wavefront, non-temporal "
-                   "link loads, XCD-contiguous sites)"}
+                   "link loads, XCD-contiguous sites)",
+                9: "k_mrhs_xblk (LDS-staged B', non-temporal link loads, XCD-block sites)"}
 
 
 def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, keep=None):

@@ -437,6 +437,31 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_min_waves<KIND, NT>())) void k_m
                                   in_scale, out, nbr, nrhs, rhs0, nr, dag);
 }
 
+// XCD-block site order: XCD k = b mod 8 takes the sites with x, y and z in
+// halves (k & 1, k >> 1 & 1, k >> 2 & 1) of the lattice and every t, walked
+// x fastest.  Against contiguous site ranges (whole t slices for 8^4) this
+// shrinks the neighbour inputs that an XCD reads from other XCDs' ranges.
+// dims packs L0 | L1 << 8 | L2 << 16 | L3 << 24 (L0, L1, L2 even).
+__device__ __forceinline__ int site_of_block_xblk(int b, int dims) {
+  const int L0 = dims & 255, L1 = (dims >> 8) & 255, L2 = (dims >> 16) & 255;
+  const int h0 = L0 >> 1, h1 = L1 >> 1, h2 = L2 >> 1;
+  const int k = b & 7, i = b >> 3;
+  const int x0 = (k & 1) * h0 + i % h0;
+  const int x1 = ((k >> 1) & 1) * h1 + (i / h0) % h1;
+  const int x2 = ((k >> 2) & 1) * h2 + (i / (h0 * h1)) % h2;
+  const int x3 = i / (h0 * h1 * h2);
+  return x0 + L0 * (x1 + L1 * (x2 + L2 * x3));
+}
+
+template <int KIND, int N, int NT, int LAUX>
+__global__ __launch_bounds__(N / 16 * 64, (mrhs_min_waves<KIND, NT>())) void k_mrhs_xblk(
+    const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
+    const float* __restrict__ in_scale, float* __restrict__ out, const int* __restrict__ nbr,
+    int nrhs, int rhs0, int nr, int dag, int dims) {
+  mrhs_lds_site<KIND, N, NT, LAUX>(site_of_block_xblk((int)blockIdx.x, dims), A, ascale, in_,
+                                  in_scale, out, nbr, nrhs, rhs0, nr, dag);
+}
+
 // Register form: no LDS and no barrier.  Each wavefront owns MPW consecutive
 // m-tiles of one site and builds the B' fragments it needs in registers
 // straight from 16-byte loads of the neighbour's input (lanes l and l^1 read
@@ -917,9 +942,10 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
 // the site count is not a multiple of 8), 5 = persistent LDS-staged form,
 // 6 = persistent with non-temporal link loads and the XCD-contiguous order,
 // 7 = direct-to-LDS input with two m-tiles per wavefront, 8 = variant 7 with
-// non-temporal link loads and the XCD-contiguous order
-constexpr int MRHS_VARIANTS = 9;
-static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6 || v == 8) ? 2 : 0; }
+// non-temporal link loads and the XCD-contiguous order, 9 = variant 1 with
+// the XCD-block site order (variant 4 where the lattice does not allow it)
+constexpr int MRHS_VARIANTS = 10;
+static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6 || v == 8 || v == 9) ? 2 : 0; }
 static int variant_xchunk(int v, int blocks) {
   return ((v == 3 || v == 4 || v == 6 || v == 8) && blocks % 8 == 0) ? blocks / 8 : 0;
 }
@@ -956,9 +982,9 @@ static void launch_persistent(int variant, const MrhsBatch& bt, int nrhs, int rh
 }
 
 template <int KIND, int N, int NT>
-static void launch_variant(int variant, int V, const uint4* A, const float* as, const void* in,
-                           const float* is, float* out, const int* nbr, int nrhs, int rhs0, int nr,
-                           int dag, hipStream_t s) {
+static void launch_variant(int variant, int V, int dims, const uint4* A, const float* as,
+                           const void* in, const float* is, float* out, const int* nbr, int nrhs,
+                           int rhs0, int nr, int dag, hipStream_t s) {
   const dim3 g(V);
   const int xc = variant_xchunk(variant, V);
   const int b = N / 16 * 64;
@@ -982,6 +1008,16 @@ static void launch_variant(int variant, int V, const uint4* A, const float* as, 
                                                         dag);
     return;
   }
+  if (variant == 9) {
+    if (dims) {
+      k_mrhs_xblk<KIND, N, NT, 2><<<g, b, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag,
+                                                   dims);
+    } else {
+      k_mrhs<KIND, N, NT, 2><<<g, b, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag,
+                                              variant_xchunk(4, V));
+    }
+    return;
+  }
   if (variant == 7 || variant == 8) {
     if (variant_laux(variant) == 2)
       k_mrhs_dma<KIND, N, NT, 2><<<g, N / 32 * 64, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0,
@@ -998,13 +1034,14 @@ static void launch_variant(int variant, int V, const uint4* A, const float* as, 
 }
 
 template <int KIND, int N>
-static void launch_mrhs_n(int variant, int NT, int V, const uint4* A, const float* as,
+static void launch_mrhs_n(int variant, int NT, int V, int dims, const uint4* A, const float* as,
                           const void* in, const float* is, float* out, const int* nbr, int nrhs,
                           int rhs0, int nr, int dag, hipStream_t s) {
   switch (NT) {
 #define CASE(nt)                                                                              \
   case nt:                                                                                    \
-    launch_variant<KIND, N, nt>(variant, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s); \
+    launch_variant<KIND, N, nt>(variant, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, \
+                                s);                                                           \
     break;
     CASE(1) CASE(2) CASE(3) CASE(4) CASE(6) CASE(8)
 #undef CASE
@@ -1015,26 +1052,38 @@ static void launch_mrhs_n(int variant, int NT, int V, const uint4* A, const floa
 }
 
 template <int KIND>
-static void launch_mrhs(int variant, int N, int NT, int V, const uint4* A, const float* as,
+static void launch_mrhs(int variant, int N, int NT, int V, int dims, const uint4* A, const float* as,
                         const void* in, const float* is, float* out, const int* nbr, int nrhs,
                         int rhs0, int nr, int dag, hipStream_t s) {
   if (N == 32)
-    launch_mrhs_n<KIND, 32>(variant, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs_n<KIND, 32>(variant, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
   else if (N == 64)
-    launch_mrhs_n<KIND, 64>(variant, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs_n<KIND, 64>(variant, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
   else
     throw Error(MGC_ERR_SHAPE, "multi-rhs stencil supports nbasis 32 and 64");
 }
 
-static void launch_any(int mp, int variant, int N, int NT, int V, const uint4* A, const float* as,
-                       const void* in, const float* is, float* out, const int* nbr, int nrhs,
-                       int rhs0, int nr, int dag, hipStream_t s) {
+static void launch_any(int mp, int variant, int N, int NT, int V, int dims, const uint4* A,
+                       const float* as, const void* in, const float* is, float* out,
+                       const int* nbr, int nrhs, int rhs0, int nr, int dag, hipStream_t s) {
   if (mp == MGC_F32)
-    launch_mrhs<MGC_F32>(variant, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs<MGC_F32>(variant, N, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
   else if (mp == MGC_F16)
-    launch_mrhs<MGC_F16>(variant, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs<MGC_F16>(variant, N, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
   else
-    launch_mrhs<MGC_BF16>(variant, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+    launch_mrhs<MGC_BF16>(variant, N, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag,
+                          s);
+}
+
+// packed lattice extents for the XCD-block site order (0 where it does not
+// apply: odd or large extents of the first three dimensions, or a site
+// count that is not a multiple of 8)
+static int xblk_dims(const Lat4& L) {
+  for (int d = 0; d < 4; ++d)
+    if (L.L[d] > 127) return 0;
+  for (int d = 0; d < 3; ++d)
+    if (L.L[d] % 2) return 0;
+  return L.L[0] | (L.L[1] << 8) | (L.L[2] << 16) | (L.L[3] << 24);
 }
 
 // Autotuner: the first launch of a (precision, nbasis, column tiles) shape
@@ -1069,15 +1118,17 @@ static int time_variants(int count, F&& launch, hipStream_t s) {
   return bestv;
 }
 
-static int tuned_variant(int mp, int N, int NT, int V, const uint4* A, const float* as,
-                         const void* in, const float* is, float* out, const int* nbr, int nrhs,
-                         int rhs0, int nr, int dag, hipStream_t s) {
+static int tuned_variant(int mp, int N, int NT, int V, int dims, const uint4* A,
+                         const float* as, const void* in, const float* is, float* out,
+                         const int* nbr, int nrhs, int rhs0, int nr, int dag, hipStream_t s) {
   if (g_mrhs_forced >= 0) return g_mrhs_forced;
   int& slot = g_mrhs_choice[mp][N == 64 ? 1 : 0][NT];
   if (slot) return slot - 1;
   const int v = time_variants(
       MRHS_VARIANTS,
-      [&](int v) { launch_any(mp, v, N, NT, V, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s); },
+      [&](int v) {
+        launch_any(mp, v, N, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
+      },
       s);
   slot = v + 1;
   return v;
@@ -1283,10 +1334,11 @@ int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scal
       const int nr = nrhs - r0 < 64 ? nrhs - r0 : 64;
       const int NT = tiles_for(nr);
       const uint4* A = (const uint4*)op->mrhs_links;
-      const int v = tuned_variant(mrhs_prec, N, NT, V, A, op->mrhs_scales, src, sc, (float*)out,
-                                  op->nbr_full, nrhs, r0, nr, dg, s);
-      launch_any(mrhs_prec, v, N, NT, V, A, op->mrhs_scales, src, sc, (float*)out, op->nbr_full,
-                 nrhs, r0, nr, dg, s);
+      const int dims = xblk_dims(op->lat);
+      const int v = tuned_variant(mrhs_prec, N, NT, V, dims, A, op->mrhs_scales, src, sc,
+                                  (float*)out, op->nbr_full, nrhs, r0, nr, dg, s);
+      launch_any(mrhs_prec, v, N, NT, V, dims, A, op->mrhs_scales, src, sc, (float*)out,
+                 op->nbr_full, nrhs, r0, nr, dg, s);
     }
     if (qbuf) {
       MGC_HIP(hipFreeAsync(qbuf, s));

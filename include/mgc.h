@@ -204,7 +204,9 @@ int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const voi
  * 5 = persistent workgroups walking the sites, 6 = persistent with
  * non-temporal link loads and the XCD-contiguous order, 7 = input copied
  * straight to LDS with two m-tiles per wavefront, 8 = variant 7 with
- * non-temporal link loads and the XCD-contiguous order; the batched apply
+ * non-temporal link loads and the XCD-contiguous order, 9 = variant 1 with
+ * each XCD taking a 4d block of the lattice (halves of x, y and z, all t);
+ * the batched apply
  * uses variants 0, 1 and 3 to 6 and tunes itself when another one is
  * forced); -1 restores autotuning.  get_variant reports the variant in use
  * (-1 = not tuned yet). */

@@ -56,7 +56,7 @@ def test_mrhs_matches_oracle(gpu, dims, N, nrhs, mp):
         assert max(rel(ref[:, k], got_h[:, k]) for k in range(nrhs)) < TOL[mp](N)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 def test_mrhs_every_kernel_variant(gpu, variant, mp):
     """Each kernel variant the autotuner may pick is checked on its own."""
@@ -80,7 +80,7 @@ def test_mrhs_every_kernel_variant(gpu, variant, mp):
         gpu.call("mgc_mrhs_set_variant", -1)
 
 
-@pytest.mark.parametrize("variant", [7, 8])
+@pytest.mark.parametrize("variant", [7, 8, 9])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 @pytest.mark.parametrize("dims,N,nrhs", [((2, 2, 2, 2), 32, 70), ((2, 4, 2, 2), 64, 48),
                                          ((4, 2, 2, 2), 64, 5)])

@@ -945,6 +945,7 @@ __global__ __launch_bounds__(N / 16 * 64, (mrhs_persistent_min_waves<KIND, NT>()
 // non-temporal link loads and the XCD-contiguous order, 9 = variant 1 with
 // the XCD-block site order (variant 4 where the lattice does not allow it)
 constexpr int MRHS_VARIANTS = 10;
+static_assert(MRHS_VARIANTS <= 16, "the autotuner times at most 16 variants");
 static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6 || v == 8 || v == 9) ? 2 : 0; }
 static int variant_xchunk(int v, int blocks) {
   return ((v == 3 || v == 4 || v == 6 || v == 8) && blocks % 8 == 0) ? blocks / 8 : 0;
@@ -1087,34 +1088,39 @@ static int xblk_dims(const Lat4& L) {
 }
 
 // Autotuner: the first launch of a (precision, nbasis, column tiles) shape
-// times every variant on the caller's own operands (three launches each,
-// HIP events on the caller's stream; the outputs are rewritten by the final
-// launch) and keeps the fastest for the rest of the process.
+// times every variant on the caller's own operands and keeps the fastest for
+// the rest of the process.  Two rounds over all variants (the first also
+// brings the clocks up), five launches each after one warm-up launch, HIP
+// events on the caller's stream; a variant's time is its better round.  The
+// outputs are rewritten by the final launch.
 static int g_mrhs_choice[4][2][9];  // [prec][N==64][NT] -> variant + 1 (0 = untuned)
 static int g_mrhs_forced = -1;      // >= 0: use this variant (set by mgc_mrhs_set_variant)
 
 template <typename F>
 static int time_variants(int count, F&& launch, hipStream_t s) {
+  constexpr int ROUNDS = 2, REPS = 5, MAXV = 16;
   hipEvent_t e0, e1;
   MGC_HIP(hipEventCreate(&e0));
   MGC_HIP(hipEventCreate(&e1));
-  float best = 1e30f;
-  int bestv = 0;
-  for (int v = 0; v < count; ++v) {
-    launch(v);  // warm-up
-    MGC_HIP(hipEventRecord(e0, s));
-    for (int k = 0; k < 3; ++k) launch(v);
-    MGC_HIP(hipEventRecord(e1, s));
-    MGC_HIP(hipEventSynchronize(e1));
-    float ms = 0.f;
-    MGC_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (ms < best) {
-      best = ms;
-      bestv = v;
+  float best_of[MAXV];
+  for (int v = 0; v < count && v < MAXV; ++v) best_of[v] = 1e30f;
+  for (int round = 0; round < ROUNDS; ++round) {
+    for (int v = 0; v < count && v < MAXV; ++v) {
+      launch(v);  // warm-up
+      MGC_HIP(hipEventRecord(e0, s));
+      for (int k = 0; k < REPS; ++k) launch(v);
+      MGC_HIP(hipEventRecord(e1, s));
+      MGC_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      MGC_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best_of[v]) best_of[v] = ms;
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  int bestv = 0;
+  for (int v = 1; v < count && v < MAXV; ++v)
+    if (best_of[v] < best_of[bestv]) bestv = v;
   return bestv;
 }
 

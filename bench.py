@@ -201,13 +201,14 @@ def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
     return res
 
 
-# HBM bytes moved per launch by the headline kernel (k_mrhs<F16, N=64, 6 column
-# tiles, non-temporal link loads>, the instantiation behind variants 1 and 4)
-# on the headline workload: two rocprofv3 --pmc runs, FETCH_SIZE then
-# WRITE_SIZE, each scaled by what the counter reads for a 1 GiB copy kernel
-# (FETCH x 2.000, WRITE x 1.000); the command lines and readings are in the
-# profile summary for this kernel under profiles/.
-HEADLINE_TRAFFIC_BYTES = {1: 938.9e6, 4: 938.9e6}
+# HBM bytes moved per launch by the headline kernel on the headline workload,
+# per autotuned variant (F16, N=64, 6 column tiles, non-temporal link loads:
+# k_mrhs for variants 1 and 4, k_mrhs_xblk for variant 9): two rocprofv3
+# --pmc runs, FETCH_SIZE then WRITE_SIZE, each scaled by what the counter
+# reads for a 1 GiB copy kernel (FETCH x 2.000, WRITE x 1.000); the command
+# lines and readings are in the profile summary for this kernel under
+# profiles/.
+HEADLINE_TRAFFIC_BYTES = {1: 938.9e6, 4: 938.9e6, 9: 900.1e6}
 
 MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 1: "k_mrhs (LDS-staged B', non-temporal link loads)",

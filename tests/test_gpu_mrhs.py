@@ -83,11 +83,13 @@ def test_mrhs_every_kernel_variant(gpu, variant, mp):
 @pytest.mark.parametrize("variant", [7, 8, 9])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 @pytest.mark.parametrize("dims,N,nrhs", [((2, 2, 2, 2), 32, 70), ((2, 4, 2, 2), 64, 48),
-                                         ((4, 2, 2, 2), 64, 5)])
+                                         ((4, 2, 2, 2), 64, 5), ((3, 2, 2, 2), 32, 12)])
 def test_mrhs_direct_lds_variants_bit_equal(gpu, dims, N, nrhs, mp, variant):
-    """The direct-to-LDS forms keep the arithmetic order of the LDS-staged
-    form: outputs equal variant 0's bit for bit (FP16/BF16 inputs quantised,
-    column passes with rhs0 > 0 and partly filled tiles included)."""
+    """The direct-to-LDS forms and the XCD-block site order keep the
+    arithmetic order of the LDS-staged form: outputs equal variant 0's bit
+    for bit (FP16/BF16 inputs quantised, column passes with rhs0 > 0, partly
+    filled tiles, and an odd extent, where the XCD-block order falls back to
+    contiguous site ranges, included)."""
     print(f"[case] mrhs direct-lds dims={dims} N={N} nrhs={nrhs} prec={mp} v={variant}",
           flush=True)
     links = rand_links(dims, N, 651)

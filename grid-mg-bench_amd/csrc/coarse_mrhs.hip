@@ -226,12 +226,14 @@ __device__ __forceinline__ uint4 bprime(uint4 r, int u, bool neg) {
   return o;
 }
 
-// Order in which a workgroup visits the nine points: self, +x, -x, +y, -y,
-// +z, -z, +t, -t.  Workgroups of one XCD run nearly in step, so consecutive
+// Order in which a workgroup visits the nine points: self, -t, +x, -x, +y,
+// -y, +z, -z, +t.  Workgroups of one XCD run nearly in step, so consecutive
 // steps read largely the same neighbour inputs (shifted by one lattice
-// spacing) while they are still in L2.
+// spacing) while they are still in L2.  With sites walked t slowest on each
+// XCD, the -t inputs were read by the workgroups that ran just before, and
+// the +t inputs are read again by the ones that run just after.
 __device__ __forceinline__ int mrhs_point(int q) {
-  return q == 0 ? 8 : ((q - 1) >> 1) + (((q - 1) & 1) ? 4 : 0);
+  return q == 0 ? 8 : q == 1 ? 7 : q == 8 ? 3 : ((q - 2) >> 1) + (((q - 2) & 1) ? 4 : 0);
 }
 
 // B' words of one staged 16-byte chunk for FP16/BF16 with a per-lane byte

@@ -204,13 +204,13 @@ def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
 # HBM bytes moved per launch by the headline kernel on the headline workload,
 # per autotuned variant (F16, N=64, 6 column tiles): two rocprofv3 --pmc
 # runs, FETCH_SIZE then WRITE_SIZE, each scaled by what the counter reads for
-# a 1 GiB copy kernel (FETCH x 2.000, WRITE x 1.000).  Variants 4 and 9 are
-# runs in which the tuner chose them; the others are their autotuner trials
-# in the same runs (variants 0/3 and 1/4 share a kernel, and a pair's trials
-# are averaged together).  The command lines and readings are in the profile
-# summary for this kernel under profiles/.
-HEADLINE_TRAFFIC_BYTES = {0: 1015.1e6, 1: 934.0e6, 2: 1017.7e6, 3: 1015.1e6, 4: 938.9e6,
-                          5: 1012.7e6, 6: 922.1e6, 7: 1030.1e6, 8: 919.9e6, 9: 900.1e6}
+# a 1 GiB copy kernel (FETCH x 2.000, WRITE x 1.000).  Each figure is the mean
+# of the variant's two autotuner trials in those runs (the tuner times every
+# variant in dispatch order, twice, six launches a trial), so variants that
+# share a kernel are still told apart.  The command lines and readings are in
+# the profile summary for this kernel under profiles/.
+HEADLINE_TRAFFIC_BYTES = {0: 987.5e6, 1: 898.2e6, 2: 996.4e6, 3: 1023.9e6, 4: 925.4e6,
+                          5: 987.6e6, 6: 921.8e6, 7: 1008.1e6, 8: 921.7e6, 9: 852.3e6}
 
 MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 1: "k_mrhs (LDS-staged B', non-temporal link loads)",

@@ -5,6 +5,14 @@
 #include <type_traits>
 #include <vector>
 
+// No floating-point contraction in this file: every a * b + c rounds the
+// product and the sum separately, so the fused passes and the two-kernel forms
+// they replace (same expressions, same element order) give the same bits in
+// FP64 as well.  With contraction left to the compiler, each kernel's choice
+// of which products to fold into an FMA differed.  The kernels are bound by
+// memory traffic; the extra FP64 instructions do not show.
+#pragma clang fp contract(off)
+
 namespace mgc {
 
 BBlas::BBlas(hipStream_t st) : s(st) {
@@ -165,14 +173,30 @@ __global__ __launch_bounds__(256) void k_bdot_part_f2(const cplx<float>* __restr
 // vector precision), then the per-rhs partials of <V_j, w>_r for j < k and of
 // |w|^2_r (row k) for the updated w, on the grid and with the per-thread
 // element order of k_bdot_part / k_bdot_part_f2 (so the same partials), with
-// V read once from memory (the second read of V_j by the same thread is an
-// L2 hit).  KT >= k.
+// V read once from memory.  With KT > 8 rows, the first BMSUB_STAGED values
+// of a thread's element are kept in that thread's own LDS slots between the
+// two passes (no thread reads another thread's slots, so no barrier): the
+// window of V a resident grid touches between the two reads (k x 16 B x 256
+// threads per workgroup) outgrows L2 at k = 16..24, and the second read would
+// go back to memory.  Rows j >= BMSUB_STAGED, and all rows when KT <= 8, are
+// read again (an L2 hit).  The staging slots and the block sums share one LDS
+// buffer: the sums are written only after the barrier that ends the element
+// loop for every thread.  Eight staged rows fill exactly the 32 KB the block
+// sums take, so occupancy is that of the unstaged kernel; sixteen (64 KB, two
+// workgroups per CU) ran slower than none.  KT >= k.
+constexpr int BMSUB_STAGED = 8;
 template <typename R, int KT, bool PAIRED>
 __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restrict__ c,
                                                    const cplx<R>* V, size_t ldv, int k,
                                                    cplx<R>* w, size_t n, int C, int dof, int nrhs,
                                                    double* __restrict__ part) {
-  __shared__ double sm[BD_ROWS][256][2];
+  constexpr int KS = KT > 8 ? BMSUB_STAGED : 0;
+  using Slot = typename std::conditional<PAIRED, float4, cplx<R>>::type;
+  constexpr size_t SUM_BYTES = sizeof(double) * BD_ROWS * 256 * 2;
+  constexpr size_t STAGE_BYTES = sizeof(Slot) * (KS > 0 ? KS : 1) * 256;
+  __shared__ __align__(16) unsigned char s_raw[SUM_BYTES > STAGE_BYTES ? SUM_BYTES : STAGE_BYTES];
+  double(&sm)[BD_ROWS][256][2] = *reinterpret_cast<double(*)[BD_ROWS][256][2]>(s_raw);
+  Slot(*s_v)[256] = reinterpret_cast<Slot(*)[256]>(s_raw);
   const size_t T = (size_t)gridDim.x * blockDim.x;
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   double re[KT + 1], im[KT];
@@ -189,17 +213,18 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
         if (j < k) {
           const cplx<double> cj = c[(size_t)j * nrhs + r];
           const cplx<R> v = V[(size_t)j * ldv + i];
+          if (j < KS) s_v[j][threadIdx.x] = v;
           xr += -1.0 * (cj.re * v.re - cj.im * v.im);
           xi += -1.0 * (cj.re * v.im + cj.im * v.re);
         }
       const cplx<R> b = {(R)xr, (R)xi};
       w[i] = b;
-      asm volatile("" ::: "memory");  // re-read V_j below instead of holding it
+      asm volatile("" ::: "memory");  // re-read V_j (LDS or memory) instead of holding it
       const double br = b.re, bi = b.im;
 #pragma unroll
       for (int j = 0; j < KT; ++j)
         if (j < k) {
-          const cplx<R> a = V[(size_t)j * ldv + i];
+          const cplx<R> a = j < KS ? s_v[j][threadIdx.x] : V[(size_t)j * ldv + i];
           const double ar = a.re, ai = a.im;
           re[j] += ar * br + ai * bi;
           im[j] += ar * bi - ai * br;
@@ -218,6 +243,7 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
         if (j < k) {
           const cplx<double> cj = c[(size_t)j * nrhs + r];
           const float4 v = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          if (j < KS) s_v[j][threadIdx.x] = v;
           x0r += -1.0 * (cj.re * v.x - cj.im * v.y);
           x0i += -1.0 * (cj.re * v.y + cj.im * v.x);
           x1r += -1.0 * (cj.re * v.z - cj.im * v.w);
@@ -229,7 +255,8 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
 #pragma unroll
       for (int j = 0; j < KT; ++j)
         if (j < k) {
-          const float4 a = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          const float4 a = j < KS ? s_v[j][threadIdx.x]
+                                  : reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
           re[j] += (double)a.x * b.x + (double)a.y * b.y;
           im[j] += (double)a.x * b.y - (double)a.y * b.x;
           re[j] += (double)a.z * b.z + (double)a.w * b.w;

@@ -4,6 +4,14 @@
 
 #include <type_traits>
 
+// No floating-point contraction in this file: every a * b + c rounds the
+// product and the sum separately, so the fused passes and the two-kernel forms
+// they replace (same expressions, same element order) give the same bits in
+// FP64 as well.  With contraction left to the compiler, each kernel's choice
+// of which products to fold into an FMA differed.  The kernels are bound by
+// memory traffic; the extra FP64 instructions do not show.
+#pragma clang fp contract(off)
+
 namespace mgc {
 
 Blas::Blas(hipStream_t st) : s(st) {
@@ -263,12 +271,23 @@ static bool pairable(const void* a, const void* b, size_t ldv, size_t n) {
 // vector precision), and the partial sums of <V_j, w> for j < k and of |w|^2
 // (row k) of the updated w, as k_vmultisub followed by k_mdot_part /
 // k_mdot_part_f2 would produce them (same grid, same per-thread element
-// order and arithmetic, so the same bits), reading V once from HBM (the
-// second read of V_j by the same thread is an L2 hit).  KT >= k.
+// order and arithmetic, so the same bits), reading V once from memory.  With
+// KT > 8 rows, the first MSUB_STAGED values of a thread's element are kept in
+// that thread's own LDS slots between the two passes (no thread reads another
+// thread's slots, so no barrier): the window of V a resident grid touches
+// between the two reads (k x 16 B x 256 threads per workgroup) outgrows L2 at
+// k = 16..24, and the second read would go back to memory.  Rows j >=
+// MSUB_STAGED, and all rows when KT <= 8, are read again (an L2 hit).
+// Sixteen staged rows (64 KB of LDS, two workgroups per CU) ran faster at
+// k = 17..24 than eight, and as fast at k = 9..16.  KT >= k.
+constexpr int MSUB_STAGED = 16;
 template <typename R, int KT, bool PAIRED>
 __global__ __launch_bounds__(256) void k_msub_mdot(const cplx<double>* __restrict__ c,
                                                    const cplx<R>* V, size_t ldv, int k,
                                                    cplx<R>* w, size_t n, double* part) {
+  constexpr int KS = KT > 8 ? MSUB_STAGED : 0;
+  using Slot = typename std::conditional<PAIRED, float4, cplx<R>>::type;
+  __shared__ Slot s_v[KS > 0 ? KS : 1][256];
   __shared__ cplx<double> s_c[KT];
   for (int j = threadIdx.x; j < k; j += blockDim.x) s_c[j] = c[j];
   __syncthreads();
@@ -285,18 +304,20 @@ __global__ __launch_bounds__(256) void k_msub_mdot(const cplx<double>* __restric
       for (int j = 0; j < KT; ++j)
         if (j < k) {
           const cplx<R> v = V[(size_t)j * ldv + i];
+          if (j < KS) s_v[j][threadIdx.x] = v;
           xr -= s_c[j].re * v.re - s_c[j].im * v.im;
           xi -= s_c[j].re * v.im + s_c[j].im * v.re;
         }
       const cplx<R> b = {(R)xr, (R)xi};
       w[i] = b;
-      // re-read V_j below instead of holding k loaded values in registers
+      // re-read V_j (from LDS or memory) below instead of holding k loaded
+      // values in registers
       asm volatile("" ::: "memory");
       const double br = b.re, bi = b.im;
 #pragma unroll
       for (int j = 0; j < KT; ++j)
         if (j < k) {
-          const cplx<R> a = V[(size_t)j * ldv + i];
+          const cplx<R> a = j < KS ? s_v[j][threadIdx.x] : V[(size_t)j * ldv + i];
           const double ar = a.re, ai = a.im;
           re[j] += ar * br + ai * bi;
           im[j] += ar * bi - ai * br;
@@ -314,6 +335,7 @@ __global__ __launch_bounds__(256) void k_msub_mdot(const cplx<double>* __restric
       for (int j = 0; j < KT; ++j)
         if (j < k) {
           const float4 v = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          if (j < KS) s_v[j][threadIdx.x] = v;
           x0r -= s_c[j].re * v.x - s_c[j].im * v.y;
           x0i -= s_c[j].re * v.y + s_c[j].im * v.x;
           x1r -= s_c[j].re * v.z - s_c[j].im * v.w;
@@ -325,7 +347,8 @@ __global__ __launch_bounds__(256) void k_msub_mdot(const cplx<double>* __restric
 #pragma unroll
       for (int j = 0; j < KT; ++j)
         if (j < k) {
-          const float4 a = reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
+          const float4 a = j < KS ? s_v[j][threadIdx.x]
+                                  : reinterpret_cast<const float4*>(V + (size_t)j * ldv)[i2];
           re[j] += (double)a.x * b.x + (double)a.y * b.y;
           im[j] += (double)a.x * b.y - (double)a.y * b.x;
           re[j] += (double)a.z * b.z + (double)a.w * b.w;

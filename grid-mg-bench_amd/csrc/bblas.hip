@@ -56,6 +56,18 @@ static int bred_blocks(const BShape& sh, bool pairs) {
 
 constexpr int BD_ROWS = 8;
 
+// i % C along a grid-stride walk i = i0, i0 + T, i0 + 2T, ...: one 64-bit
+// division at the start instead of one per element (C = nrhs * dof < 2^31).
+struct ModWalk {
+  unsigned m, step, c;
+  __device__ ModWalk(size_t i0, size_t T, int C)
+      : m((unsigned)(i0 % (size_t)C)), step((unsigned)(T % (size_t)C)), c((unsigned)C) {}
+  __device__ void next() {
+    m += step;
+    if (m >= c) m -= c;
+  }
+};
+
 // rows [BD_ROWS*y, ...): <V_j, w>_r for j < k, |w|^2_r for j == k
 template <typename R>
 __global__ __launch_bounds__(256) void k_bdot_part(const cplx<R>* __restrict__ V, size_t ldv,
@@ -64,10 +76,12 @@ __global__ __launch_bounds__(256) void k_bdot_part(const cplx<R>* __restrict__ V
                                                    int C, int dof, int nrhs,
                                                    double* __restrict__ part) {
   __shared__ double sm[BD_ROWS][256][2];
+  __shared__ int s_rt[256];  // the rhs of each thread's elements
   const int j0 = blockIdx.y * BD_ROWS;
   const int nj = min(BD_ROWS, rows - j0);
   const size_t T = (size_t)gridDim.x * blockDim.x;
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  s_rt[threadIdx.x] = (int)((g % C) / dof);
   double re[BD_ROWS], im[BD_ROWS];
 #pragma unroll
   for (int jj = 0; jj < BD_ROWS; ++jj) re[jj] = im[jj] = 0.0;
@@ -99,8 +113,7 @@ __global__ __launch_bounds__(256) void k_bdot_part(const cplx<R>* __restrict__ V
     const int jj = q / nrhs, rr = q % nrhs;
     double sr = 0.0, si = 0.0;
     for (int t = 0; t < (int)blockDim.x; ++t) {
-      const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
-      if ((int)((gt % C) / dof) == rr) {
+      if (s_rt[t] == rr) {
         sr += sm[jj][t][0];
         si += sm[jj][t][1];
       }
@@ -119,11 +132,13 @@ __global__ __launch_bounds__(256) void k_bdot_part_f2(const cplx<float>* __restr
                                                       size_t n, int C, int dof, int nrhs,
                                                       double* __restrict__ part) {
   __shared__ double sm[BD_ROWS][256][2];
+  __shared__ int s_rt[256];  // the rhs of each thread's element pairs
   const int j0 = blockIdx.y * BD_ROWS;
   const int nj = min(BD_ROWS, rows - j0);
   const size_t T = (size_t)gridDim.x * blockDim.x;
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t n2 = n / 2;
+  s_rt[threadIdx.x] = (int)((2 * (g % (size_t)(C / 2))) / dof);
   double re[BD_ROWS], im[BD_ROWS];
 #pragma unroll
   for (int jj = 0; jj < BD_ROWS; ++jj) re[jj] = im[jj] = 0.0;
@@ -152,13 +167,11 @@ __global__ __launch_bounds__(256) void k_bdot_part_f2(const cplx<float>* __restr
     sm[jj][threadIdx.x][1] = im[jj];
   }
   __syncthreads();
-  const int C2 = C / 2;
   for (int q = threadIdx.x; q < nj * nrhs; q += blockDim.x) {
     const int jj = q / nrhs, rr = q % nrhs;
     double sr = 0.0, si = 0.0;
     for (int t = 0; t < (int)blockDim.x; ++t) {
-      const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
-      if ((int)((2 * (gt % C2)) / dof) == rr) {
+      if (s_rt[t] == rr) {
         sr += sm[jj][t][0];
         si += sm[jj][t][1];
       }
@@ -197,16 +210,19 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
   __shared__ __align__(16) unsigned char s_raw[SUM_BYTES > STAGE_BYTES ? SUM_BYTES : STAGE_BYTES];
   double(&sm)[BD_ROWS][256][2] = *reinterpret_cast<double(*)[BD_ROWS][256][2]>(s_raw);
   Slot(*s_v)[256] = reinterpret_cast<Slot(*)[256]>(s_raw);
+  __shared__ int s_rt[256];  // the rhs of each thread's first element
   const size_t T = (size_t)gridDim.x * blockDim.x;
   const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  s_rt[threadIdx.x] = PAIRED ? (int)((2 * (g % (size_t)(C / 2))) / dof) : (int)((g % C) / dof);
   double re[KT + 1], im[KT];
 #pragma unroll
   for (int j = 0; j <= KT; ++j) re[j] = 0.0;
 #pragma unroll
   for (int j = 0; j < KT; ++j) im[j] = 0.0;
   if constexpr (!PAIRED) {
-    for (size_t i = g; i < n; i += T) {
-      const int r = (int)((i % C) / dof);
+    ModWalk mw(g, T, C);
+    for (size_t i = g; i < n; i += T, mw.next()) {
+      const int r = (int)(mw.m / (unsigned)dof);
       double xr = w[i].re, xi = w[i].im;
 #pragma unroll
       for (int j = 0; j < KT; ++j)
@@ -234,8 +250,9 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
   } else {
     const size_t n2 = n / 2;
     const int C2 = C / 2;
-    for (size_t i2 = g; i2 < n2; i2 += T) {
-      const int r = (int)((2 * (i2 % C2)) / dof);
+    ModWalk mw(g, T, C2);
+    for (size_t i2 = g; i2 < n2; i2 += T, mw.next()) {
+      const int r = (int)((2 * mw.m) / (unsigned)dof);
       const float4 wv = reinterpret_cast<const float4*>(w)[i2];
       double x0r = wv.x, x0i = wv.y, x1r = wv.z, x1i = wv.w;
 #pragma unroll
@@ -296,9 +313,7 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
         const int jj = q / nrhs, rr = q % nrhs;
         double sr = 0.0, si = 0.0;
         for (int t = 0; t < (int)blockDim.x; ++t) {
-          const size_t gt = (size_t)blockIdx.x * blockDim.x + t;
-          const int rt = PAIRED ? (int)((2 * (gt % (C / 2))) / dof) : (int)((gt % C) / dof);
-          if (rt == rr) {
+          if (s_rt[t] == rr) {
             sr += sm[jj][t][0];
             si += sm[jj][t][1];
           }
@@ -341,9 +356,11 @@ template <typename R>
 __global__ void k_bmultisub(const cplx<double>* __restrict__ c, const cplx<R>* __restrict__ V,
                             size_t ldv, int k, cplx<R>* __restrict__ w, size_t n, int C, int dof,
                             int nrhs, double sign) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)((i % C) / dof);
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const int r = (int)(mw.m / (unsigned)dof);
     double re = w[i].re, im = w[i].im;
     for (int j = 0; j < k; ++j) {
       const cplx<double> cj = c[(size_t)j * nrhs + r];
@@ -371,9 +388,11 @@ __global__ void k_bsubnorm(cplx<double>* h2, const cplx<R>* __restrict__ V, size
     if (blockIdx.x == 0) h2[(size_t)k * nrhs + r].im = hn;  // other blocks read only .re
   }
   __syncthreads();
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int r = (int)((i % C) / dof);
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const int r = (int)(mw.m / (unsigned)dof);
     double re = w[i].re, im = w[i].im;
     for (int j = 0; j < k; ++j) {
       const cplx<double> cj = h2[(size_t)j * nrhs + r];
@@ -389,9 +408,11 @@ __global__ void k_bsubnorm(cplx<double>* h2, const cplx<R>* __restrict__ V, size
 template <typename Rs, typename Rd>
 __global__ void k_bconvert(const cplx<Rs>* __restrict__ x, cplx<Rd>* __restrict__ y, size_t n,
                            int C, int dof, const cplx<double>* __restrict__ a) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const double s = a[(i % C) / dof].re;
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const double s = a[mw.m / (unsigned)dof].re;
     y[i] = {(Rd)((double)x[i].re * s), (Rd)((double)x[i].im * s)};
   }
 }

@@ -390,7 +390,6 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
                                                     WilsonTiles wt) {
   constexpr int CPS = 12 * (int)sizeof(cplx<R>) / 16;  // 16-byte chunks per record
   constexpr int RS = CPS + 1;  // record stride in chunks
-  constexpr bool PRELINK = sizeof(R) == 4;
   __shared__ uint4 s_rec[256 * RS];
   __shared__ int s_y[2][256];
   const int T = (int)blockDim.x;  // S sites x nrhs
@@ -437,34 +436,28 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
     if (!((mask >> h) & 1u)) continue;
     const int hn = next_hop(i);
     __syncthreads();  // every thread is done with the previous hop's records
-    // FP32: the loads this stage needs besides the records (this hop's link
-    // and the next hop's neighbour sites) are issued before the record copy,
-    // so that their latencies overlap its own: 8-12% faster.  The link takes
-    // 18 registers; in FP64 it would take 36, beyond the register budget, and
-    // moving the neighbour load alone measured slightly slower there.
-    int y = 0, ynext = 0;
-    cplx<R> lk[PRELINK ? 9 : 1];
-    if constexpr (PRELINK) {
-      y = s_y[cb][t / nrhs];  // written before the previous stage's last barrier
-      if (h != 8 && live) {
-        const cplx<R>* u = U + (size_t)(h & 3) * 9 * sv + (h < 4 ? x : y);
+    // The loads this stage needs besides the records (this hop's link, 9
+    // complex, and the next hop's neighbour sites) are issued before the
+    // record copy, so that their latencies overlap its own instead of
+    // following the barrier: FP32 8-12% faster, FP64 3% (252 registers, the
+    // same two wavefronts per SIMD as without).
+    const int y = s_y[cb][t / nrhs];  // written before the previous stage's last barrier
+    cplx<R> lk[9];
+    if (h != 8 && live) {
+      const cplx<R>* u = U + (size_t)(h & 3) * 9 * sv + (h < 4 ? x : y);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) lk[k] = u[(size_t)k * sv];
-      }
-      if (hn >= 0 && t < S) ynext = hop_site(t, hn);
+      for (int k = 0; k < 9; ++k) lk[k] = u[(size_t)k * sv];
     }
+    const int ynext = hn >= 0 && t < S ? hop_site(t, hn) : 0;
     wilson_copy<CPS, RS>(s_rec, in4, s_y[cb], t, T, nrhs);
-    if (hn >= 0 && t < S) s_y[cb ^ 1][t] = PRELINK ? ynext : hop_site(t, hn);
+    if (hn >= 0 && t < S) s_y[cb ^ 1][t] = ynext;
     __syncthreads();
-    if constexpr (!PRELINK) y = s_y[cb][t / nrhs];
     cb ^= 1;
     if (live) {
       if (h == 8)
         wilson_diag_term<R>(acc, diag, x, sv, rec);
-      else if constexpr (PRELINK)
-        wilson_hop_apply<R>(acc, G, h, [&](int k) { return lk[k]; }, rec);
       else
-        wilson_hop_term<R>(acc, U, x, y, sv, G, h, rec);
+        wilson_hop_apply<R>(acc, G, h, [&](int k) { return lk[k]; }, rec);
     }
   }
   // result: records through LDS, then coalesced stores of whole chunks

@@ -220,20 +220,18 @@ __device__ __forceinline__ int wilson_tile_site(const WilsonTiles& w, int b) {
 }
 
 // Clover term of one (site, rhs): acc += D(x) v with the packed hermitian
-// blocks of chirality l at diag[l][slot][x]; sp(e) returns element e of the
-// site's spinor.
-template <typename R, typename SP>
-__device__ __forceinline__ void wilson_diag_term(cplx<R>* acc, const cplx<R>* __restrict__ diag,
-                                                 int x, int sv, SP&& sp) {
+// blocks of chirality l; dk(l, slot) returns packed slot `slot` of block l,
+// sp(e) element e of the site's spinor.
+template <typename R, typename DK, typename SP>
+__device__ __forceinline__ void wilson_diag_apply(cplx<R>* acc, DK&& dk, SP&& sp) {
 #pragma unroll
   for (int l = 0; l < 2; ++l) {
-    const cplx<R>* D = diag + (size_t)l * 18 * sv + x;  // packed slot s at D[s * sv]
     cplx<R> v[6];
 #pragma unroll
     for (int b = 0; b < 6; ++b) v[b] = sp(6 * l + b);
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-      const cplx<R> dd = D[(size_t)a * sv];  // (d_{2a}, d_{2a+1})
+      const cplx<R> dd = dk(l, a);  // (d_{2a}, d_{2a+1})
       acc[6 * l + 2 * a].re += dd.re * v[2 * a].re;
       acc[6 * l + 2 * a].im += dd.re * v[2 * a].im;
       acc[6 * l + 2 * a + 1].re += dd.im * v[2 * a + 1].re;
@@ -244,11 +242,20 @@ __device__ __forceinline__ void wilson_diag_term(cplx<R>* acc, const cplx<R>* __
     for (int a = 0; a < 6; ++a)
 #pragma unroll
       for (int b = a + 1; b < 6; ++b, ++slot) {
-        const cplx<R> u = D[(size_t)slot * sv];
+        const cplx<R> u = dk(l, slot);
         cmac(acc[6 * l + a], u, v[b]);       // upper: D_ab v_b
         cmac_conj(acc[6 * l + b], u, v[a]);  // lower: conj(D_ab) v_a
       }
   }
+}
+
+// The clover term with the blocks read from the site-strided SoA field
+// (packed slot s of block l at diag[(l * 18 + s) * sv + x]).
+template <typename R, typename SP>
+__device__ __forceinline__ void wilson_diag_term(cplx<R>* acc, const cplx<R>* __restrict__ diag,
+                                                 int x, int sv, SP&& sp) {
+  wilson_diag_apply<R>(acc, [&](int l, int s) { return diag[((size_t)l * 18 + s) * sv + x]; },
+                       sp);
 }
 
 // Hopping term p with the rank-2 spin projector: for the upper spins s = 0, 1
@@ -437,14 +444,15 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
     const int hn = next_hop(i);
     __syncthreads();  // every thread is done with the previous hop's records
     // The loads this stage needs besides the records (this hop's link, 9
-    // complex, and the next hop's neighbour sites) are issued before the
-    // record copy, so that their latencies overlap its own instead of
-    // following the barrier: FP32 8-12% faster, FP64 3% (252 registers, the
-    // same two wavefronts per SIMD as without).
+    // complex, or in the self stage the first 9 clover slots, and the next
+    // hop's neighbour sites) are issued before the record copy, so that
+    // their latencies overlap its own instead of following the barrier:
+    // FP32 8-12% faster, FP64 3% (252 registers, the same two wavefronts per
+    // SIMD as without).
     const int y = s_y[cb][t / nrhs];  // written before the previous stage's last barrier
-    cplx<R> lk[9];
-    if (h != 8 && live) {
-      const cplx<R>* u = U + (size_t)(h & 3) * 9 * sv + (h < 4 ? x : y);
+    cplx<R> lk[9];  // the hop's link, or in the self stage clover slots 0..8 of block 0
+    if (live) {
+      const cplx<R>* u = h == 8 ? diag + x : U + (size_t)(h & 3) * 9 * sv + (h < 4 ? x : y);
 #pragma unroll
       for (int k = 0; k < 9; ++k) lk[k] = u[(size_t)k * sv];
     }
@@ -455,7 +463,9 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
     cb ^= 1;
     if (live) {
       if (h == 8)
-        wilson_diag_term<R>(acc, diag, x, sv, rec);
+        wilson_diag_apply<R>(acc, [&](int l, int s) {
+          return l == 0 && s < 9 ? lk[s] : diag[((size_t)l * 18 + s) * sv + x];
+        }, rec);
       else
         wilson_hop_apply<R>(acc, G, h, [&](int k) { return lk[k]; }, rec);
     }

@@ -66,6 +66,34 @@ __global__ void k_extract_basis(const cplx<double>* __restrict__ U, int nb, int 
   }
 }
 
+// fine fields [Vf][nr][Ef] holding the doubled basis columns n = n0 .. n0+nr-1:
+// column n is the chirality-(n / nb) part of basis vector n % nb
+__global__ void k_extract_doubled(const cplx<double>* __restrict__ U, int nb, int Ef, int n0,
+                                  int nr, size_t total, const int* __restrict__ blk_fine,
+                                  cplx<double>* __restrict__ out) {
+  size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; t < total; t += stride) {  // t = ((xc*Vb + xb)*nr + r)*Ef + e
+    const int e = (int)(t % Ef);
+    const size_t q = t / Ef;
+    const int r = (int)(q % nr);
+    const size_t rr = q / nr;  // panel row xc*Vb + xb
+    const int n = n0 + r, l = n / nb, c = n % nb;
+    cplx<double> v = U[(rr * Ef + e) * nb + c];
+    if ((2 * e >= Ef) != (l == 1)) v = {0.0, 0.0};
+    out[((size_t)blk_fine[rr] * nr + r) * Ef + e] = v;
+  }
+}
+
+void agg_extract_doubled(mgc_aggregation* a, int n0, int nr, void* out, hipStream_t s) {
+  MGC_CHECK(a && a->basis_set, MGC_ERR_STATE, "aggregation basis not set");
+  MGC_CHECK(n0 >= 0 && nr >= 1 && n0 + nr <= 2 * a->nb, MGC_ERR_ARG, "basis columns out of range");
+  const size_t total = (size_t)a->Vc * a->Vb * nr * a->Ef;
+  k_extract_doubled<<<grid_for(total, 256, 16384), 256, 0, s>>>(
+      a->basis64, a->nb, a->Ef, n0, nr, total, a->blk_fine, (cplx<double>*)out);
+  MGC_LAUNCH_CHECK();
+}
+
 __global__ void k_basis_to_f32(const cplx<double>* __restrict__ src, cplx<float>* __restrict__ dst,
                                size_t n) {
   size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -374,14 +402,18 @@ __device__ inline void wave_sum2(double a, double b, double* out) {
   }
 }
 
+// flags != null: only the panels whose flag is set (the fallback of the
+// Cholesky form below)
 __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __restrict__ U,
                                                                 int nb, int Vb, int Ef,
-                                                                int passes) {
+                                                                int passes,
+                                                                const int* __restrict__ flags) {
   constexpr int NW = MGS_THREADS / 64;
   // s_part[w][u][2]: wavefront partial sums; row MGS_NBMAX carries |v|^2
   __shared__ double s_part[NW][MGS_NBMAX + 1][2];
   __shared__ cplx<double> s_alpha[MGS_NBMAX];
   __shared__ double s_n2;
+  if (flags && !flags[blockIdx.x]) return;  // uniform over the workgroup
   const int xc = blockIdx.x >> 1;
   const int k = blockIdx.x & 1;
   const int half = Ef / 2;
@@ -486,6 +518,209 @@ __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Block orthonormalisation by Cholesky QR in two rounds (CholQR2), one
+// workgroup per (block, chirality) panel V (rows = block sites x Ef/2
+// elements, nb columns):
+//   read 1 (k_cholqr_gram, round 0):  G1 = V^H V,  R1 = chol(G1),  T1 = R1^-1
+//   read 2 (k_cholqr_gram, round 1):  V1 = V T1 on the fly (not stored),
+//                                     G2 = V1^H V1, R2 = chol(G2), T2 = R2^-1
+//   read 3 (k_cholqr_apply):          V <- (V T1) T2  (in place, the only write)
+// V1 is formed by the same code in rounds 1 and 3, so the second round
+// corrects exactly the V1 that the apply produces (applying T1 T2 at once
+// would lose that: the columns of V T1 carry errors of eps * kappa(V)).
+// For a panel of full rank Q = V T is the unique QR factor with a positive
+// real R diagonal, which is what the reference's modified Gram-Schmidt
+// computes (include/AggregationKernelsRework.h:266-341), here to O(kappa eps).
+// A panel whose Cholesky pivot drops below CHOLQR_PIVOT_FLOOR of its column
+// norm (nearly dependent columns, kappa > ~1e7) is flagged, left untouched by
+// the apply, and orthonormalised by the column sweep instead.
+// T1 and T2 are nb x nb complex per panel in a workspace of the handle; the
+// panel is streamed in chunks of CQ_CH rows through LDS.
+// ---------------------------------------------------------------------------
+// 1: always the column sweep (mgc_aggregation_set_ortho_method)
+static int g_ortho_sweep_only = 0;
+constexpr int CQ_THREADS = 256;
+constexpr int CQ_CH = 32;         // panel rows per staged chunk
+constexpr int CQ_NBMAX = 32;      // widest panel of the Cholesky form
+constexpr double CHOLQR_PIVOT_FLOOR = 1e-14;  // on |pivot|^2 / G_jj
+
+__device__ __forceinline__ size_t cq_row_off(int r, int nb, int Vb, int Ef, int xc, int k) {
+  const int half = Ef / 2;
+  return agg_row(nb, Vb, Ef, xc, r / half, half * k + r % half);
+}
+
+// Cholesky G = R^H R (upper R, real positive diagonal) of the nb x nb matrix
+// in sG (overwritten by R), and sX <- R^-1 (upper triangular).  Run by the
+// whole workgroup; the factorisation itself by wavefront 0.
+__device__ void cq_factor(cplx<double> (*sG)[CQ_NBMAX + 1], cplx<double> (*sX)[CQ_NBMAX + 1],
+                          int nb, int* flag) {
+  const int t = threadIdx.x;
+  __shared__ int s_bad;
+  if (t == 0) s_bad = 0;
+  __syncthreads();
+  if (t < 64) {
+    // right-looking factorisation, lane k owns column k
+    for (int j = 0; j < nb; ++j) {
+      const double gjj = sG[j][j].re;
+      double d = gjj;
+      for (int i = 0; i < j; ++i) d -= sG[i][j].re * sG[i][j].re + sG[i][j].im * sG[i][j].im;
+      const bool bad = !(d > CHOLQR_PIVOT_FLOOR * gjj) || !(gjj > 0.0);
+      const double rjj = bad ? 1.0 : sqrt(d);
+      if (bad && t == 0) s_bad = 1;
+      // row j of R: R[j][k] = (G[j][k] - sum_{i<j} conj(R[i][j]) R[i][k]) / R[j][j]
+      for (int k = j + 1 + t; k < nb; k += 64) {
+        cplx<double> a = sG[j][k];
+        for (int i = 0; i < j; ++i) {
+          const cplx<double> rij = sG[i][j], rik = sG[i][k];
+          a.re -= rij.re * rik.re + rij.im * rik.im;
+          a.im -= rij.re * rik.im - rij.im * rik.re;
+        }
+        sG[j][k] = {a.re / rjj, a.im / rjj};
+      }
+      // one wavefront: the fences keep its LDS writes ahead of the next reads
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+      if (t == 0) sG[j][j] = {rjj, 0.0};
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+    // X = R^-1 (upper), lane j owns column j, from the diagonal upwards
+    for (int j = t; j < nb; j += 64) {
+      for (int i = nb - 1; i > j; --i) sX[i][j] = {0.0, 0.0};
+      sX[j][j] = {1.0 / sG[j][j].re, 0.0};
+      for (int i = j - 1; i >= 0; --i) {
+        cplx<double> a = {0.0, 0.0};
+        for (int k = i + 1; k <= j; ++k) cmac(a, sG[i][k], sX[k][j]);
+        const double inv = 1.0 / sG[i][i].re;
+        sX[i][j] = {-a.re * inv, -a.im * inv};
+      }
+    }
+  }
+  __syncthreads();
+  if (s_bad && t == 0) *flag = 1;
+}
+
+// One Gram round over every panel (blockIdx.x = 2 * coarse site + chirality),
+// writing R^-1 of the round to T[round].  Round 1 skips flagged panels and
+// forms the rows of V T1 before the Gram sum.
+__device__ __forceinline__ cplx<double> cq_row_times(const cplx<double>* v,
+                                                     cplx<double> (*T)[CQ_NBMAX + 1], int j) {
+  cplx<double> a = {0.0, 0.0};
+  for (int i = 0; i <= j; ++i) cmac(a, v[i], T[i][j]);
+  return a;
+}
+
+template <int TT>
+__global__ __launch_bounds__(CQ_THREADS) void k_cholqr_gram(const cplx<double>* __restrict__ U,
+                                                            cplx<double>* __restrict__ Tg,
+                                                            int* __restrict__ flags, int nb,
+                                                            int Vb, int Ef, int round) {
+  __shared__ cplx<double> sV[CQ_CH][CQ_NBMAX + 1];
+  __shared__ cplx<double> sW[CQ_CH][CQ_NBMAX + 1];
+  __shared__ cplx<double> sT[CQ_NBMAX][CQ_NBMAX + 1];
+  __shared__ cplx<double> sG[CQ_NBMAX][CQ_NBMAX + 1];
+  const int panel = blockIdx.x;
+  const int xc = panel >> 1, k = panel & 1;
+  const int t = threadIdx.x;
+  if (round == 1 && flags[panel]) return;  // uniform over the workgroup
+  const size_t tstride = (size_t)gridDim.x * nb * nb;  // T1 block, then T2 block
+  if (round == 1) {
+    const cplx<double>* T1 = Tg + (size_t)panel * nb * nb;
+    for (int idx = t; idx < nb * nb; idx += CQ_THREADS) sT[idx / nb][idx % nb] = T1[idx];
+  }
+  const int rows = Vb * (Ef / 2);
+  const int nt = (nb + TT - 1) / TT;  // thread tiles per dimension
+  const bool active = t < nt * nt;
+  const int i0 = (t / nt) * TT, j0 = (t % nt) * TT;
+  cplx<double> acc[TT][TT];
+#pragma unroll
+  for (int a = 0; a < TT; ++a)
+#pragma unroll
+    for (int b = 0; b < TT; ++b) acc[a][b] = {0.0, 0.0};
+  for (int r0 = 0; r0 < rows; r0 += CQ_CH) {
+    const int nrow = min(CQ_CH, rows - r0);
+    __syncthreads();  // previous chunk consumed
+    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
+      const int r = idx / nb, i = idx % nb;
+      sV[r][i] = U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i];
+    }
+    __syncthreads();
+    cplx<double>(*src)[CQ_NBMAX + 1] = sV;
+    if (round == 1) {
+      // W = V T1 (T1 upper triangular)
+      for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
+        const int r = idx / nb, j = idx % nb;
+        sW[r][j] = cq_row_times(sV[r], sT, j);
+      }
+      __syncthreads();
+      src = sW;
+    }
+    if (active) {
+      for (int r = 0; r < nrow; ++r) {
+        cplx<double> x[TT], y[TT];
+#pragma unroll
+        for (int a = 0; a < TT; ++a) x[a] = i0 + a < nb ? src[r][i0 + a] : cplx<double>{0.0, 0.0};
+#pragma unroll
+        for (int b = 0; b < TT; ++b) y[b] = j0 + b < nb ? src[r][j0 + b] : cplx<double>{0.0, 0.0};
+#pragma unroll
+        for (int a = 0; a < TT; ++a)
+#pragma unroll
+          for (int b = 0; b < TT; ++b) cmac_conj(acc[a][b], x[a], y[b]);
+      }
+    }
+  }
+  __syncthreads();
+  if (active)
+#pragma unroll
+    for (int a = 0; a < TT; ++a)
+#pragma unroll
+      for (int b = 0; b < TT; ++b)
+        if (i0 + a < nb && j0 + b < nb) sG[i0 + a][j0 + b] = acc[a][b];
+  __syncthreads();
+  cq_factor(sG, sV, nb, &flags[panel]);
+  cplx<double>* Tp = Tg + round * tstride + (size_t)panel * nb * nb;
+  for (int idx = t; idx < nb * nb; idx += CQ_THREADS) Tp[idx] = sV[idx / nb][idx % nb];
+}
+
+// V <- (V T1) T2 in place on every panel that is not flagged
+__global__ __launch_bounds__(CQ_THREADS) void k_cholqr_apply(cplx<double>* __restrict__ U,
+                                                             const cplx<double>* __restrict__ Tg,
+                                                             const int* __restrict__ flags,
+                                                             int nb, int Vb, int Ef) {
+  __shared__ cplx<double> sV[CQ_CH][CQ_NBMAX + 1];
+  __shared__ cplx<double> sW[CQ_CH][CQ_NBMAX + 1];
+  __shared__ cplx<double> sT[CQ_NBMAX][CQ_NBMAX + 1];
+  __shared__ cplx<double> sT2[CQ_NBMAX][CQ_NBMAX + 1];
+  const int panel = blockIdx.x;
+  const int xc = panel >> 1, k = panel & 1;
+  const int t = threadIdx.x;
+  if (flags[panel]) return;
+  const cplx<double>* T1 = Tg + (size_t)panel * nb * nb;
+  const cplx<double>* T2 = T1 + (size_t)gridDim.x * nb * nb;
+  for (int idx = t; idx < nb * nb; idx += CQ_THREADS) {
+    sT[idx / nb][idx % nb] = T1[idx];
+    sT2[idx / nb][idx % nb] = T2[idx];
+  }
+  const int rows = Vb * (Ef / 2);
+  for (int r0 = 0; r0 < rows; r0 += CQ_CH) {
+    const int nrow = min(CQ_CH, rows - r0);
+    __syncthreads();
+    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
+      const int r = idx / nb, i = idx % nb;
+      sV[r][i] = U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i];
+    }
+    __syncthreads();
+    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
+      const int r = idx / nb, j = idx % nb;
+      sW[r][j] = cq_row_times(sV[r], sT, j);
+    }
+    __syncthreads();
+    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
+      const int r = idx / nb, j = idx % nb;
+      U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + j] = cq_row_times(sW[r], sT2, j);
+    }
+  }
+}
+
 // errs[i] += sum_{k, j} |delta_ij - <U_j, U_i>_{block,k}|^2 (CheckOrthogonal)
 __global__ __launch_bounds__(256) void k_check_orthogonal(const cplx<double>* __restrict__ U,
                                                           int nb, int Vb, int Ef,
@@ -507,7 +742,8 @@ __global__ __launch_bounds__(256) void k_check_orthogonal(const cplx<double>* __
 
 static void free_agg(mgc_aggregation* a) {
   if (!a) return;
-  void* bufs[] = {a->basis64, a->basis32, a->blk_fine, a->fine_row, a->part};
+  void* bufs[] = {a->basis64, a->basis32, a->blk_fine, a->fine_row, a->part, a->cq_T,
+                  a->cq_flag};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;
@@ -746,10 +982,46 @@ int mgc_aggregation_orthogonalise(mgc_aggregation* a, int passes, void* stream) 
               "block too large for the in-register Gram-Schmidt (|block| * site elements / 2 "
               "<= 4096)");
     if (passes == 0) return;
-    k_orthogonalise<<<2 * a->Vc, MGS_THREADS, 0, as_stream(stream)>>>(a->basis64, a->nb, a->Vb,
-                                                                     a->Ef, passes);
+    hipStream_t s = as_stream(stream);
+    const int panels = 2 * a->Vc;
+    if (a->nb > CQ_NBMAX || g_ortho_sweep_only) {
+      k_orthogonalise<<<panels, MGS_THREADS, 0, s>>>(a->basis64, a->nb, a->Vb, a->Ef, passes,
+                                                     nullptr);
+      MGC_LAUNCH_CHECK();
+      a->basis32_valid = false;
+      return;
+    }
+    if (!a->cq_T) {
+      MGC_HIP(hipMalloc(&a->cq_T, (size_t)2 * panels * a->nb * a->nb * sizeof(cplx<double>)));
+      const hipError_t e = hipMalloc(&a->cq_flag, (size_t)panels * sizeof(int));
+      if (e != hipSuccess) {
+        (void)hipFree(a->cq_T);
+        a->cq_T = nullptr;
+        MGC_HIP(e);
+      }
+    }
+    fill_zero(a->cq_flag, (size_t)panels * sizeof(int), s);
+    auto gram = a->nb > 16 ? k_cholqr_gram<2> : k_cholqr_gram<1>;
+    for (int round = 0; round < 2; ++round) {
+      gram<<<panels, CQ_THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag, a->nb, a->Vb, a->Ef,
+                                         round);
+      MGC_LAUNCH_CHECK();
+    }
+    k_cholqr_apply<<<panels, CQ_THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag, a->nb, a->Vb,
+                                                 a->Ef);
+    MGC_LAUNCH_CHECK();
+    // panels whose Cholesky broke down: the column sweep (most workgroups exit at once)
+    k_orthogonalise<<<panels, MGS_THREADS, 0, s>>>(a->basis64, a->nb, a->Vb, a->Ef, passes,
+                                                   a->cq_flag);
     MGC_LAUNCH_CHECK();
     a->basis32_valid = false;
+  });
+}
+
+int mgc_aggregation_set_ortho_method(int method) {
+  return guarded([&] {
+    MGC_CHECK(method == 0 || method == 1, MGC_ERR_ARG, "method must be 0 or 1");
+    g_ortho_sweep_only = method;
   });
 }
 

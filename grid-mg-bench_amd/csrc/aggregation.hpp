@@ -22,6 +22,10 @@ struct mgc_aggregation {
   int* fine_row = nullptr;  // [Vf] panel row (xc*Vb + xb) of a fine site
   void* part = nullptr;     // partial sums of split projections (grown on demand)
   size_t part_bytes = 0;
+  // Cholesky-QR orthonormalisation: per (block, chirality) panel the nb x nb
+  // transform T and a breakdown flag (allocated on the first call)
+  mgc::cplx<double>* cq_T = nullptr;
+  int* cq_flag = nullptr;
 };
 
 namespace mgc {
@@ -32,6 +36,9 @@ void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int n
                   bool project, int kind, int point, hipStream_t s);
 // fine field [Vf][Ef] holding the chirality-l part of basis vector i (l = -1: all)
 void agg_extract_basis(mgc_aggregation* a, int i, int l, void* out, int prec, hipStream_t s);
+// FP64 fine fields [Vf][nr][Ef] of the doubled basis columns n0 .. n0+nr-1
+// (column n = chirality-(n / nb) part of basis vector n % nb)
+void agg_extract_doubled(mgc_aggregation* a, int n0, int nr, void* out, hipStream_t s);
 __host__ __device__ inline size_t agg_row(int nb, int Vb, int Ef, int xc, int xb, int e) {
   return (((size_t)xc * Vb + xb) * Ef + e) * nb;
 }

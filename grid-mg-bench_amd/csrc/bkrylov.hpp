@@ -39,7 +39,14 @@ struct BKrylovStats {
 };
 
 template <typename R>
-class BGmres {
+struct BSolver {
+  virtual ~BSolver() {}
+  virtual BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>* P, const cplx<R>* b, cplx<R>* x,
+                             bool x_zero, double tol, int maxit) = 0;
+};
+
+template <typename R>
+class BGmres : public BSolver<R> {
  public:
   BGmres(const BShape& sh, int m, bool flexible) : sh_(sh), m_(m), flexible_(flexible) {
     MGC_CHECK(m >= 1 && 2 * m + 3 <= 2 * BBlas::KMAX + 3, MGC_ERR_ARG,
@@ -61,7 +68,7 @@ class BGmres {
   BGmres& operator=(const BGmres&) = delete;
 
   BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>* P, const cplx<R>* b, cplx<R>* x,
-                     bool x_zero, double tol, int maxit) {
+                     bool x_zero, double tol, int maxit) override {
     typedef std::complex<double> C;
     hipStream_t s = bl.s;
     const size_t n = sh_.n;
@@ -245,6 +252,90 @@ class BGmres {
   cplx<R>* w_ = nullptr;
   cplx<R>* r_ = nullptr;
   cplx<double>* d_h_ = nullptr;  // [2*KMAX+3][nrhs] CGS2 coefficients
+};
+
+// Minimal residual iteration for nrhs systems in lockstep (the MR smoother
+// of include/Helpers.h:38-56 on the multi-rhs grid): per rhs
+// a_r = <A r, r>_r / |A r|^2_r, x_r += a_r r_r, r_r -= a_r (A r)_r; a rhs that
+// has met its tolerance keeps a_r = 0, so its iterate is that of a solve of
+// its own.
+template <typename R>
+class BMinRes : public BSolver<R> {
+ public:
+  explicit BMinRes(const BShape& sh) : sh_(sh) {
+    MGC_CHECK(sh.nrhs >= 1 && sh.nrhs <= BBlas::RMAX, MGC_ERR_ARG, "batch of 1..64 rhs");
+    MGC_HIP(hipMalloc(&r_, sh.n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&ar_, sh.n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&d_out_, (size_t)2 * sh.nrhs * sizeof(cplx<double>)));
+  }
+  ~BMinRes() {
+    void* b[] = {r_, ar_, d_out_};
+    for (void* p : b)
+      if (p) (void)hipFree(p);
+  }
+  BMinRes(const BMinRes&) = delete;
+  BMinRes& operator=(const BMinRes&) = delete;
+
+  BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>*, const cplx<R>* b, cplx<R>* x,
+                     bool x_zero, double tol, int maxit) override {
+    hipStream_t s = bl.s;
+    const size_t n = sh_.n;
+    const int nr = sh_.nrhs;
+    BKrylovStats st;
+    st.rel_res.assign(nr, 0.0);
+    st.converged.assign(nr, 0);
+    st.iters_rhs.assign(nr, 0);
+    std::vector<double> b2(nr), r2(nr);
+    bnorm2<R>(bl, b, sh_, b2.data());
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, r_, n, s);
+    } else {
+      A.apply(x, ar_, s);
+      ++st.applies;
+      vsub<R>(b, ar_, r_, n, s);
+    }
+    bnorm2<R>(bl, r_, sh_, r2.data());
+    std::vector<cplx<double>> c(nr);
+    while (true) {
+      bool all = true;
+      for (int r = 0; r < nr; ++r) {
+        st.rel_res[r] = b2[r] > 0.0 ? std::sqrt(r2[r] / b2[r]) : 0.0;
+        if (!st.converged[r] && (b2[r] == 0.0 || r2[r] <= tol * tol * b2[r])) {
+          st.converged[r] = 1;
+          st.iters_rhs[r] = st.iters;
+        }
+        all = all && st.converged[r];
+      }
+      if (all || st.iters >= maxit) break;
+      A.apply(r_, ar_, s);
+      ++st.applies;
+      // <A r, r>_r and |A r|^2_r
+      bdots_dev<R>(bl, ar_, n, 1, r_, sh_, false, d_out_);
+      bdots_dev<R>(bl, ar_, n, 1, ar_, sh_, false, d_out_ + nr);
+      bfetch(bl, reinterpret_cast<const double*>(d_out_), 4 * nr);
+      for (int r = 0; r < nr; ++r) {
+        const double* num = bl.h_res + 2 * r;
+        const double den = bl.h_res[2 * (nr + r)];
+        c[r] = (st.converged[r] || !(den > 0.0)) ? cplx<double>{0.0, 0.0}
+                                                  : cplx<double>{num[0] / den, num[1] / den};
+      }
+      bmultiaxpy<R>(bl, c.data(), r_, n, 1, x, sh_);
+      for (int r = 0; r < nr; ++r) c[r] = {-c[r].re, -c[r].im};
+      bmultiaxpy<R>(bl, c.data(), ar_, n, 1, r_, sh_);
+      bnorm2<R>(bl, r_, sh_, r2.data());
+      ++st.iters;
+    }
+    for (int r = 0; r < nr; ++r)
+      if (!st.converged[r]) st.iters_rhs[r] = st.iters;
+    return st;
+  }
+
+ private:
+  BShape sh_;
+  cplx<R>* r_ = nullptr;
+  cplx<R>* ar_ = nullptr;
+  cplx<double>* d_out_ = nullptr;
 };
 
 }  // namespace mgc

@@ -120,7 +120,10 @@ static void launch_stencil_t(const void* links, size_t site_stride, size_t point
   MGC_LAUNCH_CHECK();
 }
 
-#define MGC_FOR_EACH_N(X) X(4) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
+#define MGC_FOR_EACH_N(X)                                                                  \
+  X(2) X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) \
+  X(34) X(36) X(38) X(40) X(42) X(44) X(46) X(48) X(50) X(52) X(54) X(56) X(58) X(60) X(62)   \
+  X(64)
 
 template <typename R>
 static void launch_stencil_r(int N, const void* links, size_t ss, size_t ps, const void* in,
@@ -350,7 +353,8 @@ void invert_self_rowmajor(cplx<double>* rm, int V, int N, hipStream_t s) {
 static void free_op(mgc_coarse_op* op) {
   if (!op) return;
   void* bufs[] = {op->links,    op->self_dag, op->self_inv,   op->self_invdag, op->nbr_full,
-                  op->nbr_half, op->cb_sites, op->mrhs_links, op->mrhs_scales};
+                  op->nbr_half, op->cb_sites, op->mrhs_links, op->mrhs_scales,
+                  op->q_in,     op->q_scale};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete op;

@@ -33,6 +33,11 @@ struct mgc_coarse_op {
   void* mrhs_links = nullptr;
   float* mrhs_scales = nullptr;  // [site][9] (half precisions)
   int mrhs_prec = -1;
+  // FP32 -> F16/BF16 quantised input of the multi-rhs apply (grown on demand,
+  // kept for later calls): q_in [site][rhs][N] half pairs, q_scale [site][rhs]
+  void* q_in = nullptr;
+  float* q_scale = nullptr;
+  size_t q_rows = 0;  // rows (site x rhs) the workspace holds
 
   size_t elem_bytes() const { return 2 * mgc::prec_real_bytes(prec); }
   size_t vec_bytes() const { return (size_t)V * N * elem_bytes(); }

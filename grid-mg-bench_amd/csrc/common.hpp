@@ -56,6 +56,26 @@ inline int guarded(F&& f) {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// device allocation freed when it goes out of scope (also on an exception)
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  void alloc(size_t count) {
+    reset();
+    MGC_HIP(hipMalloc((void**)&p, count * sizeof(T)));
+  }
+  void reset() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+  }
+  ~DevBuf() { reset(); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  T* get() const { return p; }
+};
+
 // --------------------------------------------------------------------------
 // complex helpers (interleaved re,im)
 // --------------------------------------------------------------------------

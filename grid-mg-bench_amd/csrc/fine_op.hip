@@ -588,10 +588,21 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
     k<<<g, threads, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in, (cplx<double>*)out,
                             op->nbr, op->V, op->sv, nrhs, mask, dag, wt);
   } else if (prec == MGC_F32) {
-    if (!op->gsoa32) {
-      MGC_HIP(hipMalloc(&op->gsoa32, (size_t)36 * op->sv * sizeof(cplx<float>)));
-      MGC_HIP(hipMalloc(&op->dsoa32, (size_t)36 * op->sv * sizeof(cplx<float>)));
-      build_soa<float>(op, op->gsoa32, op->dsoa32, s);
+    if (!op->gsoa32 || !op->dsoa32) {
+      // both copies or neither: a failed second allocation leaves the handle
+      // as it was, so a later FP32 apply retries instead of using a null copy
+      const size_t bytes = (size_t)36 * op->sv * sizeof(cplx<float>);
+      cplx<float>* g32 = nullptr;
+      cplx<float>* d32 = nullptr;
+      MGC_HIP(hipMalloc(&g32, bytes));
+      const hipError_t e = hipMalloc(&d32, bytes);
+      if (e != hipSuccess) {
+        (void)hipFree(g32);
+        MGC_HIP(e);
+      }
+      build_soa<float>(op, g32, d32, s);
+      op->gsoa32 = g32;
+      op->dsoa32 = d32;
     }
     auto k = lds ? k_wilson_lds<float> : k_wilson<float>;
     k<<<g, threads, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in, (cplx<float>*)out,
@@ -607,6 +618,31 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
 using namespace mgc;
 
 extern "C" {
+
+// the built-in operator behind the callback interface: points 0..7 = Mdir,
+// 8 = Mdiag, 9 = M
+static int builtin_linop_apply(void* ctx, int point, int prec, const void* in, void* out,
+                               int nrhs, void* stream) {
+  mgc_fine_op* op = static_cast<mgc_fine_op*>(ctx);
+  if (point >= 0 && point < 8)
+    return mgc_fine_op_Mdir(op, in, out, point_dir(point), point_disp(point), prec, nrhs, stream);
+  if (point == 8) return mgc_fine_op_Mdiag(op, in, out, prec, nrhs, stream);
+  if (point == 9) return mgc_fine_op_M(op, in, out, prec, nrhs, stream);
+  set_last_error("fine operator callback: point must be in 0..9");
+  return MGC_ERR_ARG;
+}
+
+int mgc_fine_op_linop(mgc_fine_op* op, mgc_fine_linop* out) {
+  return guarded([&] {
+    MGC_CHECK(op && out, MGC_ERR_ARG, "null argument");
+    out->apply = builtin_linop_apply;
+    out->ctx = op;
+    for (int d = 0; d < 4; ++d) out->dims[d] = op->lat.L[d];
+    out->precisions = (1 << MGC_F64) | (1 << MGC_F32);
+    out->max_nrhs = 256;
+    out->gamma5_hermitian = 1;
+  });
+}
 
 int mgc_fine_op_set_variant(int variant) {
   return guarded([&] {

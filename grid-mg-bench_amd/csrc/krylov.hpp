@@ -46,8 +46,17 @@ struct KrylovStats {
   bool converged = false;
 };
 
+// A solver of A x = b (x the initial guess unless x_zero), stopping at
+// ||b - A x|| <= tol ||b|| or maxit iterations.
 template <typename R>
-class Gmres {
+struct Solver {
+  virtual ~Solver() {}
+  virtual KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>* P, const cplx<R>* b, cplx<R>* x,
+                            bool x_zero, double tol, int maxit) = 0;
+};
+
+template <typename R>
+class Gmres : public Solver<R> {
  public:
   Gmres(size_t n, int m, bool flexible) : n_(n), m_(m), flexible_(flexible) {
     MGC_CHECK(m >= 1 && m + 1 <= Blas::KMAX, MGC_ERR_ARG, "restart length out of range");
@@ -68,7 +77,7 @@ class Gmres {
   // Solve A x = b to ||b - A x|| <= tol ||b|| or maxit iterations.  x is the
   // initial guess unless x_zero.
   KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>* P, const cplx<R>* b, cplx<R>* x, bool x_zero,
-                    double tol, int maxit) {
+                    double tol, int maxit) override {
     typedef std::complex<double> C;
     hipStream_t s = bl.s;
     const size_t n = n_;
@@ -213,6 +222,161 @@ class Gmres {
   cplx<R>* w_ = nullptr;
   cplx<R>* r_ = nullptr;
   cplx<double>* d_h_ = nullptr;  // device-resident CGS2 coefficients
+};
+
+// Unpreconditioned BiCGSTAB (van der Vorst), the setup solver of
+// include/MultigridParams.h:106-107 and a selectable solver type of
+// include/Helpers.h:38-56.  One host read per scalar.  With b = 0 (the
+// null-vector setup, include/BasisVectors.h:99-105: solve A psi = 0 from
+// psi = v) there is no relative tolerance to meet, so exactly maxit
+// iterations relax the initial guess.
+template <typename R>
+class BiCgStab : public Solver<R> {
+ public:
+  explicit BiCgStab(size_t n) : n_(n) {
+    for (int i = 0; i < 6; ++i) MGC_HIP(hipMalloc(&w_[i], n * sizeof(cplx<R>)));
+  }
+  ~BiCgStab() {
+    for (int i = 0; i < 6; ++i)
+      if (w_[i]) (void)hipFree(w_[i]);
+  }
+  BiCgStab(const BiCgStab&) = delete;
+  BiCgStab& operator=(const BiCgStab&) = delete;
+
+  KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>*, const cplx<R>* b, cplx<R>* x, bool x_zero,
+                    double tol, int maxit) override {
+    typedef std::complex<double> C;
+    hipStream_t s = bl.s;
+    const size_t n = n_;
+    cplx<R>*r = w_[0], *rh = w_[1], *p = w_[2], *v = w_[3], *sv = w_[4], *t = w_[5];
+    KrylovStats st;
+    const double b2 = vnorm2<R>(bl, b, n);
+    const bool relax = !(b2 > 0.0);  // b = 0: fixed iteration count
+    if (relax && x_zero) {
+      vzero<R>(x, n, s);
+      st.converged = true;
+      return st;
+    }
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, r, n, s);
+    } else {
+      A.apply(x, t, s);
+      ++st.applies;
+      vsub<R>(b, t, r, n, s);
+    }
+    vcopy<R>(r, rh, n, s);
+    vzero<R>(p, n, s);
+    vzero<R>(v, n, s);
+    C rho(1.0), alpha(1.0), omega(1.0);
+    const double stop = tol * tol * b2;
+    double r2 = vnorm2<R>(bl, r, n);
+    auto c = [](C z) { return cplx<double>{z.real(), z.imag()}; };
+    auto d = [](cplx<double> z) { return C(z.re, z.im); };
+    while (true) {
+      st.rel_res = relax ? std::sqrt(r2) : std::sqrt(r2 / b2);
+      if (!relax && r2 <= stop) {
+        st.converged = true;
+        break;
+      }
+      if (st.iters >= maxit) break;
+      const C rho1 = d(vdot<R>(bl, rh, r, n));
+      if (std::abs(rho1) == 0.0 || std::abs(omega) == 0.0) break;  // breakdown
+      const C beta = (rho1 / rho) * (alpha / omega);
+      rho = rho1;
+      // p = r + beta (p - omega v)
+      vaxpy<R>(c(-omega), v, p, n, s);
+      vxpay<R>(r, c(beta), p, n, s);
+      A.apply(p, v, s);
+      const C rhv = d(vdot<R>(bl, rh, v, n));
+      if (std::abs(rhv) == 0.0) break;
+      alpha = rho / rhv;
+      // s = r - alpha v
+      vcopy<R>(r, sv, n, s);
+      vaxpy<R>(c(-alpha), v, sv, n, s);
+      A.apply(sv, t, s);
+      st.applies += 2;
+      const C ts = d(vdot<R>(bl, t, sv, n));
+      const double tt = vnorm2<R>(bl, t, n);
+      omega = tt > 0.0 ? ts / tt : C(0.0);
+      // x += alpha p + omega s ; r = s - omega t
+      vaxpy<R>(c(alpha), p, x, n, s);
+      vaxpy<R>(c(omega), sv, x, n, s);
+      vcopy<R>(sv, r, n, s);
+      vaxpy<R>(c(-omega), t, r, n, s);
+      r2 = vnorm2<R>(bl, r, n);
+      ++st.iters;
+    }
+    return st;
+  }
+
+ private:
+  size_t n_;
+  cplx<R>* w_[6] = {};
+};
+
+// Minimal residual iteration (over-relaxation 1), the MR solver type of
+// include/Helpers.h:38-56: x += a r, r -= a A r with a = <A r, r> / |A r|^2.
+template <typename R>
+class MinRes : public Solver<R> {
+ public:
+  explicit MinRes(size_t n) : n_(n) {
+    MGC_HIP(hipMalloc(&r_, n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&ar_, n * sizeof(cplx<R>)));
+  }
+  ~MinRes() {
+    if (r_) (void)hipFree(r_);
+    if (ar_) (void)hipFree(ar_);
+  }
+  MinRes(const MinRes&) = delete;
+  MinRes& operator=(const MinRes&) = delete;
+
+  KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>*, const cplx<R>* b, cplx<R>* x, bool x_zero,
+                    double tol, int maxit) override {
+    hipStream_t s = bl.s;
+    const size_t n = n_;
+    KrylovStats st;
+    const double b2 = vnorm2<R>(bl, b, n);
+    if (!(b2 > 0.0)) {
+      vzero<R>(x, n, s);
+      st.converged = true;
+      return st;
+    }
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, r_, n, s);
+    } else {
+      A.apply(x, ar_, s);
+      ++st.applies;
+      vsub<R>(b, ar_, r_, n, s);
+    }
+    double r2 = vnorm2<R>(bl, r_, n);
+    const double stop = tol * tol * b2;
+    while (true) {
+      st.rel_res = std::sqrt(r2 / b2);
+      if (r2 <= stop) {
+        st.converged = true;
+        break;
+      }
+      if (st.iters >= maxit) break;
+      A.apply(r_, ar_, s);
+      ++st.applies;
+      const cplx<double> num = vdot<R>(bl, ar_, r_, n);
+      const double den = vnorm2<R>(bl, ar_, n);
+      if (!(den > 0.0)) break;
+      const cplx<double> a = {num.re / den, num.im / den};
+      vaxpy<R>(a, r_, x, n, s);
+      vaxpy<R>({-a.re, -a.im}, ar_, r_, n, s);
+      r2 = vnorm2<R>(bl, r_, n);
+      ++st.iters;
+    }
+    return st;
+  }
+
+ private:
+  size_t n_;
+  cplx<R>* r_ = nullptr;
+  cplx<R>* ar_ = nullptr;
 };
 
 }  // namespace mgc

@@ -19,7 +19,9 @@
 // preconditioner (smoothers, coarse operators, Krylov wrappers) runs in the
 // hierarchy precision (FP64 "plain form" or FP32).  Galerkin operators are
 // always built in FP64 and rounded for the FP32 hierarchy.
+#include <algorithm>
 #include <chrono>
+#include <string>
 #include <memory>
 #include <type_traits>
 #include <vector>
@@ -38,12 +40,53 @@ static void ok(int st) {
   if (st != MGC_OK) throw Error(st, mgc_last_error());
 }
 
+// The fine operator of a hierarchy: a caller's callback operator, or the
+// library's Wilson-clover operator (whose fused kernel then builds the first
+// coarse operator).  M applies the whole operator in the precision asked
+// for, converting through FP64 staging buffers when the callback does not
+// take that precision, and in calls of at most max_nrhs right-hand sides.
+struct FineSrc {
+  mgc_fine_linop lop{};
+  mgc_fine_op* fop = nullptr;
+  Lat4 lat{};
+  int V = 0;
+  DevBuf<cplx<double>> st_in, st_out;  // staging for precisions the callback lacks
+  size_t st_cap = 0;
+
+  void M(const void* in, void* out, int prec, int nrhs, hipStream_t s) {
+    // vectors are [site][rhs][12]: a batch cannot be split along rhs in place
+    MGC_CHECK(nrhs <= lop.max_nrhs, MGC_ERR_ARG,
+              "the fine operator takes fewer right-hand sides per call than the batch");
+    if (lop.precisions & (1 << prec)) {
+      call(9, prec, in, out, nrhs, s);
+      return;
+    }
+    MGC_CHECK(prec == MGC_F32 && (lop.precisions & (1 << MGC_F64)), MGC_ERR_PRECISION,
+              "the fine operator takes neither this precision nor F64");
+    const size_t n = (size_t)V * 12 * nrhs;
+    if (n > st_cap) {
+      st_in.alloc(n);
+      st_out.alloc(n);
+      st_cap = n;
+    }
+    vconvert<float, double>((const cplx<float>*)in, st_in.p, n, 1.0, s);
+    call(9, MGC_F64, st_in.p, st_out.p, nrhs, s);
+    vconvert<double, float>(st_out.p, (cplx<float>*)out, n, 1.0, s);
+  }
+  void call(int point, int prec, const void* in, void* out, int nrhs, hipStream_t s) {
+    const int st = lop.apply(lop.ctx, point, prec, in, out, nrhs, (void*)s);
+    if (st != MGC_OK)
+      throw Error(st < 0 ? st : MGC_ERR_ARG,
+                  "the fine operator callback failed (status " + std::to_string(st) + ")");
+  }
+};
+
 template <typename R>
 struct FineLin : LinOp<R> {
-  mgc_fine_op* op;
-  explicit FineLin(mgc_fine_op* o) : op(o) { this->n = (size_t)o->V * 12; }
+  FineSrc* src;
+  explicit FineLin(FineSrc* f) : src(f) { this->n = (size_t)f->V * 12; }
   void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) override {
-    ok(mgc_fine_op_M(op, in, out, prec_of<R>(), 1, (void*)s));
+    src->M(in, out, prec_of<R>(), 1, s);
   }
 };
 
@@ -56,6 +99,18 @@ struct CoarseLin : LinOp<R> {
   }
 };
 
+// the solver types of include/Helpers.h:38-56 (unpreconditioned but FGMRES)
+template <typename R>
+std::unique_ptr<Solver<R>> make_solver(int type, size_t n, int restart) {
+  switch (type) {
+    case MGC_SOLVER_GMRES: return std::make_unique<Gmres<R>>(n, restart, false);
+    case MGC_SOLVER_FGMRES: return std::make_unique<Gmres<R>>(n, restart, true);
+    case MGC_SOLVER_BICGSTAB: return std::make_unique<BiCgStab<R>>(n);
+    case MGC_SOLVER_MR: return std::make_unique<MinRes<R>>(n);
+  }
+  throw Error(MGC_ERR_ARG, "unknown solver type " + std::to_string(type));
+}
+
 // one level of the hierarchy in precision R
 template <typename R>
 struct Level {
@@ -65,8 +120,9 @@ struct Level {
   mgc_aggregation* agg = nullptr;      // to the next level (null on the coarsest)
   mgc_coarse_op* cop64 = nullptr;      // next level's operator, FP64 (Galerkin build)
   mgc_coarse_op* copR = nullptr;       // next level's operator in R (== cop64 for FP64)
-  std::unique_ptr<Gmres<R>> smoother;  // GMRES smoother / coarsest solver at this level
-  std::unique_ptr<Gmres<R>> wrapper;   // K-cycle FGMRES on the next level
+  std::unique_ptr<Solver<R>> smoother;  // smoother / coarsest solver at this level
+  std::unique_ptr<Gmres<R>> wrapper;    // K-cycle FGMRES on the next level
+  DevBuf<cplx<R>> tv;                   // test vectors [nb][n], kept for refinement
   cplx<R>* rc = nullptr;               // next-level residual / correction
   cplx<R>* ec = nullptr;
   cplx<R>* t1 = nullptr;               // level temporaries (V-cycle)
@@ -85,14 +141,14 @@ struct Level {
 // batched operators: nrhs right-hand sides per apply, vectors [site][rhs][dof]
 template <typename R>
 struct BFineLin : BLinOp<R> {
-  mgc_fine_op* op;
-  BFineLin(mgc_fine_op* o, int nrhs) : op(o) {
-    this->sh.n = (size_t)o->V * 12 * nrhs;
+  FineSrc* src;
+  BFineLin(FineSrc* f, int nrhs) : src(f) {
+    this->sh.n = (size_t)f->V * 12 * nrhs;
     this->sh.nrhs = nrhs;
     this->sh.dof = 12;
   }
   void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) override {
-    ok(mgc_fine_op_M(op, in, out, prec_of<R>(), this->sh.nrhs, (void*)s));
+    src->M(in, out, prec_of<R>(), this->sh.nrhs, s);
   }
 };
 
@@ -128,7 +184,7 @@ struct MGBase {
 
 template <typename R>
 struct MG : MGBase {
-  mgc_fine_op* fop;
+  FineSrc* fsrc;
   mgc_mg_params p;
   std::vector<std::unique_ptr<Level<R>>> L;
   Blas bl;
@@ -145,7 +201,7 @@ struct MG : MGBase {
   };
   std::vector<std::unique_ptr<LevelPrec>> precs;
 
-  MG(mgc_fine_op* f, const mgc_mg_params& pp, hipStream_t s) : fop(f), p(pp), bl(s) {}
+  MG(FineSrc* f, const mgc_mg_params& pp, hipStream_t s) : fsrc(f), p(pp), bl(s) {}
   ~MG() override {
     free_batch();
     L.clear();
@@ -169,22 +225,112 @@ struct MG : MGBase {
     lv.smoother->solve(bl, *lv.op, nullptr, in, out, false, p.smoother_tol, p.smoother_iters);
   }
 
+  // Orthonormalise the set of nb vectors [nb][n] over the whole lattice
+  // (Gram-Schmidt with a second projection, as the reference's
+  // orthogonalise(V) of include/BasisVectors.h:152-156 does for the set)
+  void orthonormalise_set(cplx<R>* V, int nb, size_t n, hipStream_t s) {
+    std::vector<cplx<double>> h(nb);
+    for (int i = 0; i < nb; ++i) {
+      cplx<R>* v = V + (size_t)i * n;
+      for (int rep = 0; rep < 2 && i > 0; ++rep) {
+        vmultidot<R>(bl, V, n, i, v, n, h.data());
+        for (int j = 0; j < i; ++j) h[j] = {-h[j].re, -h[j].im};
+        vmultiaxpy<R>(bl, h.data(), V, n, i, v, n);
+      }
+      const double nv = std::sqrt(vnorm2<R>(bl, v, n));
+      MGC_CHECK(nv > 0.0 && std::isfinite(nv), MGC_ERR_STATE,
+                "test vectors linearly dependent or not finite in the setup");
+      vscale<R>(1.0 / nv, v, n, s);
+    }
+  }
+
+  // Test vectors of level l (MGBasisVectors::Generate, include/BasisVectors.h:74-149):
+  // Gaussian, then setup_iters rounds of [pre-orthonormalise,] v_k <- A^-1 v_k
+  // approximately (test vectors: from 0 with rhs v_k; null vectors: A psi = 0
+  // from psi = v_k) [, post-orthonormalise].
+  void generate(int l, cplx<R>* tv, hipStream_t s) {
+    Level<R>& lv = *L[l];
+    const size_t n = lv.n;
+    const int nb = p.nb[l];
+    ok(mgc_random_gaussian(tv, (size_t)nb * n, prec_of<R>(), p.seed, 1000 + l, (void*)s));
+    auto solver = make_solver<R>(p.setup_solver, n, std::max(1, std::min(p.setup_restart, 63)));
+    const bool nullv = p.setup_basis_type == MGC_BASIS_NULL_VECTORS;
+    DevBuf<cplx<R>> src(n), e(n);
+    for (int it = 0; it < p.setup_iters; ++it) {
+      if (p.setup_pre_orthonormalise) orthonormalise_set(tv, nb, n, s);
+      for (int k = 0; k < nb; ++k) {
+        cplx<R>* v = tv + (size_t)k * n;
+        const double nv = std::sqrt(vnorm2<R>(bl, v, n));
+        MGC_CHECK(nv > 0.0 && std::isfinite(nv), MGC_ERR_STATE,
+                  "test vector with zero or non-finite norm in the setup");
+        vscale<R>(1.0 / nv, v, n, s);
+        if (!nullv) {
+          vcopy<R>(v, src.p, n, s);
+          solver->solve(bl, *lv.op, nullptr, src.p, v, true, p.setup_tol, p.setup_solve_iters);
+        } else if (p.setup_solver == MGC_SOLVER_BICGSTAB) {
+          // rhs 0: BiCGSTAB relaxes psi = v for exactly setup_solve_iters steps
+          vzero<R>(src.p, n, s);
+          solver->solve(bl, *lv.op, nullptr, src.p, v, false, p.setup_tol, p.setup_solve_iters);
+        } else {
+          // A psi = 0 from psi = v as psi = v + e with A e = -A v, to setup_tol of |A v|
+          lv.op->apply(v, src.p, s);
+          vscale<R>(-1.0, src.p, n, s);
+          solver->solve(bl, *lv.op, nullptr, src.p, e.p, true, p.setup_tol, p.setup_solve_iters);
+          vaxpy<R>({1.0, 0.0}, e.p, v, n, s);
+        }
+      }
+      if (p.setup_post_orthonormalise) orthonormalise_set(tv, nb, n, s);
+    }
+  }
+
+  // Aggregation::Create (copy + block orthonormalisation) from the level's
+  // test vectors
+  void install_basis(int l, const cplx<R>* tv, hipStream_t s) {
+    Level<R>& lv = *L[l];
+    const size_t total = (size_t)p.nb[l] * lv.n;
+    DevBuf<cplx<double>> tv64(total);
+    vconvert<R, double>(tv, tv64.p, total, 1.0, s);
+    ok(mgc_aggregation_set_basis(lv.agg, tv64.p, (void*)s));
+    ok(mgc_aggregation_orthogonalise(lv.agg, 2, (void*)s));
+    MGC_HIP(hipStreamSynchronize(s));  // tv64 is freed on return
+  }
+
+  // CoarsenOperator of level l onto l+1 (FP64), rounded into the FP32 copy
+  void coarsen_level(int l, hipStream_t s) {
+    Level<R>& lv = *L[l];
+    if (l == 0 && fsrc->fop)
+      ok(mgc_coarsen_operator(lv.cop64, fsrc->fop, lv.agg, p.coarsen_mode, (void*)s));
+    else if (l == 0)
+      ok(mgc_coarsen_operator_linop(lv.cop64, &fsrc->lop, lv.agg, p.coarsen_mode, (void*)s));
+    else
+      ok(mgc_coarsen_coarse_operator(lv.cop64, L[l - 1]->cop64, lv.agg, p.coarsen_mode,
+                                     (void*)s));
+    if (lv.copR != lv.cop64) {
+      const size_t cnt = (size_t)9 * lv.cop64->V * lv.cop64->N * lv.cop64->N;
+      DevBuf<cplx<double>> tmp(cnt);
+      ok(mgc_coarse_op_get_links(lv.cop64, MGC_LAYOUT_FLAT, tmp.p, (void*)s));
+      ok(mgc_coarse_op_set_links(lv.copR, MGC_LAYOUT_FLAT, tmp.p, 0, (void*)s));
+      MGC_HIP(hipStreamSynchronize(s));
+    }
+  }
+
   void setup(hipStream_t s) override {
     MGC_CHECK(p.nlevels >= 2 && p.nlevels <= 4, MGC_ERR_ARG, "nlevels must be 2..4");
+    ready = false;
     L.clear();
     precs.clear();
     // level 0: the fine operator
     {
       auto lv = std::make_unique<Level<R>>();
       lv->index = 0;
-      lv->op = std::make_unique<FineLin<R>>(fop);
+      lv->op = std::make_unique<FineLin<R>>(fsrc);
       lv->n = lv->op->n;
       L.push_back(std::move(lv));
     }
     int dims[4];
-    for (int d = 0; d < 4; ++d) dims[d] = fop->lat.L[d];
+    for (int d = 0; d < 4; ++d) dims[d] = fsrc->lat.L[d];
     int Ef = 12;
-    mgc_coarse_op* prev64 = nullptr;
+    const bool keep_tv = p.setup_refine_iters > 0;
     for (int l = 0; l + 1 < p.nlevels; ++l) {
       Level<R>& lv = *L[l];
       const int nb = p.nb[l];
@@ -195,58 +341,15 @@ struct MG : MGBase {
         cd[d] = dims[d] / p.block[l][d];
       }
       ok(mgc_aggregation_create_ex(&lv.agg, dims, p.block[l], nb, Ef));
-      // test vectors: Gaussian, then setup_iters rounds of psi = A^{-1} v (approx.)
-      const size_t n = lv.n;
-      cplx<R>* tv = nullptr;
-      cplx<double>* tv64 = nullptr;
-      MGC_HIP(hipMalloc(&tv, (size_t)nb * n * sizeof(cplx<R>)));
-      MGC_HIP(hipMalloc(&tv64, (size_t)nb * n * sizeof(cplx<double>)));
-      try {
-        ok(mgc_random_gaussian(tv, (size_t)nb * n, prec_of<R>(), p.seed, 1000 + l, (void*)s));
-        Gmres<R> setup_solver(n, std::max(1, std::min(p.setup_solve_iters, 32)), false);
-        cplx<R>* src = nullptr;
-        MGC_HIP(hipMalloc(&src, n * sizeof(cplx<R>)));
-        for (int it = 0; it < p.setup_iters; ++it)
-          for (int k = 0; k < nb; ++k) {
-            cplx<R>* v = tv + (size_t)k * n;
-            const double nv = std::sqrt(vnorm2<R>(bl, v, n));
-            vconvert<R, R>(v, src, n, 1.0 / nv, s);
-            setup_solver.solve(bl, *lv.op, nullptr, src, v, true, p.setup_tol, p.setup_solve_iters);
-          }
-        (void)hipFree(src);
-        vconvert<R, double>(tv, tv64, (size_t)nb * n, 1.0, s);
-        ok(mgc_aggregation_set_basis(lv.agg, tv64, (void*)s));
-        ok(mgc_aggregation_orthogonalise(lv.agg, 2, (void*)s));
-      } catch (...) {
-        (void)hipStreamSynchronize(s);
-        (void)hipFree(tv);
-        (void)hipFree(tv64);
-        throw;
-      }
-      MGC_HIP(hipStreamSynchronize(s));
-      (void)hipFree(tv);
-      (void)hipFree(tv64);
-      // Galerkin operator of the next level, FP64
+      lv.tv.alloc((size_t)nb * lv.n);
+      generate(l, lv.tv.p, s);
+      install_basis(l, lv.tv.p, s);
+      if (!keep_tv) lv.tv.reset();
+      // Galerkin operator of the next level (FP64, rounded for an FP32 hierarchy)
       ok(mgc_coarse_op_create(&lv.cop64, cd, 2 * nb, MGC_F64, 0));
-      if (l == 0) ok(mgc_coarsen_operator(lv.cop64, fop, lv.agg, p.coarsen_mode, (void*)s));
-      else ok(mgc_coarsen_coarse_operator(lv.cop64, prev64, lv.agg, p.coarsen_mode, (void*)s));
-      if (prec_of<R>() == MGC_F64) {
-        lv.copR = lv.cop64;
-      } else {
-        const int N = 2 * nb;
-        const int Vc = cd[0] * cd[1] * cd[2] * cd[3];
-        ok(mgc_coarse_op_create(&lv.copR, cd, N, MGC_F32, 0));
-        cplx<double>* tmp = nullptr;
-        MGC_HIP(hipMalloc(&tmp, (size_t)9 * Vc * N * N * sizeof(cplx<double>)));
-        const int st1 = mgc_coarse_op_get_links(lv.cop64, MGC_LAYOUT_FLAT, tmp, (void*)s);
-        const int st2 = st1 == MGC_OK
-                            ? mgc_coarse_op_set_links(lv.copR, MGC_LAYOUT_FLAT, tmp, 0, (void*)s)
-                            : st1;
-        (void)hipStreamSynchronize(s);
-        (void)hipFree(tmp);
-        ok(st2);
-      }
-      prev64 = lv.cop64;
+      if (prec_of<R>() == MGC_F64) lv.copR = lv.cop64;
+      else ok(mgc_coarse_op_create(&lv.copR, cd, 2 * nb, MGC_F32, 0));
+      coarsen_level(l, s);
       auto nx = std::make_unique<Level<R>>();
       nx->index = l + 1;
       nx->op = std::make_unique<CoarseLin<R>>(lv.copR);
@@ -257,15 +360,15 @@ struct MG : MGBase {
       Ef = 2 * nb;
       L.push_back(std::move(nx));
     }
-    // solvers
+    // solvers: smoother (GMRES or MR), K-cycle FGMRES wrapper, coarsest solver
     for (int l = 0; l < p.nlevels; ++l) {
       Level<R>& lv = *L[l];
       if (l + 1 < p.nlevels) {
-        lv.smoother = std::make_unique<Gmres<R>>(lv.n, p.smoother_restart, false);
+        lv.smoother = make_solver<R>(p.smoother_solver, lv.n, p.smoother_restart);
         if (l + 2 < p.nlevels)
           lv.wrapper = std::make_unique<Gmres<R>>(L[l + 1]->n, p.wrapper_restart, true);
       } else {
-        lv.smoother = std::make_unique<Gmres<R>>(lv.n, p.coarsest_restart, false);
+        lv.smoother = make_solver<R>(p.coarsest_solver, lv.n, p.coarsest_restart);
       }
       MGC_HIP(hipMalloc(&lv.t1, lv.n * sizeof(cplx<R>)));
       MGC_HIP(hipMalloc(&lv.t2, lv.n * sizeof(cplx<R>)));
@@ -276,6 +379,38 @@ struct MG : MGBase {
     if (!out_r) MGC_HIP(hipMalloc(&out_r, L[0]->n * sizeof(cplx<R>)));
     MGC_HIP(hipStreamSynchronize(s));
     ready = true;
+    if (keep_tv) refine(s);
+  }
+
+  // Refinement setup (include/MultigridPreconditioner.h:213-251, the
+  // Wuppertal form): on each level l, setup_refine_iters rounds of
+  // v_k <- K-cycle_l(v_k) with the current hierarchy; with recursive updates
+  // the coarse correction of that K-cycle becomes test vector k of level l+1.
+  // After each round the operators of level l and every coarser level are
+  // rebuilt from the current test vectors.
+  void refine(hipStream_t s) {
+    for (int l = 0; l + 1 < p.nlevels; ++l) {
+      Level<R>& lv = *L[l];
+      const size_t n = lv.n;
+      const int nb = p.nb[l];
+      DevBuf<cplx<R>> out(n);
+      for (int it = 0; it < p.setup_refine_iters; ++it) {
+        if (p.setup_pre_orthonormalise) orthonormalise_set(lv.tv.p, nb, n, s);
+        for (int k = 0; k < nb; ++k) {
+          cplx<R>* v = lv.tv.p + (size_t)k * n;
+          kcycle(l, v, out.p, s);
+          vcopy<R>(out.p, v, n, s);
+          if (p.setup_recursive_updates && l + 2 < p.nlevels && k < p.nb[l + 1])
+            vcopy<R>(lv.ec, L[l + 1]->tv.p + (size_t)k * L[l + 1]->n, L[l + 1]->n, s);
+        }
+        if (p.setup_post_orthonormalise) orthonormalise_set(lv.tv.p, nb, n, s);
+        for (int m = l; m + 1 < p.nlevels; ++m) {
+          if (m == l || p.setup_recursive_updates) install_basis(m, L[m]->tv.p, s);
+          coarsen_level(m, s);
+        }
+      }
+    }
+    MGC_HIP(hipStreamSynchronize(s));
   }
 
   void precondition64(const cplx<double>* in, cplx<double>* out, hipStream_t s) override {
@@ -340,7 +475,8 @@ struct MG : MGBase {
   // ---- batched K-cycle (nrhs right-hand sides in lockstep) ----------------
   struct BLevelState {
     std::unique_ptr<BLinOp<R>> op;
-    std::unique_ptr<BGmres<R>> smoother, wrapper;
+    std::unique_ptr<BSolver<R>> smoother;
+    std::unique_ptr<BGmres<R>> wrapper;
     cplx<R>* rc = nullptr;  // next-level residual / correction, batched
     cplx<R>* ec = nullptr;
     ~BLevelState() {
@@ -384,19 +520,25 @@ struct MG : MGBase {
     for (int l = 0; l < p.nlevels; ++l) {
       auto st = std::make_unique<BLevelState>();
       if (l == 0) {
-        st->op = std::make_unique<BFineLin<R>>(fop, nrhs);
+        st->op = std::make_unique<BFineLin<R>>(fsrc, nrhs);
       } else {
         if constexpr (std::is_same<R, float>::value)
           st->op = std::make_unique<BCoarseLin>(L[l - 1]->copR, nrhs, coarse_prec);
       }
       const BShape& sh = st->op->sh;
+      auto bsolver = [&](int type, int restart) -> std::unique_ptr<BSolver<R>> {
+        if (type == MGC_SOLVER_MR) return std::make_unique<BMinRes<R>>(sh);
+        MGC_CHECK(type == MGC_SOLVER_GMRES, MGC_ERR_ARG,
+                  "the batched solve takes GMRES or MR smoothers and coarsest solvers");
+        return std::make_unique<BGmres<R>>(sh, restart, false);
+      };
       if (l + 1 < p.nlevels) {
-        st->smoother = std::make_unique<BGmres<R>>(sh, p.smoother_restart, false);
+        st->smoother = bsolver(p.smoother_solver, p.smoother_restart);
         const size_t nc = L[l + 1]->n * (size_t)nrhs;
         MGC_HIP(hipMalloc(&st->rc, nc * sizeof(cplx<R>)));
         MGC_HIP(hipMalloc(&st->ec, nc * sizeof(cplx<R>)));
       } else {
-        st->smoother = std::make_unique<BGmres<R>>(sh, p.coarsest_restart, false);
+        st->smoother = bsolver(p.coarsest_solver, p.coarsest_restart);
       }
       BL.push_back(std::move(st));
     }
@@ -468,7 +610,7 @@ struct BPrecAdapter : BPrec<double> {
 }  // namespace mgc
 
 struct mgc_mg {
-  mgc_fine_op* fop = nullptr;
+  mgc::FineSrc fine;
   mgc_mg_params params;
   std::unique_ptr<mgc::MGBase> impl;
   hipStream_t stream = nullptr;
@@ -481,17 +623,30 @@ extern "C" {
 int mgc_mg_params_default(mgc_mg_params* p) {
   return guarded([&] {
     MGC_CHECK(p, MGC_ERR_ARG, "null params");
-    // include/MultigridParams.h:95-133 (3 levels, K-cycle, FGMRES wrapper
-    // 0.1 / 10 / 5, GMRES smoother 1e-3 / 4 / 4, coarsest GMRES 5e-2 / 50 / 10)
+    // include/MultigridParams.h:95-133 (3 levels, blocks 4^4 then 2^4, one
+    // initial setup round of BiCGSTAB 1e-3 / 100 with pre- and
+    // post-orthonormalisation of test vectors, no refinement, K-cycle,
+    // FGMRES wrapper 0.1 / 10 / 5, GMRES smoother 1e-3 / 4 / 4, coarsest
+    // GMRES 5e-2 / 50 / 10); nb = 20 is the default NBASIS = 40 of
+    // src/Test_wilsonclover_mg.cc:39-40 (20 vectors, chirally doubled)
     p->nlevels = 3;
     for (int l = 0; l < 3; ++l)
       for (int d = 0; d < 4; ++d) p->block[l][d] = l == 0 ? 4 : 2;
-    p->nb[0] = p->nb[1] = p->nb[2] = 16;
+    p->nb[0] = p->nb[1] = p->nb[2] = 20;
     p->precision = MGC_F32;
     p->coarsen_mode = MGC_COARSEN_DIRSAVE;
     p->setup_iters = 1;
-    p->setup_solve_iters = 8;
+    p->setup_solve_iters = 100;
     p->setup_tol = 1e-3;
+    p->setup_solver = MGC_SOLVER_BICGSTAB;
+    p->setup_restart = 20;
+    p->setup_basis_type = MGC_BASIS_TEST_VECTORS;
+    p->setup_pre_orthonormalise = 1;
+    p->setup_post_orthonormalise = 1;
+    p->setup_refine_iters = 0;
+    p->setup_recursive_updates = 1;
+    p->smoother_solver = MGC_SOLVER_GMRES;
+    p->coarsest_solver = MGC_SOLVER_GMRES;
     p->smoother_tol = 1e-3;
     p->smoother_iters = 4;
     p->smoother_restart = 4;
@@ -505,14 +660,52 @@ int mgc_mg_params_default(mgc_mg_params* p) {
   });
 }
 
+static void check_params(const mgc_mg_params* params) {
+  MGC_CHECK(params->precision == MGC_F64 || params->precision == MGC_F32, MGC_ERR_PRECISION,
+            "hierarchy precision must be F64 or F32");
+  auto known = [](int t) {
+    return t == MGC_SOLVER_MR || t == MGC_SOLVER_GMRES || t == MGC_SOLVER_FGMRES ||
+           t == MGC_SOLVER_BICGSTAB;
+  };
+  MGC_CHECK(known(params->setup_solver) && known(params->coarsest_solver), MGC_ERR_ARG,
+            "unknown setup or coarsest solver type");
+  MGC_CHECK(params->smoother_solver == MGC_SOLVER_GMRES ||
+                params->smoother_solver == MGC_SOLVER_MR,
+            MGC_ERR_ARG, "the smoother is GMRES or MR");
+  MGC_CHECK(params->setup_basis_type == MGC_BASIS_TEST_VECTORS ||
+                params->setup_basis_type == MGC_BASIS_NULL_VECTORS,
+            MGC_ERR_ARG, "unknown basis-vector type");
+  MGC_CHECK(params->setup_refine_iters >= 0 && params->setup_iters >= 0, MGC_ERR_ARG,
+            "negative setup iteration count");
+}
+
+int mgc_mg_create_linop(mgc_mg** out, const mgc_fine_linop* lop, const mgc_mg_params* params) {
+  return guarded([&] {
+    MGC_CHECK(out && lop && lop->apply && params, MGC_ERR_ARG, "null argument");
+    MGC_CHECK(lop->precisions & (1 << MGC_F64), MGC_ERR_PRECISION,
+              "the fine operator must apply in F64 (outer solve, CoarsenOperator)");
+    MGC_CHECK(lop->max_nrhs >= 1, MGC_ERR_ARG, "max_nrhs < 1");
+    for (int d = 0; d < 4; ++d) MGC_CHECK(lop->dims[d] >= 1, MGC_ERR_SHAPE, "lattice extent < 1");
+    check_params(params);
+    auto* m = new mgc_mg();
+    m->fine.lop = *lop;
+    for (int d = 0; d < 4; ++d) m->fine.lat.L[d] = lop->dims[d];
+    m->fine.V = m->fine.lat.volume();
+    m->params = *params;
+    *out = m;
+  });
+}
+
 int mgc_mg_create(mgc_mg** out, mgc_fine_op* fop, const mgc_mg_params* params) {
   return guarded([&] {
     MGC_CHECK(out && fop && params, MGC_ERR_ARG, "null argument");
     MGC_CHECK(fop->gauge_set, MGC_ERR_STATE, "fine gauge field not set");
-    MGC_CHECK(params->precision == MGC_F64 || params->precision == MGC_F32, MGC_ERR_PRECISION,
-              "hierarchy precision must be F64 or F32");
+    check_params(params);
     auto* m = new mgc_mg();
-    m->fop = fop;
+    ok(mgc_fine_op_linop(fop, &m->fine.lop));
+    m->fine.fop = fop;
+    m->fine.lat = fop->lat;
+    m->fine.V = fop->V;
     m->params = *params;
     *out = m;
   });
@@ -528,9 +721,9 @@ int mgc_mg_setup(mgc_mg* m, void* stream) {
     hipStream_t s = as_stream(stream);
     m->impl.reset();
     if (m->params.precision == MGC_F64)
-      m->impl = std::make_unique<MG<double>>(m->fop, m->params, s);
+      m->impl = std::make_unique<MG<double>>(&m->fine, m->params, s);
     else
-      m->impl = std::make_unique<MG<float>>(m->fop, m->params, s);
+      m->impl = std::make_unique<MG<float>>(&m->fine, m->params, s);
     m->stream = s;
     m->impl->setup(s);
   });
@@ -560,7 +753,7 @@ int mgc_mg_solve(mgc_mg* m, const void* b, void* x, double tol, int maxit, int r
     MGC_CHECK(as_stream(stream) == m->stream, MGC_ERR_ARG, "use the stream given to setup");
     hipStream_t s = m->stream;
     Blas bl(s);
-    FineLin<double> A(m->fop);
+    FineLin<double> A(&m->fine);
     PrecAdapter P(m->impl.get());
     Gmres<double> outer(A.n, restart, true);
     MGC_HIP(hipStreamSynchronize(s));
@@ -589,7 +782,9 @@ int mgc_mg_solve_mrhs(mgc_mg* m, const void* b, void* x, int nrhs, int coarse_pr
     hipStream_t s = m->stream;
     m->impl->prepare_batch(nrhs, coarse_prec, s);
     BBlas bl(s);
-    BFineLin<double> A(m->fop, nrhs);
+    MGC_CHECK(nrhs <= m->fine.lop.max_nrhs, MGC_ERR_ARG,
+              "nrhs exceeds the right-hand sides the fine operator takes per call");
+    BFineLin<double> A(&m->fine, nrhs);
     BPrecAdapter P(m->impl.get());
     BGmres<double> outer(A.sh, restart, true);
     MGC_HIP(hipStreamSynchronize(s));
@@ -630,7 +825,12 @@ int mgc_fine_gmres_solve(mgc_fine_op* fop, const void* b, void* x, double tol, i
     MGC_CHECK(fop && b && x, MGC_ERR_ARG, "null argument");
     hipStream_t s = as_stream(stream);
     Blas bl(s);
-    FineLin<double> A(fop);
+    FineSrc src;
+    ok(mgc_fine_op_linop(fop, &src.lop));
+    src.fop = fop;
+    src.lat = fop->lat;
+    src.V = fop->V;
+    FineLin<double> A(&src);
     Gmres<double> solver(A.n, restart, false);
     const auto t0 = std::chrono::steady_clock::now();
     const KrylovStats st =

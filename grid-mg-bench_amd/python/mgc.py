@@ -15,6 +15,8 @@ from typing import Optional, Sequence
 import numpy as np
 
 F64, F32, F16, BF16 = 0, 1, 2, 3
+SOLVER_MR, SOLVER_GMRES, SOLVER_FGMRES, SOLVER_BICGSTAB = 1, 2, 3, 4
+BASIS_NULL_VECTORS, BASIS_TEST_VECTORS = 1, 2
 LAYOUT_FLAT, LAYOUT_SPLIT = 0, 1
 EVEN, ODD, FULL = 0, 1, -1
 
@@ -37,7 +39,31 @@ class MGParams(ctypes.Structure):
                 ("wrapper_tol", _c.c_double), ("wrapper_iters", _c.c_int),
                 ("wrapper_restart", _c.c_int), ("coarsest_tol", _c.c_double),
                 ("coarsest_iters", _c.c_int), ("coarsest_restart", _c.c_int),
-                ("seed", _c.c_ulonglong)]
+                ("seed", _c.c_ulonglong), ("setup_solver", _c.c_int),
+                ("setup_restart", _c.c_int), ("setup_basis_type", _c.c_int),
+                ("setup_pre_orthonormalise", _c.c_int), ("setup_post_orthonormalise", _c.c_int),
+                ("setup_refine_iters", _c.c_int), ("setup_recursive_updates", _c.c_int),
+                ("smoother_solver", _c.c_int), ("coarsest_solver", _c.c_int)]
+
+
+LINOP_APPLY = ctypes.CFUNCTYPE(_c.c_int, _c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p,
+                               _c.c_void_p, _c.c_int, _c.c_void_p)
+
+
+class FineLinop(ctypes.Structure):
+    """mgc_fine_linop of include/mgc.h: a fine operator given as a callback."""
+    _fields_ = [("apply", LINOP_APPLY), ("ctx", _c.c_void_p), ("dims", _c.c_int * 4),
+                ("precisions", _c.c_int), ("max_nrhs", _c.c_int),
+                ("gamma5_hermitian", _c.c_int)]
+
+
+class TestParams(ctypes.Structure):
+    """mgc_test_params of include/mgc.h (MGTestOtherParams)."""
+    _fields_ = [("outer_tol", _c.c_double), ("outer_maxit", _c.c_int),
+                ("outer_restart", _c.c_int), ("mass_setup", _c.c_double),
+                ("mass_solve", _c.c_double), ("csw", _c.c_double),
+                ("config", _c.c_char * 1024), ("source_type", _c.c_char * 32),
+                ("antiperiodic_t", _c.c_int)]
 
 
 class SolveStats(ctypes.Structure):
@@ -121,9 +147,19 @@ _SIGS = {
     "mgc_fine_gmres_solve": ([_vp, _vp, _vp, _c.c_double, _i, _i, _c.POINTER(SolveStats),
                               _vp], _i),
     "mgc_coarsen_operator": ([_vp, _vp, _vp, _i, _vp], _i),
+    "mgc_coarsen_set_variant": ([_i], _i),
+    "mgc_test_params_default": ([_c.POINTER(TestParams)], _i),
+    "mgc_params_read_xml": ([_c.c_char_p, _c.c_void_p, _c.c_void_p], _i),
+    "mgc_params_write_xml": ([_c.c_char_p, _c.c_void_p, _c.c_void_p], _i),
+    "mgc_gauge_read": ([_c.c_char_p, _ip, _vp, _sz, _c.POINTER(_c.c_double)], _i),
+    "mgc_gauge_write": ([_c.c_char_p, _i, _ip, _vp, _i], _i),
+    "mgc_coarsen_operator_linop": ([_vp, _c.POINTER(FineLinop), _vp, _i, _vp], _i),
+    "mgc_fine_op_linop": ([_vp, _c.POINTER(FineLinop)], _i),
+    "mgc_mg_create_linop": ([_c.POINTER(_vp), _c.POINTER(FineLinop), _c.POINTER(MGParams)], _i),
     "mgc_coarsen_coarse_operator": ([_vp, _vp, _vp, _i, _vp], _i),
     "mgc_aggregation_create_ex": ([_c.POINTER(_vp), _ip, _ip, _i, _i], _i),
     "mgc_aggregation_project_masked": ([_vp, _vp, _vp, _i, _i, _i, _i, _vp], _i),
+    "mgc_aggregation_set_ortho_method": ([_i], _i),
     "mgc_coarse_op_construct_remaining_links": ([_vp, _i, _vp], _i),
     "mgc_aggregation_create": ([_c.POINTER(_vp), _ip, _ip, _i], _i),
     "mgc_aggregation_destroy": ([_vp], _i),
@@ -574,6 +610,37 @@ def coarsen_operator(cop: CoarseOp, fop: FineOp, agg: "Aggregation", mode=COARSE
          stream.handle if stream else None)
 
 
+def fine_op_linop(fop: "FineOp") -> FineLinop:
+    """The library's own fine operator behind the callback interface."""
+    lop = FineLinop()
+    call("mgc_fine_op_linop", fop.handle, _c.byref(lop))
+    return lop
+
+
+def make_linop(fn, dims, precisions=(F64,), max_nrhs=1, gamma5_hermitian=True) -> FineLinop:
+    """A callback operator from a Python function
+    fn(point, prec, in_ptr, out_ptr, nrhs, stream) -> int (0 on success).
+    Keep the returned object alive while the library may call it."""
+    lop = FineLinop()
+    lop._py_fn = LINOP_APPLY(lambda ctx, point, prec, i, o, nrhs, st: fn(point, prec, i, o, nrhs,
+                                                                        st))
+    lop.apply = lop._py_fn
+    lop.ctx = None
+    for d in range(4):
+        lop.dims[d] = int(dims[d])
+    lop.precisions = sum(1 << p for p in precisions)
+    lop.max_nrhs = int(max_nrhs)
+    lop.gamma5_hermitian = int(bool(gamma5_hermitian))
+    return lop
+
+
+def coarsen_operator_linop(cop: CoarseOp, lop: FineLinop, agg: "Aggregation",
+                           mode=COARSEN_DIRSAVE, stream=None):
+    """CoarsenOperator of a callback operator."""
+    call("mgc_coarsen_operator_linop", cop.handle, _c.byref(lop), agg.handle, int(mode),
+         stream.handle if stream else None)
+
+
 def coarsen_coarse_operator(cop2: CoarseOp, cop1: CoarseOp, agg: "Aggregation",
                             mode=COARSEN_DIRSAVE, stream=None):
     """CoarsenOperator of a coarse operator onto the next level."""
@@ -602,9 +669,13 @@ def mg_params(**kw) -> MGParams:
 class Multigrid:
     """Handle of the multigrid hierarchy (MultiGridPreconditioner)."""
 
-    def __init__(self, fop: "FineOp", params: MGParams):
+    def __init__(self, fop, params: MGParams):
+        """fop: a FineOp, or a FineLinop (callback operator, kept alive here)."""
         h = _vp()
-        call("mgc_mg_create", _c.byref(h), fop.handle, _c.byref(params))
+        if isinstance(fop, FineLinop):
+            call("mgc_mg_create_linop", _c.byref(h), _c.byref(fop), _c.byref(params))
+        else:
+            call("mgc_mg_create", _c.byref(h), fop.handle, _c.byref(params))
         self.handle = h.value
         self.fop = fop
         self.params = params
@@ -668,3 +739,44 @@ def fine_gmres_solve(fop, b, x, tol=1e-10, maxit=1000, restart=20, stream=None) 
 def construct_remaining_links(cop: CoarseOp, disp_have=-1, stream=None):
     call("mgc_coarse_op_construct_remaining_links", cop.handle, int(disp_have),
          stream.handle if stream else None)
+
+
+# ----------------------------------------------------------------------------
+# on-disk formats (host only)
+# ----------------------------------------------------------------------------
+GAUGE_NERSC, GAUGE_ILDG = 0, 1
+
+
+def test_params() -> TestParams:
+    t = TestParams()
+    call("mgc_test_params_default", _c.byref(t))
+    return t
+
+
+def read_params_xml(path: str):
+    """(MGParams, TestParams) from the reference drivers' XML parameter file."""
+    mg, t = MGParams(), TestParams()
+    call("mgc_params_read_xml", os.fsencode(path), _c.byref(mg), _c.byref(t))
+    return mg, t
+
+
+def write_params_xml(path: str, mg: MGParams, test: Optional[TestParams] = None) -> None:
+    call("mgc_params_write_xml", os.fsencode(path), _c.byref(mg),
+         _c.byref(test) if test is not None else None)
+
+
+def read_gauge(path: str):
+    """(dims, gauge [4][V][3][3] complex128 on the host, plaquette) of a NERSC or ILDG file."""
+    dims = (_c.c_int * 4)()
+    call("mgc_gauge_read", os.fsencode(path), dims, None, _sz(0), None)
+    V = int(np.prod(list(dims)))
+    U = np.empty((4, V, 3, 3), dtype=np.complex128)
+    plaq = _c.c_double()
+    call("mgc_gauge_read", os.fsencode(path), dims, U.ctypes.data, _sz(U.nbytes), _c.byref(plaq))
+    return tuple(dims), U, plaq.value
+
+
+def write_gauge(path: str, dims, U: np.ndarray, fmt: int = GAUGE_NERSC, bits: int = 64) -> None:
+    U = np.ascontiguousarray(U, dtype=np.complex128)
+    d = (_c.c_int * 4)(*[int(v) for v in dims])
+    call("mgc_gauge_write", os.fsencode(path), int(fmt), d, U.ctypes.data, int(bits))

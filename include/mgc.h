@@ -179,8 +179,12 @@ int mgc_coarse_op_MooeeInvDag(mgc_coarse_op* op, const void* in, void* out, int 
  *     accumulation; links carry one FP32 scale per (site, point); in is
  *     either complex FP32 (quantised on the fly) or already half precision
  *     (in_prec == mrhs_prec) with in_scales[site][nrhs] from mgc_quantize.
+ * nbasis must be a multiple of 8 up to 64 (MGC_ERR_SHAPE otherwise); other
+ * than 16, 32, 48 and 64 the operand is zero-padded to the next multiple of 16.
  * The operator keeps a swizzled copy of its links for the last mrhs_prec
- * used (built from the F64/F32 links on first use or by prepare_mrhs). */
+ * used (built from the F64/F32 links on first use or by prepare_mrhs), and a
+ * quantisation workspace for FP32 input to an F16/BF16 apply, grown on the
+ * first call that needs more and kept for later calls. */
 int mgc_coarse_op_prepare_mrhs(mgc_coarse_op* op, int mrhs_prec, void* stream);
 int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scales, int in_prec,
                          void* out, int nrhs, int mrhs_prec, int dag, void* stream);
@@ -196,8 +200,8 @@ int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const voi
                                  const float* const* in_scales, int in_prec, void* const* out,
                                  int nrhs, int mrhs_prec, int dag, void* stream);
 /* Kernel-variant control of the multi-rhs stencil.  By default the first
- * call for a (precision, nbasis, column-tile) shape times every variant on
- * its own operands and keeps the fastest (autotuner).  variant >= 0 forces a
+ * call for a (precision, nbasis, column-tile, lattice) shape times every
+ * variant on its own operands and keeps the fastest (autotuner).  variant >= 0 forces a
  * variant (0 = LDS-staged operand, 1 = LDS-staged operand with non-temporal
  * link loads, 2 = register operand with two m-tiles per wavefront, 3 and 4 =
  * variants 0 and 1 with sites dealt to the eight XCDs in contiguous ranges,
@@ -206,10 +210,12 @@ int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const voi
  * straight to LDS with two m-tiles per wavefront, 8 = variant 7 with
  * non-temporal link loads and the XCD-contiguous order, 9 = variant 1 with
  * each XCD taking a 4d block of the lattice (halves of x, y and z, all t);
- * the batched apply
+ * variants 2, 7 and 8 are built for nbasis 32 and 64 and run as 0, 3 and 4
+ * for the other nbasis values; the batched apply
  * uses variants 0, 1 and 3 to 6 and tunes itself when another one is
- * forced); -1 restores autotuning.  get_variant reports the variant in use
- * (-1 = not tuned yet). */
+ * forced); -1 restores autotuning.  get_variant reports the forced variant,
+ * else the one tuned last for that (precision, nbasis, rhs count) on any
+ * lattice (-1 = not tuned yet). */
 int mgc_mrhs_set_variant(int variant);
 int mgc_mrhs_get_variant(int mrhs_prec, int nbasis, int nrhs, int* variant);
 /* rows x n complex FP32 -> rows x n complex F16/BF16 with scales[row] =
@@ -242,13 +248,22 @@ int mgc_aggregation_info(const mgc_aggregation* agg, int coarse_dims[4], int* nb
  * writable Subspace() (:75). */
 int mgc_aggregation_set_basis(mgc_aggregation* agg, const void* basis_f64, void* stream);
 int mgc_aggregation_get_basis(const mgc_aggregation* agg, void* basis_f64, void* stream);
-/* Block Gram-Schmidt per (block, chirality), FP64, `passes` sweeps over the
+/* Block orthonormalisation per (block, chirality), FP64: Cholesky QR in two
+ * rounds (see mgc_aggregation_set_ortho_method), or `passes` sweeps over the
  * columns (each column's inner products reduced together, projected again
  * when its norm drops below 1/sqrt(2)); for a basis of full rank the result is
  * the modified Gram-Schmidt one of the reference to rounding.
  * Orthogonalise (include/AggregationRework.h:80-89) -> aggregateOrthogonalise
  * (include/AggregationKernelsRework.h:266-341). */
 int mgc_aggregation_orthogonalise(mgc_aggregation* agg, int passes, void* stream);
+/* Method of mgc_aggregation_orthogonalise, process-wide: 0 (default) =
+ * Cholesky QR in two rounds (three streaming reads of each block panel and
+ * one write; a panel whose columns are nearly dependent, pivot^2 < 1e-14 of
+ * its column norm^2, falls back to the column sweep; the sweep alone for nb >
+ * 32), 1 = the column sweep everywhere.  Both give the reference's Q of a
+ * full-rank panel to rounding.  No counterpart in the reference: it serves
+ * tests and measurements. */
+int mgc_aggregation_set_ortho_method(int method);
 /* CheckOrthogonal (include/AggregationRework.h:91-108): errs_host[i] =
  * norm2(e_i - R v_i) summed over coarse sites (host array of nb doubles). */
 int mgc_aggregation_check_orthogonal(mgc_aggregation* agg, double* errs_host, void* stream);
@@ -308,6 +323,39 @@ int mgc_random_gauge(void* gauge_f64, const int dims[4], double eps, uint64_t se
                      uint64_t stream_id, void* stream);
 
 /* ------------------------------------------------------------------ */
+/* a caller's fine operator (the reference's LinearOperatorBase)       */
+/* ------------------------------------------------------------------ */
+/* The reference's CoarsenOperator and multigrid take any fine linear
+ * operator (CoarsenOperator(GridBase*, LinearOperatorBase<FineField>&,
+ * Aggregation&), include/CoarsenedMatrixRework.h:401-403; the MdagM operator
+ * of include/MultigridPreconditioner.h:123-124).  A caller hands its operator
+ * to the library as a callback:
+ *   apply(ctx, point, prec, in, out, nrhs, stream)
+ *     point 0..7 : out = the hopping term of that stencil point (OpDir with
+ *                  (dir, disp) of the point order, include/CoarsenedMatrixBaseline.h:628-633)
+ *     point 8    : out = the site-diagonal term (OpDiag)
+ *     point 9    : out = the whole operator (Op = sum of points 0..8)
+ *   in, out: device fields [site][nrhs][spin 4][colour 3], complex, in
+ *   precision prec (MGC_F64 or MGC_F32, among `precisions`), nrhs <= max_nrhs;
+ *   the work is enqueued on `stream` (or completed before returning); the
+ *   return value is 0 on success, anything else is reported as an error.
+ * The callback is called from the thread that called the library. */
+typedef int (*mgc_linop_apply_fn)(void* ctx, int point, int prec, const void* in, void* out,
+                                  int nrhs, void* stream);
+typedef struct mgc_fine_linop {
+  mgc_linop_apply_fn apply;
+  void* ctx;
+  int dims[4];          /* fine lattice extents */
+  int precisions;       /* bit mask of accepted precisions: (1 << MGC_F64) | (1 << MGC_F32) */
+  int max_nrhs;         /* largest nrhs of one call (>= 1) */
+  int gamma5_hermitian; /* nonzero: D^dag = g5 D g5 (needed by MGC_COARSEN_DIRSAVE) */
+} mgc_fine_linop;
+
+/* Fill a callback operator that applies the library's own Wilson-clover
+ * operator (F64 and F32, up to 256 rhs, gamma5-hermitian). */
+int mgc_fine_op_linop(mgc_fine_op* op, mgc_fine_linop* out);
+
+/* ------------------------------------------------------------------ */
 /* coarse operator construction                                        */
 /* ------------------------------------------------------------------ */
 enum { MGC_COARSEN_FULL = 0, MGC_COARSEN_DIRSAVE = 1 };
@@ -322,6 +370,22 @@ enum { MGC_COARSEN_FULL = 0, MGC_COARSEN_DIRSAVE = 1 };
  * 999-1032).  cop must have the aggregation's coarse lattice and N = 2*nb. */
 int mgc_coarsen_operator(mgc_coarse_op* cop, mgc_fine_op* fop, mgc_aggregation* agg, int mode,
                          void* stream);
+/* CoarsenOperator(FineGrid, linop, Aggregation) of a caller's operator
+ * (include/CoarsenedMatrixRework.h:401-611, in the inner/outer-mask form of
+ * include/CoarsenedMatrixBaseline.h:609-675): for batches of up to
+ * min(max_nrhs, 16) doubled basis vectors the callback applies every stencil
+ * point (FULL: 0..8; DIRSAVE: 0..8 as well, the forward points for their
+ * inner sites only, forward links rebuilt by gamma5-hermiticity), and the
+ * masked block projections of the results give the links in FP64, installed
+ * in `cop` with their self-link inverses.  The caller's operator is used in
+ * F64 only. */
+int mgc_coarsen_operator_linop(mgc_coarse_op* cop, const mgc_fine_linop* lop,
+                               mgc_aggregation* agg, int mode, void* stream);
+/* Kernel of mgc_coarsen_operator's build (process-wide): 0 = FP64 on the
+ * vector ALU, 1 = FP64 MFMA (v_mfma_f64_16x16x4_f64), -1 = the default (1).
+ * Both give the same links to FP64 rounding.  No counterpart in the
+ * reference: it serves tests and measurements. */
+int mgc_coarsen_set_variant(int variant);
 /* CoarsenOperator of a coarse operator (the next level of the hierarchy,
  * src/Benchmark_create_coarse_coarse_operator.cc:250-381): cop1 (F64) on the
  * fine side, agg created with site_elems = cop1's nbasis, result in cop2. */
@@ -336,6 +400,11 @@ int mgc_coarse_op_construct_remaining_links(mgc_coarse_op* op, int disp_have, vo
 /* ------------------------------------------------------------------ */
 /* multigrid hierarchy, K-cycle / V-cycle, outer solve                 */
 /* ------------------------------------------------------------------ */
+/* Solver types (MGUtilizedSolverType, include/MultigridParams.h:44-49, same
+ * values) and basis-vector types (MGBasisVectorsType, :36-39). */
+enum { MGC_SOLVER_MR = 1, MGC_SOLVER_GMRES = 2, MGC_SOLVER_FGMRES = 3, MGC_SOLVER_BICGSTAB = 4 };
+enum { MGC_BASIS_NULL_VECTORS = 1, MGC_BASIS_TEST_VECTORS = 2 };
+
 /* Parameters of the hierarchy (MultiGridParams, include/MultigridParams.h:73-139;
  * mgc_mg_params_default fills the reference defaults of :95-133). */
 typedef struct mgc_mg_params {
@@ -345,7 +414,7 @@ typedef struct mgc_mg_params {
   int precision;           /* hierarchy precision: MGC_F64 (plain) or MGC_F32 */
   int coarsen_mode;        /* MGC_COARSEN_FULL or MGC_COARSEN_DIRSAVE */
   int setup_iters;         /* test-vector rounds (setupInitialIter) */
-  int setup_solve_iters;   /* GMRES iterations per test vector and round */
+  int setup_solve_iters;   /* setup solver iterations per test vector and round (maxIter) */
   double setup_tol;
   double smoother_tol;     /* post-smoother GMRES */
   int smoother_iters;
@@ -357,6 +426,19 @@ typedef struct mgc_mg_params {
   int coarsest_iters;
   int coarsest_restart;
   unsigned long long seed; /* test-vector generator seed */
+  /* setup and solver types (the per-level vectors of the reference, one
+   * value for every level here) */
+  int setup_solver;        /* setupSolverType: MGC_SOLVER_BICGSTAB (default), _GMRES or _MR */
+  int setup_restart;       /* restart length of a GMRES setup solver (default 20) */
+  int setup_basis_type;    /* setupBasisVectorsType: MGC_BASIS_TEST_VECTORS (default):
+                              solve A psi = v from psi = 0; MGC_BASIS_NULL_VECTORS: relax
+                              A psi = 0 from psi = v for setup_solve_iters iterations */
+  int setup_pre_orthonormalise;  /* setupPreOrthonormalise (default 1) */
+  int setup_post_orthonormalise; /* setupPostOrthonormalise (default 1) */
+  int setup_refine_iters;  /* setupRefinementIter (default 0): rounds of v <- K-cycle(v) */
+  int setup_recursive_updates;   /* setupRecursiveUpdates (default 1) */
+  int smoother_solver;     /* smootherSolverType: MGC_SOLVER_GMRES (default) or _MR */
+  int coarsest_solver;     /* coarsestSolverType: MGC_SOLVER_GMRES (default), _BICGSTAB, _MR */
 } mgc_mg_params;
 
 typedef struct mgc_solve_stats {
@@ -373,6 +455,13 @@ int mgc_mg_params_default(mgc_mg_params* params);
 /* MultiGridPreconditioner over a fine Wilson-clover operator
  * (include/MultigridPreconditioner.h:137-163, createMGInstance :665). */
 int mgc_mg_create(mgc_mg** mg, mgc_fine_op* fop, const mgc_mg_params* params);
+/* The same hierarchy over a caller's fine operator (the reference's
+ * MultiGridPreconditioner takes any fine LinearOperatorBase,
+ * include/MultigridPreconditioner.h:123-124): the operator must apply in F64
+ * (outer FGMRES, true residuals, CoarsenOperator through the point callback);
+ * an FP32 hierarchy applies it in F32 where `precisions` allows, else through
+ * F64 with conversions.  The callback stays in use until mgc_mg_destroy. */
+int mgc_mg_create_linop(mgc_mg** mg, const mgc_fine_linop* lop, const mgc_mg_params* params);
 int mgc_mg_destroy(mgc_mg* mg);
 /* initialSetup (include/MultigridPreconditioner.h:165-211): test vectors on
  * every level, aggregation + block orthonormalisation, FP64 Galerkin
@@ -398,7 +487,8 @@ int mgc_mg_solve(mgc_mg* mg, const void* b, void* x, double tol, int maxit, int 
  * level: multi-rhs fine operator and transfers, and the coarse levels
  * through the multi-rhs MFMA stencil (mgc_coarse_op_M_mrhs) with operands in
  * coarse_prec (MGC_F32, MGC_F16 or MGC_BF16; FP32 accumulation).  Needs an
- * FP32 hierarchy whose coarse nbasis is 32 or 64.  stats holds nrhs entries:
+ * FP32 hierarchy whose coarse nbasis is a multiple of 8 up to 64.  stats
+ * holds nrhs entries:
  * per rhs its outer iteration count at convergence and its true residual;
  * precond_applies and seconds are those of the whole batch.  Replaces one
  * solve per rhs with the reference's solver on the multi-rhs (5d) grid
@@ -411,6 +501,53 @@ int mgc_mg_aggregation(mgc_mg* mg, int level, mgc_aggregation** agg);
 /* unpreconditioned restarted GMRES on the fine operator, FP64 (baseline solve) */
 int mgc_fine_gmres_solve(mgc_fine_op* fop, const void* b, void* x, double tol, int maxit,
                          int restart, mgc_solve_stats* stats, void* stream);
+
+/* ------------------------------------------------------------------ */
+/* on-disk formats of the reference's drivers                          */
+/* ------------------------------------------------------------------ */
+/* The driver parameters besides the multigrid ones (MGTestOtherParams,
+ * include/MultigridParams.h:140-176; mgc_test_params_default fills its
+ * defaults). */
+typedef struct mgc_test_params {
+  double outer_tol;        /* outerSolverTol */
+  int outer_maxit;         /* outerSolverMaxIter */
+  int outer_restart;       /* outerSolverRestartLength */
+  double mass_setup;       /* massSetup */
+  double mass_solve;       /* massSolve */
+  double csw;              /* csw */
+  char config[1024];       /* config: gauge file, "foo" = a random field */
+  char source_type[32];    /* sourceType: ones, random, gaussian */
+  int antiperiodic_t;      /* useAntiPeriodicBC */
+} mgc_test_params;
+int mgc_test_params_default(mgc_test_params* test);
+/* Read the reference drivers' XML parameter file (Grid's XML serialisation
+ * of MGTestParams: <grid><Params><mg>..</mg><test>..</test></Params></grid>,
+ * read by src/Test_wilsonclover_mg.cc:51-58; a bare <Multigrid> element of
+ * MultiGridParams is read too).  Members missing from the file keep the
+ * defaults; per-level vectors must hold one value for all levels (this
+ * build's parameters are per hierarchy), and the cycling strategy must be
+ * KCYCLE with an FGMRES wrapper.  Either output may be NULL. */
+int mgc_params_read_xml(const char* path, mgc_mg_params* mg, mgc_test_params* test);
+/* Write the same file (test may be NULL: its defaults are written). */
+int mgc_params_write_xml(const char* path, const mgc_mg_params* mg, const mgc_test_params* test);
+
+enum { MGC_GAUGE_NERSC = 0, MGC_GAUGE_ILDG = 1 };
+/* Read a gauge configuration in the NERSC format (4D_SU3_GAUGE two-row or
+ * 4D_SU3_GAUGE_3x3, IEEE32/64 big or little endian, header plaquette, link
+ * trace and checksum verified) or in ILDG (LIME records ildg-format and
+ * ildg-binary-data, big-endian, precision 32 or 64), as
+ * src/Test_wilsonclover_mg.cc:94-101 reads a configuration with IldgReader.
+ * dims receives the lattice extents; gauge_host_f64 (host memory, NULL to
+ * read only the header) receives [mu][site][3][3] complex double, at least
+ * 4 * volume * 9 * 16 bytes of capacity; plaquette (may be NULL) the average
+ * plaquette Re tr / 3. */
+int mgc_gauge_read(const char* path, int dims[4], void* gauge_host_f64, size_t capacity_bytes,
+                   double* plaquette);
+/* Write a host gauge field [mu][site][3][3] complex double as a NERSC
+ * (4D_SU3_GAUGE_3x3, big endian, with plaquette, link trace and checksum) or
+ * ILDG file in 32- or 64-bit precision. */
+int mgc_gauge_write(const char* path, int format, const int dims[4], const void* gauge_host_f64,
+                    int precision_bits);
 
 #ifdef __cplusplus
 }

@@ -28,6 +28,15 @@ def test_mg_params_layout_and_reference_defaults():
     assert (p.wrapper_tol, p.wrapper_iters, p.wrapper_restart) == (1e-1, 10, 5)
     assert (p.coarsest_tol, p.coarsest_iters, p.coarsest_restart) == (5e-2, 50, 10)
     assert p.seed == 1 | (2 << 16) | (3 << 32) | (4 << 48)
+    # setup: BiCGSTAB 1e-3 / 100, test vectors, pre- and post-orthonormalised,
+    # no refinement (include/MultigridParams.h:97-107); GMRES smoother and
+    # coarsest solver; nb = 20 (NBASIS 40, src/Test_wilsonclover_mg.cc:39-40)
+    assert (p.setup_solver, p.setup_tol, p.setup_solve_iters) == (mgc.SOLVER_BICGSTAB, 1e-3, 100)
+    assert (p.setup_iters, p.setup_refine_iters) == (1, 0)
+    assert p.setup_basis_type == mgc.BASIS_TEST_VECTORS
+    assert (p.setup_pre_orthonormalise, p.setup_post_orthonormalise) == (1, 1)
+    assert (p.smoother_solver, p.coarsest_solver) == (mgc.SOLVER_GMRES, mgc.SOLVER_GMRES)
+    assert list(p.nb) == [20, 20, 20]
     q = mgc.mg_params(precision=mgc.F64, nb=[12, 8], block=[(2, 2, 2, 2), (3, 3, 3, 3)])
     assert q.precision == mgc.F64 and list(q.nb)[:2] == [12, 8] and list(q.block[1]) == [3] * 4
 

@@ -18,7 +18,9 @@ from helpers import g5_links, half_sites, rand_links, rand_vec, rel
 pytestmark = pytest.mark.gpu
 
 CASES = [((2, 2, 2, 2), 24), ((4, 2, 2, 2), 8), ((2, 4, 2, 2), 32), ((4, 4, 2, 2), 64),
-         ((2, 2, 2, 4), 4), ((4, 2, 4, 2), 12), ((2, 2, 2, 2), 48), ((4, 2, 2, 2), 16)]
+         ((2, 2, 2, 4), 4), ((4, 2, 4, 2), 12), ((2, 2, 2, 2), 48), ((4, 2, 2, 2), 16),
+         ((2, 2, 2, 2), 40), ((4, 2, 2, 2), 40), ((2, 2, 2, 2), 2), ((2, 2, 2, 2), 6),
+         ((2, 4, 2, 2), 18), ((2, 2, 2, 2), 62)]
 
 
 def tol(prec, N):
@@ -76,7 +78,41 @@ def test_full_lattice_kinds(gpu, dims, N, prec):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("dims,N", [((4, 2, 2, 2), 8), ((2, 2, 4, 2), 32), ((4, 2, 2, 2), 64)])
+@pytest.mark.parametrize("dims,N", [((2, 4, 2, 2), 40), ((4, 2, 2, 2), 64)])
+def test_hermitian_operator_dagger_is_M(gpu, dims, N, prec):
+    """Petrov-Galerkin operators (hermitian = 1, include/CoarsenedMatrixUpstream.h:609-621):
+    Mdag and the multi-rhs dag apply return M unchanged, while Dhop(dag) keeps
+    the G5C sandwich for either kind, as the reference's DhopInternal does
+    (include/CoarsenedMatrixRework.h:293-300)."""
+    print(f"[case] hermitian dims={dims} N={N} prec={prec}", flush=True)
+    links = rand_links(dims, N, 131)
+    op = gpu.CoarseOp(dims, N, prec, hermitian=1)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    V = int(np.prod(dims))
+    v = rand_vec(dims, N, 132)
+    vq = v.astype(gpu.complex_dtype(prec)).astype(np.complex128)
+    t = tol(prec, N)
+    ref = oracle.coarse_apply(dims, N, links, vq)
+    assert rel(ref, run(gpu, lambda i, o: op.Mdag(i, o), v, prec, (V, N))) < t
+    refh = oracle.coarse_apply(dims, N, links, vq, 0xFF, dag=1)
+    assert rel(refh, run(gpu, lambda i, o: op.Dhop(i, o, 1), v, prec, (V, N))) < t
+    if prec == 1:
+        nrhs = 5
+        vm = rand_vec(dims, N, 133, nrhs=nrhs).astype(np.complex64)
+        din = gpu.DeviceBuffer.from_numpy(vm)
+        dout = gpu.DeviceBuffer(V * nrhs * N * 8)
+        op.M_mrhs(din, dout, nrhs, mrhs_prec=gpu.F32, dag=1)
+        gpu.synchronize()
+        got = dout.to_numpy(np.complex64, (V, nrhs, N)).astype(np.complex128)
+        for k in range(nrhs):
+            r = oracle.coarse_apply(dims, N, links, vm[:, k].astype(np.complex128))
+            assert rel(r, got[:, k]) < 2e-6 * math.sqrt(N / 8), k
+    op.close()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("dims,N", [((4, 2, 2, 2), 8), ((2, 2, 4, 2), 32), ((4, 2, 2, 2), 64),
+                                    ((2, 2, 2, 4), 40)])
 def test_checkerboarded_kinds(gpu, dims, N, prec):
     print(f"[case] cb dims={dims} N={N} prec={prec}", flush=True)
     links = rand_links(dims, N, 111)

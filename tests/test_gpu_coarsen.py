@@ -47,8 +47,12 @@ def test_clover_term_matches_oracle(gpu, dims):
 
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("nrhs", [1, 3])
-def test_fine_operator_matches_oracle(gpu, prec, nrhs):
-    dims = (4, 4, 2, 6)
+@pytest.mark.parametrize("dims", [(4, 4, 2, 6), (4, 4, 2, 8)])
+def test_fine_operator_matches_oracle(gpu, prec, nrhs, dims):
+    """(4, 4, 2, 8) has L3 % 8 == 0, so the apply's tiles are dealt to the
+    eight XCDs by time slices (the order used at 32^4 and 48^4); (4, 4, 2, 6)
+    takes the plain tile order."""
+    print(f"[case] fine op dims={dims} prec={prec} nrhs={nrhs}", flush=True)
     V = int(np.prod(dims))
     fop, U = fine_setup(gpu, dims, 41)
     diag = oracle.wilson_diag(dims, U, MASS, CSW)
@@ -118,13 +122,27 @@ def test_fine_operator_kernel_variants_bit_equal(gpu, prec, dims, nrhs):
 CO_CASES = [((8, 8, 8, 8), (4, 4, 4, 4), 12),   # the reference's CPU case: 8^4 -> 2^4, N = 24
             ((6, 4, 2, 2), (2, 2, 2, 2), 4),    # odd coarse extent 3
             ((4, 4, 2, 2), (4, 2, 2, 2), 8),    # coarse extents 1 and 2
-            ((4, 4, 4, 4), (2, 2, 2, 2), 16)]
+            ((4, 4, 4, 4), (2, 2, 2, 2), 16),
+            ((8, 8, 8, 8), (4, 4, 4, 4), 20),   # the reference's default nbasis 40
+            ((4, 4, 4, 4), (2, 2, 2, 2), 30)]
+
+
+@pytest.fixture(params=[1, 0], ids=["mfma", "vector"])
+def coarsen_variant(gpu, request):
+    """Both FP64 kernels of the CoarsenOperator build: MFMA (default) and vector ALU."""
+    gpu.call("mgc_coarsen_set_variant", request.param)
+    yield request.param
+    gpu.call("mgc_coarsen_set_variant", -1)
 
 
 @pytest.mark.parametrize("mode", [0, 1])
-@pytest.mark.parametrize("fd,blk,nb", CO_CASES)
-def test_coarsen_operator_matches_oracle(gpu, fd, blk, nb, mode):
-    print(f"[case] coarsen fd={fd} blk={blk} nb={nb} mode={mode}", flush=True)
+@pytest.mark.parametrize("fd,blk,nb", CO_CASES + [((4, 4, 4, 4), (2, 2, 2, 2), 7),
+                                                  ((4, 4, 4, 4), (2, 2, 2, 2), 17)])
+def test_coarsen_operator_matches_oracle(gpu, fd, blk, nb, mode, coarsen_variant):
+    if coarsen_variant == 0 and nb in (7, 17):
+        pytest.skip("the vector form takes nb = 2 and even nb from 4 to 32")
+    print(f"[case] coarsen fd={fd} blk={blk} nb={nb} mode={mode} variant={coarsen_variant}",
+          flush=True)
     Vf = int(np.prod(fd))
     fop, U = fine_setup(gpu, fd, 51)
     diag = oracle.wilson_diag(fd, U, MASS, CSW)
@@ -194,23 +212,30 @@ def test_coarse_of_coarse_matches_oracle(gpu, d1, blk, N1, nb2, mode):
         assert rel(ref[p], got[p]) < 1e-12, f"point {p}"
 
 
-def test_full_size_galerkin_identity_cfg4(gpu):
-    """Fine 32^4, 4^4 blocks, nb = 32 (coarse 8^4, N = 64): D_c v = R D P v on
-    random coarse vectors, every operator the library's own, gauge field and
-    basis generated on the device."""
+CFG4_SETUP = dict(nlevels=2, block=[(4, 4, 4, 4)], nb=[32], precision=0, setup_iters=8,
+                  setup_solve_iters=4)
+
+
+def test_full_size_galerkin_identity_cfg4(gpu, coarsen_variant):
+    """Fine 32^4, 4^4 blocks, nbasis 64 (coarse 8^4, nb = 32): the basis is
+    generated by 8 seeded inverse iterations (setup rounds of BiCGSTAB,
+    4 iterations each, on the Wilson-clover operator, pre- and
+    post-orthonormalised, then block-orthonormalised), and D_c v = R D P v on
+    random coarse vectors for the coarse operator CoarsenOperator builds from
+    it; every operator the library's own, all fields generated on the device."""
     fd, blk, nb = (32, 32, 32, 32), (4, 4, 4, 4), 32
     Vf, Vc, N = 32 ** 4, 8 ** 4, 64
-    print(f"[case] full-size coarsen fd={fd} nb={nb}", flush=True)
+    print(f"[case] full-size coarsen fd={fd} nb={nb} variant={coarsen_variant}", flush=True)
     s = gpu.Stream()
     g = gpu.DeviceBuffer(4 * Vf * 9 * 16)
     gpu.random_gauge(g, fd, 0.4, 5, 1, s)
     fop = gpu.FineOp(fd, MASS, CSW)
     fop.set_gauge(g, s)
-    agg = gpu.Aggregation(fd, blk, nb)
-    basis = gpu.DeviceBuffer(nb * Vf * 12 * 16)
-    gpu.random_gaussian(basis, nb * Vf * 12, gpu.F64, 5, 2, s)
-    agg.set_basis(basis, s)
-    agg.orthogonalise(2, s)
+    s.sync()
+    mg = gpu.Multigrid(fop, gpu.mg_params(**CFG4_SETUP))
+    mg.setup()
+    agg = gpu.Aggregation.borrow(mg.aggregation_handle(0), fd, blk, nb)
+    assert agg.check_orthogonal().max() < 1e-20 * Vc
     cop = gpu.CoarseOp((8, 8, 8, 8), N, gpu.F64)
     e0, e1 = gpu.Event(), gpu.Event()
     e0.record(s)
@@ -232,3 +257,9 @@ def test_full_size_galerkin_identity_cfg4(gpu):
     a = lhs.to_numpy(np.complex128, (Vc, N))
     b = rhs.to_numpy(np.complex128, (Vc, N))
     assert rel(b, a) < 1e-12
+    # the hierarchy's own coarse operator is the same build
+    own = gpu.CoarseOp.borrow(mg.coarse_op_handle(1), (8, 8, 8, 8), N, gpu.F64)
+    own.M(vc, lhs, s)
+    s.sync()
+    assert rel(b, lhs.to_numpy(np.complex128, (Vc, N))) < 1e-12
+    mg.close()

@@ -156,3 +156,53 @@ def test_batched_solve_each_rhs_reaches_1e10(gpu, small, cprec):
         assert res <= 1.2e-10
         assert sts[r]["iterations"] <= st1["iterations"] + 3
     mg.close()
+
+
+SOLVER_CASES = [
+    # (label, overrides)
+    ("mr-smoother", dict(smoother_solver=1, smoother_iters=4)),
+    ("bicgstab-coarsest", dict(coarsest_solver=4, coarsest_iters=50)),
+    ("gmres-setup", dict(setup_solver=2, setup_solve_iters=20, setup_restart=10)),
+    ("null-vectors", dict(setup_basis_type=1, setup_solve_iters=20)),
+    ("refinement", dict(setup_refine_iters=1)),
+]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("label,over", SOLVER_CASES, ids=[c[0] for c in SOLVER_CASES])
+def test_solver_types_and_setup_variants_reach_1e10(gpu, small, prec, label, over):
+    """The selectable solver types (MR smoother, BiCGSTAB coarsest solver,
+    include/Helpers.h:38-56), the setup variants (GMRES setup solver, the
+    null-vector setup of include/BasisVectors.h:99-105, one refinement round
+    of include/MultigridPreconditioner.h:213-251): each hierarchy preconditions
+    FGMRES to 1e-10 in the recomputed true residual."""
+    dims, V, fop, b = small
+    print(f"[case] mg {label} prec={prec}", flush=True)
+    mg = gpu.Multigrid(fop, gpu.mg_params(precision=prec, **SMALL, **over))
+    mg.setup()
+    x = gpu.DeviceBuffer(V * 12 * 16)
+    st = mg.solve(b, x, tol=1e-10, maxit=300, restart=20)
+    res = true_residual(gpu, fop, b, x, V)
+    print(f"  {label}: {st} recomputed residual {res:.3e}", flush=True)
+    assert st["converged"] == 1 and res < 1.2e-10
+    mg.close()
+
+
+def test_batched_solve_with_mr_smoother(gpu, small):
+    dims, V, fop, _ = small
+    nrhs = 3
+    print(f"[case] batched mg solve MR smoother nrhs={nrhs}", flush=True)
+    mg = gpu.Multigrid(fop, gpu.mg_params(precision=1, smoother_solver=1, **MRHS_CFG))
+    mg.setup()
+    bs = gauss(V * nrhs * 12, 94).reshape(V, nrhs, 12)
+    b = gpu.DeviceBuffer.from_numpy(bs)
+    x = gpu.DeviceBuffer(V * nrhs * 12 * 16)
+    sts = mg.solve_mrhs(b, x, nrhs, coarse_prec=1, tol=1e-10, maxit=300, restart=20)
+    xs = x.to_numpy(np.complex128, (V, nrhs, 12))
+    for r in range(nrhs):
+        br = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(bs[:, r]))
+        xr = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(xs[:, r]))
+        res = true_residual(gpu, fop, br, xr, V)
+        print(f"  rhs {r}: {sts[r]} residual {res:.3e}", flush=True)
+        assert sts[r]["converged"] == 1 and res <= 1.2e-10
+    mg.close()

@@ -56,6 +56,55 @@ def test_mrhs_matches_oracle(gpu, dims, N, nrhs, mp):
         assert max(rel(ref[:, k], got_h[:, k]) for k in range(nrhs)) < TOL[mp](N)
 
 
+@pytest.mark.parametrize("mp", [F32, F16, BF16])
+@pytest.mark.parametrize("N", [8, 16, 24, 40, 48, 56])
+def test_mrhs_other_nbasis_match_oracle(gpu, N, mp):
+    """nbasis values other than 32 and 64 (40 is the reference's default,
+    src/Benchmark_coarse_operator.cc:46-47): the operand is zero-padded to
+    16-row tiles; M and Mdag against the oracle, in one and two column passes
+    (nrhs 70 > 64), with FP32 and pre-quantised half-precision inputs."""
+    dims = (2, 4, 2, 2)
+    for nrhs in (12, 70):
+        print(f"[case] mrhs N={N} nrhs={nrhs} prec={mp}", flush=True)
+        links = rand_links(dims, N, 661 + N)
+        op = gpu.CoarseOp(dims, N, gpu.F64)
+        op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+        v = rand_vec(dims, N, 662 + N, nrhs=nrhs)
+        vq = v.astype(np.complex64).astype(np.complex128)
+        for dag in (0, 1):
+            got = run_mrhs(gpu, op, v, nrhs, mp, dag=dag)
+            goth = run_mrhs(gpu, op, v, nrhs, mp, dag=dag, half_input=True) if mp != F32 else got
+            for k in range(nrhs):
+                r = oracle.coarse_apply(dims, N, links, np.ascontiguousarray(vq[:, k]), dag=dag)
+                assert rel(r, got[:, k]) < TOL[mp](N), (dag, k)
+                assert rel(r, goth[:, k]) < TOL[mp](N), (dag, k)
+        op.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("mp", [F32, F16, BF16])
+@pytest.mark.parametrize("N", [40])
+def test_mrhs_every_kernel_variant_n40(gpu, variant, mp, N):
+    """Every variant request at nbasis 40 (2, 7 and 8 run as 0, 3 and 4 there)."""
+    dims, nrhs = (4, 2, 2, 2), 20
+    print(f"[case] mrhs N={N} variant={variant} prec={mp}", flush=True)
+    links = rand_links(dims, N, 671)
+    op = gpu.CoarseOp(dims, N, gpu.F64)
+    op.set_links(gpu.DeviceBuffer.from_numpy(links), gpu.LAYOUT_FLAT, invert=False)
+    v = rand_vec(dims, N, 672, nrhs=nrhs)
+    vq = v.astype(np.complex64).astype(np.complex128)
+    gpu.call("mgc_mrhs_set_variant", variant)
+    try:
+        for dag in (0, 1):
+            got = run_mrhs(gpu, op, v, nrhs, mp, dag=dag)
+            for k in range(nrhs):
+                r = oracle.coarse_apply(dims, N, links, np.ascontiguousarray(vq[:, k]), dag=dag)
+                assert rel(r, got[:, k]) < TOL[mp](N), (dag, k)
+    finally:
+        gpu.call("mgc_mrhs_set_variant", -1)
+    op.close()
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 def test_mrhs_every_kernel_variant(gpu, variant, mp):
@@ -183,7 +232,7 @@ def _batched_case(gpu, dims_list, N, nrhs, mp, seed, herm=None):
 
 
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
-@pytest.mark.parametrize("N,nrhs", [(32, 12), (64, 24), (32, 70)])
+@pytest.mark.parametrize("N,nrhs", [(32, 12), (64, 24), (32, 70), (40, 12), (24, 20)])
 def test_mrhs_batched_lattices(gpu, N, nrhs, mp):
     """Lattices of uneven volume in one launch: each matches the oracle, and,
     with an LDS-staged variant forced, matches its own single-lattice launch

@@ -28,6 +28,7 @@
 // scale; a point's product is accumulated in its own FP32 tile and folded
 // into the result as C += s_link(x,p) * s_in(x+d_p, j) * T_p.
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <type_traits>
 
@@ -307,6 +308,33 @@ static void quantize(const void* in, size_t rows, int N, int prec, void* out, fl
   MGC_LAUNCH_CHECK();
 }
 
+// Scratch of the no-allocation multi-rhs apply: the quantised FP32 input
+// (sites x rhs rows of half-precision complex, then one FP32 scale per row).
+// One copy per device, held until the process ends; every call on a device
+// shares it, so calls that overlap on one device race.  Host code uses the
+// device address from hipGetSymbolAddress (scratch_ptr), never the symbol.
+constexpr size_t MRHS_SCRATCH_BYTES = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[MRHS_SCRATCH_BYTES];
+constexpr int MRHS_MAX_DEVICES = 64;
+static void* g_scratch_ptr[MRHS_MAX_DEVICES];
+static std::mutex g_scratch_mutex;
+
+// device address of this device's scratch, looked up once per device on the
+// first call made with that device current
+static void* scratch_ptr() {
+  int dev = 0;
+  MGC_HIP(hipGetDevice(&dev));
+  MGC_CHECK(dev >= 0 && dev < MRHS_MAX_DEVICES, MGC_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    MGC_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    MGC_CHECK(ptr, MGC_ERR_HIP, "no device address for the multi-rhs scratch");
+    g_scratch_ptr[dev] = ptr;
+  }
+  return g_scratch_ptr[dev];
+}
+
 static void check_mrhs_args(const mgc_coarse_op* op, const void* in, const void* out, int nrhs,
                             int mrhs_prec) {
   MGC_CHECK(op && in && out && in != out, MGC_ERR_ARG, "null or aliased argument");
@@ -430,6 +458,30 @@ int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scal
       qsc = op->q_scale;
     }
     mrhs_apply(op, in, in_scales, in_prec, out, nrhs, mrhs_prec, dag, s, true, qbuf, qsc);
+  });
+}
+
+int mgc_coarse_op_M_mrhs_noalloc(mgc_coarse_op* op, const void* in_f32, void* out, int nrhs,
+                                 int mrhs_prec, int dag, void* stream) {
+  return guarded([&] {
+    check_mrhs_args(op, in_f32, out, nrhs, mrhs_prec);
+    MGC_CHECK(op->mrhs_links && op->mrhs_prec == mrhs_prec, MGC_ERR_STATE,
+              "multi-rhs operand not prepared for this precision "
+              "(call mgc_coarse_op_prepare_mrhs first)");
+    void* qbuf = nullptr;
+    float* qsc = nullptr;
+    if (mrhs_prec != MGC_F32) {
+      const size_t rows = (size_t)op->V * nrhs;
+      const size_t qbytes = (rows * op->N * 4 + 255) / 256 * 256;
+      MGC_CHECK(qbytes + rows * sizeof(float) <= MRHS_SCRATCH_BYTES, MGC_ERR_SHAPE,
+                "sites x nrhs x (4 nbasis + 4) bytes exceed the 64 MiB scratch of the "
+                "no-allocation multi-rhs apply");
+      char* base = (char*)scratch_ptr();
+      qbuf = base;
+      qsc = (float*)(base + qbytes);
+    }
+    mrhs_apply(op, in_f32, nullptr, MGC_F32, out, nrhs, mrhs_prec, dag, as_stream(stream), false,
+               qbuf, qsc);
   });
 }
 

@@ -121,6 +121,7 @@ _SIGS = {
     "mgc_coarse_op_prepare_mrhs": ([_vp, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs_batched": ([_vp, _i, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
+    "mgc_coarse_op_M_mrhs_noalloc": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
     "mgc_mrhs_set_variant": ([_i], _i),
     "mgc_mrhs_get_variant": ([_i, _i, _i, _ip], _i),
@@ -447,6 +448,12 @@ class CoarseOp:
                stream=None):
         call("mgc_coarse_op_M_mrhs", self.handle, _p(i), _p(in_scales), int(in_prec), _p(o),
              int(nrhs), int(mrhs_prec), int(dag), self._s(stream))
+
+
+    def M_mrhs_noalloc(self, i, o, nrhs, mrhs_prec=F16, dag=0, stream=None):
+        """FP32 input, no device allocation (operand prepared beforehand)."""
+        call("mgc_coarse_op_M_mrhs_noalloc", self.handle, _p(i), _p(o), int(nrhs),
+             int(mrhs_prec), int(dag), self._s(stream))
 
 
 def M_mrhs_batched(ops, ins, outs, nrhs, mrhs_prec=F32, in_scales=None, dag=0, stream=None):

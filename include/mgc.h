@@ -199,6 +199,19 @@ int mgc_coarse_op_M_mrhs(mgc_coarse_op* op, const void* in, const float* in_scal
 int mgc_coarse_op_M_mrhs_batched(mgc_coarse_op* const* ops, int count, const void* const* in,
                                  const float* const* in_scales, int in_prec, void* const* out,
                                  int nrhs, int mrhs_prec, int dag, void* stream);
+/* The multi-rhs M of mgc_coarse_op_M_mrhs for complex FP32 input, calling no
+ * device allocator: no counterpart in the reference.  It is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  For
+ * mrhs_prec MGC_F16 / MGC_BF16 the quantised input is staged in a 64 MiB
+ * scratch array of the library that every device holds until the process
+ * ends, so sites x nrhs x (4 nbasis + 4) bytes must not exceed 64 MiB
+ * (MGC_ERR_SHAPE otherwise); MGC_F32 needs no scratch.  The operand must be
+ * prepared for mrhs_prec beforehand (mgc_coarse_op_prepare_mrhs, else
+ * MGC_ERR_STATE), and a shape the autotuner has not timed runs the default
+ * kernel variant.  All calls on one device share its scratch: calls that
+ * overlap on one device race. */
+int mgc_coarse_op_M_mrhs_noalloc(mgc_coarse_op* op, const void* in_f32, void* out, int nrhs,
+                                 int mrhs_prec, int dag, void* stream);
 /* Kernel-variant control of the multi-rhs stencil.  By default the first
  * call for a (precision, nbasis, column-tile, lattice) shape times every
  * variant on its own operands and keeps the fastest (autotuner).  variant >= 0 forces a

@@ -151,10 +151,19 @@ def fine_op_leg(mgc, stream, dims, prec, nrhs, steps, warmup):
 
 
 def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
-    """CoarsenOperator (FP64, backward points + self, remaining links by
-    gamma5-hermiticity).  Contraction flops per coarse site: for each computed
-    point, every site of its face (the whole block for self) contributes
-    4 nb^2 link entries x 6 spin-colour terms x 8 flops."""
+    """The nbasis-2nb construction workload: basis vectors from 8 seeded inverse
+    iterations (a 2-level hierarchy's setup: 8 rounds of BiCGSTAB, 4 iterations
+    each, on the Wilson-clover operator, FP64, pre- and post-orthonormalised),
+    then
+
+    * the block orthonormalisation of that basis: Cholesky QR (two Gram rounds
+      and one apply, 3 reads + 1 write of the panel) against the column sweep;
+      algorithmic bytes = one read and one write of the panel;
+    * CoarsenOperator (FP64, backward points + self, remaining links by
+      gamma5-hermiticity), MFMA and vector-ALU builds.  Contraction flops per
+      coarse site: every site of each computed point's face (the whole block
+      for self) contributes 4 nb^2 link entries x 6 spin-colour terms x 8 flops.
+    """
     Vf = int(np.prod(fine_dims))
     cd = tuple(f // b for f, b in zip(fine_dims, blk))
     Vc = int(np.prod(cd))
@@ -164,35 +173,60 @@ def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
     mgc.random_gauge(g, fine_dims, 0.4, SEED, 31, stream)
     fop = mgc.FineOp(fine_dims, -0.1, 1.0)
     fop.set_gauge(g, stream)
-    agg = mgc.Aggregation(fine_dims, blk, nb)
+    stream.sync()
+    p = mgc.mg_params(nlevels=2, block=[blk], nb=[nb], precision=mgc.F64, setup_iters=8,
+                      setup_solve_iters=4)
+    mg = mgc.Multigrid(fop, p)
+    t0 = time.perf_counter()
+    mg.setup(stream)
+    t_setup = time.perf_counter() - t0
+    src = mgc.Aggregation.borrow(mg.aggregation_handle(0), fine_dims, blk, nb)
     basis = mgc.DeviceBuffer(nb * Vf * 12 * 16)
-    mgc.random_gaussian(basis, nb * Vf * 12, mgc.F64, SEED, 32, stream)
-    agg.set_basis(basis, stream)
+    src.get_basis(basis, stream)
+    agg = mgc.Aggregation(fine_dims, blk, nb)
     e0, e1 = mgc.Event(), mgc.Event()
-    e0.record(stream)
+    panel_bytes = nb * Vf * 12 * 16
+    orth = {}
+    for method, name in ((0, "cholqr"), (1, "sweep")):
+        mgc.call("mgc_aggregation_set_ortho_method", method)
+        agg.set_basis(basis, stream)
+        e0.record(stream)
+        agg.orthogonalise(2, stream)
+        e1.record(stream)
+        e1.sync()
+        t = e0.elapsed_ms(e1) * 1e-3
+        orth[name] = {"ms": t * 1e3, "GBps": 2 * panel_bytes / t / 1e9,
+                      "hbm_frac": 2 * panel_bytes / t / 1e9 / HBM_PEAK_GBS}
+    mgc.call("mgc_aggregation_set_ortho_method", 0)
+    agg.set_basis(basis, stream)
     agg.orthogonalise(2, stream)
-    e1.record(stream)
-    e1.sync()
-    t_orth = e0.elapsed_ms(e1) * 1e-3
-    cop = mgc.CoarseOp(cd, N, mgc.F64)
-    mgc.coarsen_operator(cop, fop, agg, mgc.COARSEN_DIRSAVE, stream)  # warm-up
-    e0.record(stream)
-    for _ in range(reps):
-        mgc.coarsen_operator(cop, fop, agg, mgc.COARSEN_DIRSAVE, stream)
-    e1.record(stream)
-    e1.sync()
-    t = e0.elapsed_ms(e1) * 1e-3 / reps
     face_sites = sum(Vb // b for b in blk) + Vb
     flops = Vc * face_sites * 4 * nb * nb * 6 * 8
+    cop = mgc.CoarseOp(cd, N, mgc.F64)
+    build = {}
+    for variant, name in ((1, "mfma"), (0, "vector")):
+        mgc.call("mgc_coarsen_set_variant", variant)
+        mgc.coarsen_operator(cop, fop, agg, mgc.COARSEN_DIRSAVE, stream)  # warm-up
+        e0.record(stream)
+        for _ in range(reps):
+            mgc.coarsen_operator(cop, fop, agg, mgc.COARSEN_DIRSAVE, stream)
+        e1.record(stream)
+        e1.sync()
+        t = e0.elapsed_ms(e1) * 1e-3 / reps
+        build[name] = {"ms": t * 1e3, "TFLOPs": flops / t / 1e12,
+                       "fp64_frac": flops / t / 1e12 / PEAK_TFLOPS["f64"]}
+    mgc.call("mgc_coarsen_set_variant", -1)
     res = {"workload": f"CoarsenOperator {'x'.join(map(str, fine_dims))} -> "
-                       f"{'x'.join(map(str, cd))} nbasis={N} fp64 (backward points + self)",
-           "ms": t * 1e3, "flops": flops, "TFLOPs": flops / t / 1e12,
-           "fp64_frac": flops / t / 1e12 / PEAK_TFLOPS["f64"],
-           # block orthonormalisation of the same basis, two passes: each pass
-           # reads and writes the panel once and takes nb^2 * |B| * 6 * 8 flops
-           # per (block, chirality) for the projections
-           "orthogonalise_2pass_ms": t_orth * 1e3,
-           "orthogonalise_GBps": 2 * 2 * nb * Vf * 12 * 16 / t_orth / 1e9}
+                       f"{'x'.join(map(str, cd))} nbasis={N} fp64 (backward points + self), "
+                       f"basis from 8 inverse iterations",
+           "basis_setup_s": t_setup,
+           "flops": flops,
+           "ms": build["mfma"]["ms"], "TFLOPs": build["mfma"]["TFLOPs"],
+           "fp64_frac": build["mfma"]["fp64_frac"],
+           "build": build,
+           "orthogonalise": orth,
+           "orthogonalise_panel_bytes": panel_bytes}
+    mg.close()
     for b in (g, basis):
         b.close()
     cop.close()
@@ -299,7 +333,10 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10):
     """Three-level K-cycle multigrid solve of the Wilson-clover system
     (m = -0.1, csw = 1.0, random SU(3) field eps = 0.4) on one Gaussian rhs:
     FGMRES(20) outer in FP64 with true-residual recomputation, hierarchy in
-    FP64 ("plain form") and in FP32 ("mixed").  Setup and solve timed apart."""
+    FP64 ("plain form") and in FP32 ("mixed"), the reference's default setup
+    (BiCGSTAB 1e-3 / 100 test vectors).  "mixed_fp32_graph" runs the K-cycle
+    as a captured HIP graph of device-scalar GMRES with fixed step counts.
+    Setup and solve timed apart."""
     V = int(np.prod(dims))
     g = mgc.DeviceBuffer(4 * V * 9 * 16)
     mgc.random_gauge(g, dims, 0.4, SEED, 21, stream)
@@ -315,8 +352,10 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10):
     out = {"workload": "Wilson-clover " + " -> ".join("x".join(map(str, d)) for d in lat)
                        + f", nbasis {'/'.join(str(2 * n) for n in nbs)}, K-cycle, FGMRES(20) "
                          f"to {tol:g}"}
-    for name, prec in (("fp64_plain", mgc.F64), ("mixed_fp32", mgc.F32)):
-        p = mgc.mg_params(nlevels=len(blocks) + 1, block=blocks, nb=nbs, precision=prec)
+    for name, prec, graph in (("fp64_plain", mgc.F64, 0), ("mixed_fp32", mgc.F32, 0),
+                              ("mixed_fp32_graph", mgc.F32, 2)):
+        p = mgc.mg_params(nlevels=len(blocks) + 1, block=blocks, nb=nbs, precision=prec,
+                          graph=graph)
         mg = mgc.Multigrid(fop, p)
         t0 = time.perf_counter()
         mg.setup(stream)
@@ -385,35 +424,69 @@ def mg_batched_leg(mgc, stream, dims, blocks, nbs, nrhs, coarse_prec, tol=1e-10)
     }
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_protocol(fn, ncal=10, min_sec=5.0, max_calls=1000):
+    """include/Benchmark_helpers.h:36-59 with the defaults of
+    src/Benchmark_coarse_operator.cc:60-61: ncal calibration calls, then
+    min(max_calls, min_sec * ncal / t_cal + 1) timed calls; seconds per call."""
+    t0 = time.perf_counter()
+    for _ in range(ncal):
+        fn()
+    tcal = time.perf_counter() - t0
+    n = int(min(max_calls, min_sec * ncal / tcal + 1))
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fn()
+    return (time.perf_counter() - t0) / n, n
+
+
 def cpu_baseline(dims, N, nrhs, head):
     """FP64 oracle multi-rhs coarse M (one OpenMP-parallel site loop, dense
     N x N matvec per point and rhs) on the host cores, on the headline's own
-    links and FP32 inputs, timed with the reference's protocol
-    (include/Benchmark_helpers.h:36-59: calibration calls, then
-    min(1000, nSecMin*ncal/tcal + 1) calls) bounded to ~10 s.  The oracle's
-    result also checks the headline's output (per-rhs relative L2 error)."""
+    links and FP32 inputs, timed with the reference's protocol (10 calibration
+    calls, then enough calls for 5 s).  The same per-site work is timed on one
+    thread on a 4^4 sample of the lattice (rhs·sites/s is an intensive rate).
+    The oracle's result also checks the headline's output (per-rhs relative
+    L2 error)."""
     import oracle
     V = int(np.prod(dims))
     links, v = head["links"], head["in"]
-    ncal = 2
-    t0 = time.perf_counter()
-    for _ in range(ncal):
-        ref = oracle.coarse_mrhs_M(dims, N, nrhs, links, v)
-    tcal = time.perf_counter() - t0
-    n = int(min(1000, 8.0 * ncal / tcal + 1))
-    t0 = time.perf_counter()
-    for _ in range(n):
-        oracle.coarse_mrhs_M(dims, N, nrhs, links, v)
-    t = (time.perf_counter() - t0) / n
+    ref = oracle.coarse_mrhs_M(dims, N, nrhs, links, v)
     errs = [oracle.relative_error(ref[:, k], head["out"][:, k]) for k in range(nrhs)]
+    # the host threads this process may use (the OpenMP setting where given)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    oracle.set_num_threads(threads)
+    t_all, n_all = time_protocol(lambda: oracle.coarse_mrhs_M(dims, N, nrhs, links, v))
+    # one thread: the same operator on a 4^4 sample (its links and inputs)
+    sd = (4, 4, 4, 4)
+    Vs = int(np.prod(sd))
+    sl = np.ascontiguousarray(links.reshape(9, V, N, N)[:, :Vs])
+    sv = np.ascontiguousarray(v.reshape(V, nrhs, N)[:Vs])
+    oracle.set_num_threads(1)
+    t_one, n_one = time_protocol(lambda: oracle.coarse_mrhs_M(sd, N, nrhs, sl, sv))
+    oracle.set_num_threads(threads)
     return {
-        "value": V * nrhs / t,
+        "value": V * nrhs / t_all,
         "unit": "rhs·sites/s",
-        "cores": oracle.num_threads(),
+        "cores": threads,
         "kind": "port",
+        "cpu_model": cpu_model(),
         "sample": f"FP64 oracle multi-rhs coarse M, {'x'.join(map(str, dims))} nbasis={N} "
-                  f"nrhs={nrhs}: {n} timed calls after {ncal} calibration calls "
-                  f"({n * t:.1f} s)",
+                  f"nrhs={nrhs}, {threads} OpenMP threads: {n_all} timed calls after 10 "
+                  f"calibration calls ({n_all * t_all:.1f} s)",
+        "one_core": {"value": Vs * nrhs / t_one, "unit": "rhs·sites/s", "cores": 1,
+                     "sample": f"the same operator on a 4^4 part of the lattice, one thread: "
+                               f"{n_one} timed calls after 10 calibration calls "
+                               f"({n_one * t_one:.1f} s)"},
         "headline_max_rel_err_vs_oracle": max(errs),
     }
 

@@ -416,6 +416,37 @@ __global__ void k_vmultiaxpy(const cplx<double>* c, const cplx<R>* V, size_t ldv
   }
 }
 
+// y = x * a with a = d_s[0] (scale_mode 0), 1 / sqrt(d_s[0]) (1) or
+// sqrt(d_s[0]) (2), 0 where the scalar is not positive; x and y given
+// directly or through device pointer slots (px, py non-null)
+template <typename Rs, typename Rd>
+__global__ void k_vconvert_dev(const cplx<Rs>* x, cplx<Rd>* y, const cplx<Rs>* const* px,
+                               cplx<Rd>* const* py, size_t n, const double* d_s, int mode) {
+  const double v = d_s[0];
+  const double a = mode == 0 ? v : !(v > 0.0) ? 0.0 : mode == 1 ? 1.0 / sqrt(v) : sqrt(v);
+  if (px) x = *px;
+  if (py) y = *py;
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) y[i] = {(Rd)(x[i].re * a), (Rd)(x[i].im * a)};
+}
+
+// |x|^2 into res[0] (res[1] = 0) with x read through a device pointer slot
+template <typename R>
+__global__ __launch_bounds__(256) void k_norm_part_ind(const cplx<R>* const* px, size_t n,
+                                                       double* part) {
+  const cplx<R>* x = *px;
+  double re = 0.0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    re += (double)x[i].re * x[i].re + (double)x[i].im * x[i].im;
+  const double2 t = block_sum2(re, 0.0);
+  if (threadIdx.x == 0) {
+    part[(size_t)blockIdx.x * 2] = t.x;
+    part[(size_t)blockIdx.x * 2 + 1] = t.y;
+  }
+}
+
 template <typename R> void vzero(cplx<R>* x, size_t n, hipStream_t s) {
   k_vzero<R><<<ew_grid(n), 256, 0, s>>>(x, n);
   MGC_LAUNCH_CHECK();
@@ -562,6 +593,23 @@ template <typename R> bool vmultisub_dot_dev(Blas& b, const cplx<double>* d_c, c
   return true;
 }
 
+template <typename Rs, typename Rd>
+void vconvert_dev(const cplx<Rs>* x, cplx<Rd>* y, const cplx<Rs>* const* px, cplx<Rd>* const* py,
+                  size_t n, const double* d_s, int mode, hipStream_t s) {
+  k_vconvert_dev<Rs, Rd><<<ew_grid(n), 256, 0, s>>>(x, y, px, py, n, d_s, mode);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void vnorm2_dev(Blas& b, const cplx<R>* x, const cplx<R>* const* px, size_t n, double* d_res) {
+  const int np = nparts_for(n);
+  if (px) k_norm_part_ind<R><<<np, 256, 0, b.s>>>(px, n, b.d_part);
+  else k_dot_part<R, true><<<dim3(np, 1), 256, 0, b.s>>>(x, 0, x, n, b.d_part);
+  MGC_LAUNCH_CHECK();
+  k_dot_final<<<1, 256, 0, b.s>>>(b.d_part, np, d_res);
+  MGC_LAUNCH_CHECK();
+}
+
 void fetch(Blas& b, const double* d_src, int count) {
   MGC_CHECK(count >= 0 && count <= 4 * (Blas::KMAX + 1) * 2, MGC_ERR_ARG, "fetch too large");
   MGC_HIP(hipMemcpyAsync(b.h_res, d_src, (size_t)count * sizeof(double), hipMemcpyDeviceToHost,
@@ -595,6 +643,18 @@ void fetch(Blas& b, const double* d_src, int count) {
 INST(double)
 INST(float)
 #undef INST
+template void vnorm2_dev<double>(Blas&, const cplx<double>*, const cplx<double>* const*, size_t,
+                                 double*);
+template void vnorm2_dev<float>(Blas&, const cplx<float>*, const cplx<float>* const*, size_t,
+                                double*);
+#define INSTC(Rs, Rd)                                                                           \
+  template void vconvert_dev<Rs, Rd>(const cplx<Rs>*, cplx<Rd>*, const cplx<Rs>* const*,       \
+                                     cplx<Rd>* const*, size_t, const double*, int, hipStream_t);
+INSTC(double, float)
+INSTC(float, double)
+INSTC(double, double)
+INSTC(float, float)
+#undef INSTC
 template void vconvert<double, float>(const cplx<double>*, cplx<float>*, size_t, double,
                                       hipStream_t);
 template void vconvert<float, double>(const cplx<float>*, cplx<double>*, size_t, double,

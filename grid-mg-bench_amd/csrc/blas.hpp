@@ -73,6 +73,16 @@ template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, si
 template <typename R> bool vmultisub_dot_dev(Blas& b, const cplx<double>* d_c, const cplx<R>* V,
                                              size_t ldv, int k, cplx<R>* w, size_t n,
                                              cplx<double>* d_h);
+// y = x * a with a from the device scalar d_s[0]: mode 0 a = d_s[0], mode 1
+// a = 1/sqrt(d_s[0]), mode 2 a = sqrt(d_s[0]) (a = 0 where d_s[0] <= 0 in modes
+// 1, 2); px / py, when not null, are device slots holding the vector
+// pointers (read by the kernel, so a captured graph can take new vectors)
+template <typename Rs, typename Rd>
+void vconvert_dev(const cplx<Rs>* x, cplx<Rd>* y, const cplx<Rs>* const* px, cplx<Rd>* const* py,
+                  size_t n, const double* d_s, int mode, hipStream_t s);
+// d_res[0] = |x|^2 (d_res[1] = 0) in device memory, x direct or through the slot px
+template <typename R>
+void vnorm2_dev(Blas& b, const cplx<R>* x, const cplx<R>* const* px, size_t n, double* d_res);
 // copy count doubles from device to the pinned h_res and wait for them
 void fetch(Blas& b, const double* d_src, int count);
 

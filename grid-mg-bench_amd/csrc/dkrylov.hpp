@@ -1,0 +1,166 @@
+// Device-scalar (F)GMRES for the K-cycle in graph mode: a fixed number m of
+// Arnoldi steps from one starting residual, with every scalar (norms,
+// Hessenberg column, complex Givens rotations, the triangular solve) kept in
+// device memory and updated by single-workgroup kernels.  It makes no host
+// synchronisation and no allocation after construction, so a preconditioner
+// built from it can be captured once into a HIP graph and replayed.
+//
+// Arithmetic per step is that of the host-driven Gmres: classical
+// Gram-Schmidt with the second projection always taken (the host form takes
+// it only when the norm drops below 1/sqrt(2)), the new norm by Pythagoras.
+// A step whose new norm is not positive marks the Krylov space exhausted;
+// the solution then uses the columns before it.
+#pragma once
+#include <complex>
+
+#include "krylov.hpp"
+
+namespace mgc {
+
+// device state of one solve
+struct DGmresState {
+  cplx<double> H[(Blas::KMAX + 1) * Blas::KMAX];  // [i][j], row stride KMAX
+  cplx<double> sn[Blas::KMAX];
+  double cs[Blas::KMAX];
+  cplx<double> g[Blas::KMAX + 1];
+  cplx<double> negy[Blas::KMAX];  // -y, for x += sum y_i Z_i by the multi-subtract
+  double r2[2];                   // |r0|^2 (from vnorm2_dev)
+  int jmax;                       // Arnoldi columns in use
+};
+
+__global__ void k_dg_begin(DGmresState* st) {
+  if (threadIdx.x != 0) return;
+  for (int i = 0; i <= Blas::KMAX; ++i) st->g[i] = {0.0, 0.0};
+  const double b2 = st->r2[0];
+  st->g[0] = {b2 > 0.0 ? sqrt(b2) : 0.0, 0.0};
+  st->jmax = b2 > 0.0 ? Blas::KMAX : 0;  // no Krylov space for a zero residual
+}
+
+// column j of the Hessenberg matrix from the two projections (h1 at dh1[0..j],
+// h2 at dh2[0..j], the new norm at dh2[j+1].im), the earlier rotations, and
+// the rotation that zeroes H[j+1][j]
+__global__ void k_dg_step(DGmresState* st, const cplx<double>* dh1, const cplx<double>* dh2,
+                          int j) {
+  if (threadIdx.x != 0) return;
+  typedef std::complex<double> C;
+  if (j >= st->jmax) return;
+  const int ld = Blas::KMAX;
+  C col[Blas::KMAX + 1];
+  for (int i = 0; i <= j; ++i) col[i] = C(dh1[i].re + dh2[i].re, dh1[i].im + dh2[i].im);
+  const double hn = dh2[j + 1].im;
+  for (int i = 0; i < j; ++i) {
+    const C a = col[i], b = col[i + 1];
+    const C s(st->sn[i].re, st->sn[i].im);
+    col[i] = st->cs[i] * a + s * b;
+    col[i + 1] = -std::conj(s) * a + st->cs[i] * b;
+  }
+  const C a = col[j];
+  const double bb = hn > 0.0 ? hn : 0.0;
+  const double t = sqrt(std::norm(a) + bb * bb);
+  double c;
+  C s;
+  if (std::abs(a) == 0.0) {
+    c = 0.0;
+    s = 1.0;
+  } else {
+    c = std::abs(a) / t;
+    s = (a / std::abs(a)) * (bb / t);
+  }
+  col[j] = c * a + s * bb;
+  st->cs[j] = c;
+  st->sn[j] = {s.real(), s.imag()};
+  const C gj(st->g[j].re, st->g[j].im);
+  const C g1 = -std::conj(s) * gj, g0 = c * gj;
+  st->g[j] = {g0.real(), g0.imag()};
+  st->g[j + 1] = {g1.real(), g1.imag()};
+  for (int i = 0; i <= j; ++i) st->H[i * ld + j] = {col[i].real(), col[i].imag()};
+  if (!(hn > 0.0)) st->jmax = j + 1;  // the next vector is zero: stop using columns
+}
+
+// y = H^-1 g over the columns in use (m at most), stored negated
+__global__ void k_dg_solve(DGmresState* st, int m) {
+  if (threadIdx.x != 0) return;
+  typedef std::complex<double> C;
+  const int ld = Blas::KMAX;
+  const int jm = st->jmax < m ? st->jmax : m;
+  C y[Blas::KMAX];
+  for (int i = jm - 1; i >= 0; --i) {
+    C acc(st->g[i].re, st->g[i].im);
+    for (int k = i + 1; k < jm; ++k) acc -= C(st->H[i * ld + k].re, st->H[i * ld + k].im) * y[k];
+    const C d(st->H[i * ld + i].re, st->H[i * ld + i].im);
+    y[i] = std::abs(d) > 0.0 ? acc / d : C(0.0);
+  }
+  for (int i = 0; i < m; ++i) st->negy[i] = i < jm ? cplx<double>{-y[i].real(), -y[i].imag()}
+                                                  : cplx<double>{0.0, 0.0};
+}
+
+template <typename R>
+class DGmres : public Solver<R> {
+ public:
+  DGmres(size_t n, int m, bool flexible) : n_(n), m_(m), flexible_(flexible) {
+    MGC_CHECK(m >= 1 && m + 1 <= Blas::KMAX, MGC_ERR_ARG, "Arnoldi steps out of range");
+    V_.alloc((size_t)(m + 1) * n);
+    if (flexible) Z_.alloc((size_t)m * n);
+    w_.alloc(n);
+    dh_.alloc((size_t)2 * (Blas::KMAX + 2));
+    st_.alloc(1);
+  }
+
+  // m steps from x (or 0); tol and maxit are not used: the step count is fixed
+  KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>* P, const cplx<R>* b, cplx<R>* x, bool x_zero,
+                    double, int) override {
+    hipStream_t s = bl.s;
+    const size_t n = n_;
+    cplx<R>* V = V_.p;
+    KrylovStats st;
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, V, n, s);
+    } else {
+      A.apply(x, w_.p, s);
+      ++st.applies;
+      vsub<R>(b, w_.p, V, n, s);
+    }
+    vnorm2_dev<R>(bl, V, nullptr, n, st_.p->r2);
+    k_dg_begin<<<1, 64, 0, s>>>(st_.p);
+    MGC_LAUNCH_CHECK();
+    vconvert_dev<R, R>(V, V, nullptr, nullptr, n, st_.p->r2, 1, s);  // v0 = r / |r|
+    cplx<double>* dh1 = dh_.p;
+    for (int j = 0; j < m_; ++j) {
+      cplx<R>* vj = V + (size_t)j * n;
+      const cplx<R>* z = vj;
+      if (flexible_) {
+        cplx<R>* zj = Z_.p + (size_t)j * n;
+        if (P) P->apply(vj, zj, s);
+        else vcopy<R>(vj, zj, n, s);
+        z = zj;
+      }
+      A.apply(z, w_.p, s);
+      ++st.applies;
+      cplx<double>* dh2 = dh_.p + (j + 2);
+      vmultidot_dev<R>(bl, V, n, j + 1, w_.p, n, true, dh1);
+      if (!vmultisub_dot_dev<R>(bl, dh1, V, n, j + 1, w_.p, n, dh2)) {
+        vmultisub_dev<R>(dh1, V, n, j + 1, w_.p, n, s);
+        vmultidot_dev<R>(bl, V, n, j + 1, w_.p, n, true, dh2);
+      }
+      vsubnorm_dev<R>(dh2, V, n, j + 1, w_.p, V + (size_t)(j + 1) * n, n, s);
+      k_dg_step<<<1, 64, 0, s>>>(st_.p, dh1, dh2, j);
+      MGC_LAUNCH_CHECK();
+      ++st.iters;
+    }
+    k_dg_solve<<<1, 64, 0, s>>>(st_.p, m_);
+    MGC_LAUNCH_CHECK();
+    vmultisub_dev<R>(st_.p->negy, flexible_ ? Z_.p : V, n, m_, x, n, s);  // x += sum y_i Z_i
+    return st;
+  }
+
+ private:
+  size_t n_;
+  int m_;
+  bool flexible_;
+  DevBuf<cplx<R>> V_, Z_, w_;
+  DevBuf<cplx<double>> dh_;
+  DevBuf<DGmresState> st_;
+};
+
+}  // namespace mgc

@@ -30,6 +30,7 @@
 #include "bkrylov.hpp"
 #include "coarse_op.hpp"
 #include "fine_op.hpp"
+#include "dkrylov.hpp"
 #include "krylov.hpp"
 
 namespace mgc {
@@ -121,7 +122,7 @@ struct Level {
   mgc_coarse_op* cop64 = nullptr;      // next level's operator, FP64 (Galerkin build)
   mgc_coarse_op* copR = nullptr;       // next level's operator in R (== cop64 for FP64)
   std::unique_ptr<Solver<R>> smoother;  // smoother / coarsest solver at this level
-  std::unique_ptr<Gmres<R>> wrapper;    // K-cycle FGMRES on the next level
+  std::unique_ptr<Solver<R>> wrapper;   // K-cycle FGMRES on the next level
   DevBuf<cplx<R>> tv;                   // test vectors [nb][n], kept for refinement
   cplx<R>* rc = nullptr;               // next-level residual / correction
   cplx<R>* ec = nullptr;
@@ -201,12 +202,30 @@ struct MG : MGBase {
   };
   std::vector<std::unique_ptr<LevelPrec>> precs;
 
+  // graph mode (p.graph != 0): device slots holding the current in / out
+  // pointers, |in|^2, the pinned staging of the slots, and the captured graph
+  DevBuf<void*> slots;
+  DevBuf<double> d_nin;
+  void** h_slots = nullptr;
+  hipEvent_t slots_done = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
   MG(FineSrc* f, const mgc_mg_params& pp, hipStream_t s) : fsrc(f), p(pp), bl(s) {}
   ~MG() override {
     free_batch();
+    free_graph();
     L.clear();
     if (in_r) (void)hipFree(in_r);
     if (out_r) (void)hipFree(out_r);
+    if (h_slots) (void)hipHostFree(h_slots);
+    if (slots_done) (void)hipEventDestroy(slots_done);
+  }
+  void free_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
   }
 
   // restrict, coarse solve (K-cycle wrapper or coarsest GMRES), prolongate,
@@ -360,10 +379,21 @@ struct MG : MGBase {
       Ef = 2 * nb;
       L.push_back(std::move(nx));
     }
-    // solvers: smoother (GMRES or MR), K-cycle FGMRES wrapper, coarsest solver
+    // solvers: smoother (GMRES or MR), K-cycle FGMRES wrapper, coarsest
+    // solver; in graph mode device-scalar GMRES / FGMRES with fixed step
+    // counts (smoother_iters, wrapper_restart, coarsest_restart)
+    free_graph();
     for (int l = 0; l < p.nlevels; ++l) {
       Level<R>& lv = *L[l];
-      if (l + 1 < p.nlevels) {
+      if (p.graph) {
+        if (l + 1 < p.nlevels) {
+          lv.smoother = std::make_unique<DGmres<R>>(lv.n, p.smoother_iters, false);
+          if (l + 2 < p.nlevels)
+            lv.wrapper = std::make_unique<DGmres<R>>(L[l + 1]->n, p.wrapper_restart, true);
+        } else {
+          lv.smoother = std::make_unique<DGmres<R>>(lv.n, p.coarsest_restart, false);
+        }
+      } else if (l + 1 < p.nlevels) {
         lv.smoother = make_solver<R>(p.smoother_solver, lv.n, p.smoother_restart);
         if (l + 2 < p.nlevels)
           lv.wrapper = std::make_unique<Gmres<R>>(L[l + 1]->n, p.wrapper_restart, true);
@@ -413,8 +443,65 @@ struct MG : MGBase {
     MGC_HIP(hipStreamSynchronize(s));
   }
 
+  // the preconditioner on the vectors named by the device slots, with no host
+  // synchronisation: |in|^2 on the device, in / |in| rounded to R, the
+  // K-cycle, the result scaled back by |in|
+  void precondition_dev(hipStream_t s) {
+    const size_t n = L[0]->n;
+    auto pin = reinterpret_cast<const cplx<double>* const*>(slots.p);
+    auto pout = reinterpret_cast<cplx<double>* const*>(slots.p + 1);
+    vnorm2_dev<double>(bl, nullptr, pin, n, d_nin.p);
+    vconvert_dev<double, R>(nullptr, in_r, pin, nullptr, n, d_nin.p, 1, s);
+    kcycle(0, in_r, out_r, s);
+    vconvert_dev<R, double>(out_r, nullptr, nullptr, pout, n, d_nin.p, 2, s);
+  }
+
+  // graph mode: 1 = the device-scalar K-cycle issued eagerly, 2 = the same
+  // captured into a HIP graph at the first call (after one eager application
+  // that also makes every first-use allocation) and replayed from then on
+  void precondition_graph(const cplx<double>* in, cplx<double>* out, hipStream_t s) {
+    MGC_CHECK(s != nullptr, MGC_ERR_ARG,
+              "graph mode needs a created stream (the null stream cannot be captured)");
+    if (!slots.p) {
+      slots.alloc(2);
+      d_nin.alloc(2);
+      MGC_HIP(hipHostMalloc((void**)&h_slots, 2 * sizeof(void*), hipHostMallocDefault));
+      MGC_HIP(hipEventCreateWithFlags(&slots_done, hipEventDisableTiming));
+      MGC_HIP(hipEventRecord(slots_done, s));
+    }
+    MGC_HIP(hipEventSynchronize(slots_done));  // the last upload has been read
+    h_slots[0] = const_cast<cplx<double>*>(in);
+    h_slots[1] = out;
+    MGC_HIP(hipMemcpyAsync(slots.p, h_slots, 2 * sizeof(void*), hipMemcpyHostToDevice, s));
+    MGC_HIP(hipEventRecord(slots_done, s));
+    if (p.graph == 1) {
+      precondition_dev(s);
+      return;
+    }
+    if (!gexec) {
+      precondition_dev(s);  // first-use allocations happen outside the capture
+      MGC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        precondition_dev(s);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      MGC_HIP(hipStreamEndCapture(s, &graph));
+      MGC_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    }
+    MGC_HIP(hipGraphLaunch(gexec, s));
+  }
+
   void precondition64(const cplx<double>* in, cplx<double>* out, hipStream_t s) override {
     MGC_CHECK(ready, MGC_ERR_STATE, "multigrid setup not run");
+    if (p.graph) {
+      precondition_graph(in, out, s);
+      ++precond_applies;
+      return;
+    }
     const size_t n = L[0]->n;
     // normalise the residual before rounding it to the hierarchy precision
     const double nin = std::sqrt(vnorm2<double>(bl, in, n));
@@ -647,6 +734,7 @@ int mgc_mg_params_default(mgc_mg_params* p) {
     p->setup_recursive_updates = 1;
     p->smoother_solver = MGC_SOLVER_GMRES;
     p->coarsest_solver = MGC_SOLVER_GMRES;
+    p->graph = 0;
     p->smoother_tol = 1e-3;
     p->smoother_iters = 4;
     p->smoother_restart = 4;
@@ -677,6 +765,10 @@ static void check_params(const mgc_mg_params* params) {
             MGC_ERR_ARG, "unknown basis-vector type");
   MGC_CHECK(params->setup_refine_iters >= 0 && params->setup_iters >= 0, MGC_ERR_ARG,
             "negative setup iteration count");
+  MGC_CHECK(params->graph >= 0 && params->graph <= 2, MGC_ERR_ARG, "graph must be 0, 1 or 2");
+  MGC_CHECK(!params->graph || (params->smoother_solver == MGC_SOLVER_GMRES &&
+                               params->coarsest_solver == MGC_SOLVER_GMRES),
+            MGC_ERR_ARG, "graph mode runs GMRES smoothers and a GMRES coarsest solver");
 }
 
 int mgc_mg_create_linop(mgc_mg** out, const mgc_fine_linop* lop, const mgc_mg_params* params) {

@@ -43,7 +43,8 @@ class MGParams(ctypes.Structure):
                 ("setup_restart", _c.c_int), ("setup_basis_type", _c.c_int),
                 ("setup_pre_orthonormalise", _c.c_int), ("setup_post_orthonormalise", _c.c_int),
                 ("setup_refine_iters", _c.c_int), ("setup_recursive_updates", _c.c_int),
-                ("smoother_solver", _c.c_int), ("coarsest_solver", _c.c_int)]
+                ("smoother_solver", _c.c_int), ("coarsest_solver", _c.c_int),
+                ("graph", _c.c_int)]
 
 
 LINOP_APPLY = ctypes.CFUNCTYPE(_c.c_int, _c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p,

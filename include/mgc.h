@@ -452,6 +452,13 @@ typedef struct mgc_mg_params {
   int setup_recursive_updates;   /* setupRecursiveUpdates (default 1) */
   int smoother_solver;     /* smootherSolverType: MGC_SOLVER_GMRES (default) or _MR */
   int coarsest_solver;     /* coarsestSolverType: MGC_SOLVER_GMRES (default), _BICGSTAB, _MR */
+  int graph;               /* K-cycle of mgc_mg_precondition / mgc_mg_solve (no counterpart in
+                              the reference): 0 (default) = host-driven Krylov solvers;
+                              1 = device-scalar GMRES with fixed step counts (smoother_iters
+                              smoother steps, wrapper_restart FGMRES steps, coarsest_restart
+                              coarsest steps) and no host synchronisation; 2 = the same
+                              captured once into a HIP graph and replayed (bit-equal to 1).
+                              1 and 2 need GMRES smoothers and a created (non-null) stream */
 } mgc_mg_params;
 
 typedef struct mgc_solve_stats {

@@ -39,6 +39,10 @@ int orc_num_threads(void) {
   return n;
 }
 
+/* threads of the following parallel regions (the CPU-baseline timing runs
+   the same work on all host threads and on one) */
+void orc_set_num_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+
 /* ---------------- counter-based Gaussian generator ---------------- */
 static uint64_t splitmix64(uint64_t x) {
   x += 0x9E3779B97F4A7C15ull;

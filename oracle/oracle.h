@@ -19,6 +19,7 @@ extern "C" {
 
 /* number of OpenMP threads the oracle's parallel loops use */
 int orc_num_threads(void);
+void orc_set_num_threads(int n);
 
 /* ---- counter-based Gaussian generator (same definition as libmgc) ---- */
 void orc_gaussian(double* out, size_t n_complex, uint64_t seed, uint64_t stream);

@@ -40,6 +40,10 @@ def num_threads() -> int:
     return int(lib().orc_num_threads())
 
 
+def set_num_threads(n: int) -> None:
+    lib().orc_set_num_threads(int(n))
+
+
 def volume(dims) -> int:
     return int(np.prod(dims))
 

@@ -424,6 +424,73 @@ def mg_batched_leg(mgc, stream, dims, blocks, nbs, nrhs, coarse_prec, tol=1e-10)
     }
 
 
+def coarse_batch_leg(mgc, stream, fines=((48, 48, 48, 48), (32, 32, 32, 32), (24, 24, 24, 24)),
+                     nrhs=12, mp=None, tol=1e-3):
+    """Coarse solves of three lattices in one lockstep batch
+    (mgc_coarse_solve_batched): the level-1 operators (4^4 blocks, nbasis 32)
+    of FP32 Wilson-clover hierarchies on the given fine lattices, nrhs
+    Gaussian rhs each, GMRES(20) to tol with the stencil in mp (FP16 MFMA by
+    default) and one multi-lattice launch per application; against the same
+    solves run one lattice at a time."""
+    mp = mgc.F16 if mp is None else mp
+    mgs, fops, gs, ops, bs, xs, coarse = [], [], [], [], [], [], []
+    for k, d in enumerate(fines):
+        V = int(np.prod(d))
+        g = mgc.DeviceBuffer(4 * V * 9 * 16)
+        mgc.random_gauge(g, d, 0.4, SEED, 41 + k, stream)
+        fop = mgc.FineOp(d, -0.1, 1.0)
+        fop.set_gauge(g, stream)
+        p = mgc.mg_params(nlevels=2, block=[(4, 4, 4, 4)], nb=[16], precision=mgc.F32)
+        mg = mgc.Multigrid(fop, p)
+        mg.setup(stream)
+        cd = tuple(a // 4 for a in d)
+        Vc = int(np.prod(cd))
+        op = mgc.CoarseOp.borrow(mg.coarse_op_handle(1), cd, 32, mgc.F64)
+        b = mgc.DeviceBuffer(Vc * nrhs * 32 * 8)
+        mgc.random_gaussian(b, Vc * nrhs * 32, mgc.F32, SEED, 51 + k, stream)
+        xs.append(mgc.DeviceBuffer(Vc * nrhs * 32 * 8))
+        mgs.append(mg); fops.append(fop); gs.append(g); ops.append(op); bs.append(b)
+        coarse.append(cd)
+    stream.sync()
+    mgc.coarse_solve_batched(ops, bs, xs, nrhs, mrhs_prec=mp, tol=tol, maxit=300,
+                             stream=stream)  # warm-up (autotuner, workspaces)
+    t0 = time.perf_counter()
+    st = mgc.coarse_solve_batched(ops, bs, xs, nrhs, mrhs_prec=mp, tol=tol, maxit=300,
+                                  stream=stream)
+    t_batch = time.perf_counter() - t0
+    t_sep, st_sep = 0.0, []
+    for k in range(len(ops)):
+        mgc.coarse_solve_batched([ops[k]], [bs[k]], [xs[k]], nrhs, mrhs_prec=mp, tol=tol,
+                                 maxit=300, stream=stream)  # warm-up
+        t0 = time.perf_counter()
+        st_sep += mgc.coarse_solve_batched([ops[k]], [bs[k]], [xs[k]], nrhs, mrhs_prec=mp,
+                                           tol=tol, maxit=300, stream=stream)
+        t_sep += time.perf_counter() - t0
+    for lst in (xs, bs, gs):
+        for b in lst:
+            b.close()
+    for mg in mgs:
+        mg.close()
+    for fop in fops:
+        fop.close()
+    names = {mgc.F32: "f32", mgc.F16: "f16", mgc.BF16: "bf16"}
+    its = [s["iterations"] for s in st]
+    return {
+        "workload": "coarse GMRES(20) to %g on %s (level 1 of %s, nbasis 32), %d rhs each, "
+                    "%s MFMA stencil, all lattices in one lockstep batch" % (
+                        tol, " + ".join("x".join(map(str, c)) for c in coarse),
+                        " / ".join("x".join(map(str, d)) for d in fines), nrhs, names[mp]),
+        "batched_s": t_batch,
+        "separate_s": t_sep,
+        "speedup": t_sep / t_batch,
+        "iterations_per_lattice": [max(its[k * nrhs:(k + 1) * nrhs]) for k in range(len(ops))],
+        "separate_iterations_per_lattice": [max(s["iterations"] for s in st_sep[k * nrhs:(k + 1) * nrhs])
+                                            for k in range(len(ops))],
+        "all_converged": all(s["converged"] for s in st),
+        "true_residual_max_fp32_stencil": max(s["true_residual"] for s in st),
+    }
+
+
 def cpu_model() -> str:
     try:
         for line in open("/proc/cpuinfo"):
@@ -532,6 +599,10 @@ def main() -> int:
                     help="run only the batched multigrid-solve leg and print it (for profiling)")
     ap.add_argument("--only-aux", action="store_true",
                     help="run only the transfer, fine-operator and CoarsenOperator legs")
+    ap.add_argument("--only-coarse-batch", action="store_true",
+                    help="only run the multi-lattice batched coarse-solve leg")
+    ap.add_argument("--only-coarsen", action="store_true",
+                    help="only run the CoarsenOperator / block-orthonormalisation leg")
     ap.add_argument("--only-mg", action="store_true",
                     help="run only the multigrid-solve leg and print its result (for profiling)")
     ap.add_argument("--mrhs-variant", type=int, default=-1,
@@ -575,6 +646,10 @@ def main() -> int:
     if args.only_mg_batch:
         print(json.dumps({"mg_batched": batch_leg()}), flush=True)
         return 0
+    if args.only_coarse_batch:
+        log("leg: batched coarse solves 12^4 + 8^4 + 6^4, 12 rhs each")
+        print(json.dumps({"coarse_batched": coarse_batch_leg(mgc, stream)}), flush=True)
+        return 0
     if args.only_mg:
         d = tuple(int(v) for v in args.mg_dims.split(","))
         log(f"leg: multigrid solve {d}")
@@ -582,6 +657,11 @@ def main() -> int:
                                              [16, 16])}), flush=True)
         return 0
     legs = {}
+    if args.only_coarsen:
+        log("leg: CoarsenOperator 32^4 nbasis=64")
+        legs["coarsen_f64_n64"] = coarsen_leg(mgc, stream, (32, 32, 32, 32), (4, 4, 4, 4), 32)
+        print(json.dumps({"legs": legs}), flush=True)
+        return 0
     if args.only_aux:
         aux_legs(mgc, stream, legs, args)
         print(json.dumps({"legs": legs}), flush=True)
@@ -622,6 +702,8 @@ def main() -> int:
         mg = mg_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)], [16, 16])
         if args.mg_batch:
             mg["batched"] = batch_leg()
+        log("leg: batched coarse solves 12^4 + 8^4 + 6^4, 12 rhs each")
+        mg["coarse_batched"] = coarse_batch_leg(mgc, stream)
     ref64 = legs.get("M_1rhs_f64_n64")
     line = {
         "metric": METRIC,

@@ -536,11 +536,11 @@ __global__ __launch_bounds__(MGS_THREADS) void k_orthogonalise(cplx<double>* __r
 // norm (nearly dependent columns, kappa > ~1e7) is flagged, left untouched by
 // the apply, and orthonormalised by the column sweep instead.
 // T1 and T2 are nb x nb complex per panel in a workspace of the handle; the
-// panel is streamed in chunks of CQ_CH rows through LDS.
+// panel is streamed in chunks of CQ_CH rows through LDS, and the products run
+// on FP64 MFMA (below).
 // ---------------------------------------------------------------------------
 // 1: always the column sweep (mgc_aggregation_set_ortho_method)
 static int g_ortho_sweep_only = 0;
-constexpr int CQ_THREADS = 256;
 constexpr int CQ_CH = 32;         // panel rows per staged chunk
 constexpr int CQ_NBMAX = 32;      // widest panel of the Cholesky form
 constexpr double CHOLQR_PIVOT_FLOOR = 1e-14;  // on |pivot|^2 / G_jj
@@ -553,7 +553,8 @@ __device__ __forceinline__ size_t cq_row_off(int r, int nb, int Vb, int Ef, int 
 // Cholesky G = R^H R (upper R, real positive diagonal) of the nb x nb matrix
 // in sG (overwritten by R), and sX <- R^-1 (upper triangular).  Run by the
 // whole workgroup; the factorisation itself by wavefront 0.
-__device__ void cq_factor(cplx<double> (*sG)[CQ_NBMAX + 1], cplx<double> (*sX)[CQ_NBMAX + 1],
+template <int NS>
+__device__ void cq_factor(cplx<double> (*sG)[NS], cplx<double> (*sX)[NS],
                           int nb, int* flag) {
   const int t = threadIdx.x;
   __shared__ int s_bad;
@@ -599,124 +600,223 @@ __device__ void cq_factor(cplx<double> (*sG)[CQ_NBMAX + 1], cplx<double> (*sX)[C
   if (s_bad && t == 0) *flag = 1;
 }
 
-// One Gram round over every panel (blockIdx.x = 2 * coarse site + chirality),
-// writing R^-1 of the round to T[round].  Round 1 skips flagged panels and
-// forms the rows of V T1 before the Gram sum.
-__device__ __forceinline__ cplx<double> cq_row_times(const cplx<double>* v,
-                                                     cplx<double> (*T)[CQ_NBMAX + 1], int j) {
-  cplx<double> a = {0.0, 0.0};
-  for (int i = 0; i <= j; ++i) cmac(a, v[i], T[i][j]);
-  return a;
+// The Gram sums and the products V T run on FP64 MFMA (v_mfma_f64_16x16x4_f64)
+// over chunks of CQ_CH panel rows staged in LDS as interleaved doubles
+// (row m: Re v(m,0), Im v(m,0), Re v(m,1), ...), the columns zero-padded to
+// NP = 16 or 32.  A complex product is one real GEMM with the complex index
+// folded into K (K = 2i + t, t = 0: real, 1: imaginary part):
+//   X = V T:    A'[m][2i+t] = the stored row itself,
+//               B'[2i+t][2j+u] = (t,u) (0,0): Re T, (1,0): -Im T, (0,1): Im T, (1,1): Re T
+//   G = V^H V:  A'[c][2m+t] = t ? -Im v(m,c) : Re v(m,c),  B'[2m+t][2c'+u] as B' above with v
+// so the FP64 accumulator holds Re and Im of the result interleaved along the
+// columns, in the layout of a stored row.  Lane layout of the 16x16x4 f64
+// form: A[i = l%16][k = l/16], B[k = l/16][j = l%16], D[i = 4r + l/16][j = l%16].
+typedef double cq_dx4 __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct CqShape {
+  static constexpr int MT = NP / 16;            // 16-row tiles of the Gram
+  static constexpr int NTL = NP / 8;            // 16-column tiles (2 NP real columns)
+  static constexpr int WAVES = NP == 32 ? 4 : 2;
+  static constexpr int THREADS = 64 * WAVES;
+  static constexpr int GTPW = MT * NTL / WAVES;  // Gram tiles per wavefront
+  static constexpr int PER = CQ_CH * NP / THREADS;  // staged complex per thread and chunk
+  static constexpr int LDP = 2 * NP + 2;  // LDS row pitch in doubles (conflict-free A reads)
+  static constexpr int NS = NP + 1;       // row stride (complex) of the factorisation's matrices
+};
+
+// chunk rows r0 .. r0+CQ_CH-1 of panel (xc, k) into registers (zero past the
+// panel's rows and columns)
+template <int NP>
+__device__ __forceinline__ void cq_fetch(const cplx<double>* __restrict__ U, cplx<double>* reg,
+                                         int r0, int rows, int nb, int Vb, int Ef, int xc, int k) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < CqShape<NP>::PER; ++q) {
+    const int idx = t + q * CqShape<NP>::THREADS, r = idx / NP, i = idx % NP;
+    reg[q] = (r0 + r < rows && i < nb) ? U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i]
+                                       : cplx<double>{0.0, 0.0};
+  }
+}
+template <int NP>
+__device__ __forceinline__ void cq_stage(double* sV, const cplx<double>* reg) {
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int q = 0; q < CqShape<NP>::PER; ++q) {
+    const int idx = t + q * CqShape<NP>::THREADS, r = idx / NP, i = idx % NP;
+    sV[r * CqShape<NP>::LDP + 2 * i] = reg[q].re;
+    sV[r * CqShape<NP>::LDP + 2 * i + 1] = reg[q].im;
+  }
 }
 
-template <int TT>
-__global__ __launch_bounds__(CQ_THREADS) void k_cholqr_gram(const cplx<double>* __restrict__ U,
-                                                            cplx<double>* __restrict__ Tg,
-                                                            int* __restrict__ flags, int nb,
-                                                            int Vb, int Ef, int round) {
-  __shared__ cplx<double> sV[CQ_CH][CQ_NBMAX + 1];
-  __shared__ cplx<double> sW[CQ_CH][CQ_NBMAX + 1];
-  __shared__ cplx<double> sT[CQ_NBMAX][CQ_NBMAX + 1];
-  __shared__ cplx<double> sG[CQ_NBMAX][CQ_NBMAX + 1];
+// T (nb x nb complex, row-major, upper triangular) into LDS as interleaved
+// doubles, zero-padded to NP x NP
+template <int NP>
+__device__ __forceinline__ void cq_load_T(double* sT, const cplx<double>* __restrict__ T, int nb) {
+  for (int idx = threadIdx.x; idx < NP * NP; idx += CqShape<NP>::THREADS) {
+    const int i = idx / NP, j = idx % NP;
+    const cplx<double> v = (i < nb && j < nb) ? T[i * nb + j] : cplx<double>{0.0, 0.0};
+    sT[i * CqShape<NP>::LDP + 2 * j] = v.re;
+    sT[i * CqShape<NP>::LDP + 2 * j + 1] = v.im;
+  }
+}
+
+// dst = src T for the CQ_CH staged rows (T upper triangular: output column
+// tile nt needs K steps 0 .. 4 nt + 3 only; the skipped terms are exact
+// zeros).  Wavefront w computes row tile w / (WAVES / 2) and two column
+// tiles chosen so that every wavefront runs the same number of steps.
+template <int NP>
+__device__ __forceinline__ void cq_times_T(const double* src, const double* sT, double* dst) {
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lk = l >> 4, t = lk & 1;
+  int mt, nts[2];
+  if (NP == 32) {
+    mt = w >> 1;
+    nts[0] = (w & 1) ? 1 : 0;
+    nts[1] = (w & 1) ? 2 : 3;
+  } else {
+    mt = w;
+    nts[0] = 0;
+    nts[1] = 1;
+  }
+  const int m = mt * 16 + (l & 15);
+#pragma unroll
+  for (int q = 0; q < 2; ++q) {
+    const int nt = nts[q];
+    const int n = nt * 16 + (l & 15), cq = n >> 1, u = n & 1;
+    const double bsign = (t == 1 && u == 0) ? -1.0 : 1.0;
+    cq_dx4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int ksteps = 4 * (nt + 1);
+    for (int ks = 0; ks < ksteps; ++ks) {
+      const double a = src[m * CqShape<NP>::LDP + 4 * ks + lk];
+      const int i = 2 * ks + (lk >> 1);
+      const double b = bsign * sT[i * CqShape<NP>::LDP + 2 * cq + (t ^ u)];
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) dst[(mt * 16 + 4 * r + lk) * CqShape<NP>::LDP + n] = acc[r];
+  }
+}
+
+// acc += src^H src over the CQ_CH staged rows, for this wavefront's Gram tiles
+// (tiles wholly below the diagonal are skipped: the factorisation reads the
+// upper triangle only)
+template <int NP>
+__device__ __forceinline__ void cq_gram_acc(const double* src, cq_dx4* acc) {
+  typedef CqShape<NP> S;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lk = l >> 4, t = lk & 1;
+  const double asign = t ? -1.0 : 1.0;
+#pragma unroll
+  for (int q = 0; q < S::GTPW; ++q) {
+    const int tile = w * S::GTPW + q, mt = tile / S::NTL, nt = tile % S::NTL;
+    if (mt * 16 >= (nt + 1) * 8) continue;  // every c > every c' of the tile
+    const int c = mt * 16 + (l & 15);
+    const int n = nt * 16 + (l & 15), cq = n >> 1, u = n & 1;
+    const double bsign = (t == 1 && u == 0) ? -1.0 : 1.0;
+#pragma unroll 4
+    for (int ks = 0; ks < CQ_CH / 2; ++ks) {
+      const int m = 2 * ks + (lk >> 1);
+      const double a = asign * src[m * S::LDP + 2 * c + t];
+      const double b = bsign * src[m * S::LDP + 2 * cq + (t ^ u)];
+      acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[q], 0, 0, 0);
+    }
+  }
+}
+
+// One Gram round over every panel (blockIdx.x = 2 * coarse site + chirality),
+// writing R^-1 of the round to T[round].  Round 1 skips flagged panels and
+// forms the rows of V T1 (cq_times_T, the same code as the apply) before the
+// Gram sum.  The next chunk's loads are issued before the current chunk's
+// products.
+template <int NP>
+__global__ __launch_bounds__(CqShape<NP>::THREADS) void k_cholqr_gram(
+    const cplx<double>* __restrict__ U, cplx<double>* __restrict__ Tg, int* __restrict__ flags,
+    int nb, int Vb, int Ef, int round) {
+  typedef CqShape<NP> S;
+  static_assert(CQ_CH * S::LDP >= 2 * NP * S::NS, "factorisation matrices fit the staging");
+  __shared__ double sV[CQ_CH * S::LDP];  // staged rows; also R^-1 of the factorisation
+  __shared__ double sW[CQ_CH * S::LDP];  // V T1; also the Gram matrix
+  __shared__ double sT[NP * S::LDP];
   const int panel = blockIdx.x;
   const int xc = panel >> 1, k = panel & 1;
-  const int t = threadIdx.x;
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (round == 1 && flags[panel]) return;  // uniform over the workgroup
   const size_t tstride = (size_t)gridDim.x * nb * nb;  // T1 block, then T2 block
-  if (round == 1) {
-    const cplx<double>* T1 = Tg + (size_t)panel * nb * nb;
-    for (int idx = t; idx < nb * nb; idx += CQ_THREADS) sT[idx / nb][idx % nb] = T1[idx];
-  }
+  if (round == 1) cq_load_T<NP>(sT, Tg + (size_t)panel * nb * nb, nb);
   const int rows = Vb * (Ef / 2);
-  const int nt = (nb + TT - 1) / TT;  // thread tiles per dimension
-  const bool active = t < nt * nt;
-  const int i0 = (t / nt) * TT, j0 = (t % nt) * TT;
-  cplx<double> acc[TT][TT];
+  cq_dx4 acc[S::GTPW];
 #pragma unroll
-  for (int a = 0; a < TT; ++a)
-#pragma unroll
-    for (int b = 0; b < TT; ++b) acc[a][b] = {0.0, 0.0};
+  for (int q = 0; q < S::GTPW; ++q) acc[q] = {0.0, 0.0, 0.0, 0.0};
+  cplx<double> reg[S::PER];
+  cq_fetch<NP>(U, reg, 0, rows, nb, Vb, Ef, xc, k);
   for (int r0 = 0; r0 < rows; r0 += CQ_CH) {
-    const int nrow = min(CQ_CH, rows - r0);
     __syncthreads();  // previous chunk consumed
-    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
-      const int r = idx / nb, i = idx % nb;
-      sV[r][i] = U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i];
-    }
+    cq_stage<NP>(sV, reg);
     __syncthreads();
-    cplx<double>(*src)[CQ_NBMAX + 1] = sV;
+    if (r0 + CQ_CH < rows) cq_fetch<NP>(U, reg, r0 + CQ_CH, rows, nb, Vb, Ef, xc, k);
+    const double* src = sV;
     if (round == 1) {
-      // W = V T1 (T1 upper triangular)
-      for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
-        const int r = idx / nb, j = idx % nb;
-        sW[r][j] = cq_row_times(sV[r], sT, j);
-      }
+      cq_times_T<NP>(sV, sT, sW);
       __syncthreads();
       src = sW;
     }
-    if (active) {
-      for (int r = 0; r < nrow; ++r) {
-        cplx<double> x[TT], y[TT];
+    cq_gram_acc<NP>(src, acc);
+  }
+  __syncthreads();
+  auto sG = reinterpret_cast<cplx<double>(*)[S::NS]>(sW);
 #pragma unroll
-        for (int a = 0; a < TT; ++a) x[a] = i0 + a < nb ? src[r][i0 + a] : cplx<double>{0.0, 0.0};
+  for (int q = 0; q < S::GTPW; ++q) {
+    const int tile = w * S::GTPW + q, mt = tile / S::NTL, nt = tile % S::NTL;
+    const int n = nt * 16 + (l & 15), cq = n >> 1;
 #pragma unroll
-        for (int b = 0; b < TT; ++b) y[b] = j0 + b < nb ? src[r][j0 + b] : cplx<double>{0.0, 0.0};
-#pragma unroll
-        for (int a = 0; a < TT; ++a)
-#pragma unroll
-          for (int b = 0; b < TT; ++b) cmac_conj(acc[a][b], x[a], y[b]);
-      }
+    for (int r = 0; r < 4; ++r) {
+      const int c = mt * 16 + 4 * r + (l >> 4);
+      if (c < nb && cq < nb) reinterpret_cast<double*>(&sG[c][cq])[n & 1] = acc[q][r];
     }
   }
   __syncthreads();
-  if (active)
-#pragma unroll
-    for (int a = 0; a < TT; ++a)
-#pragma unroll
-      for (int b = 0; b < TT; ++b)
-        if (i0 + a < nb && j0 + b < nb) sG[i0 + a][j0 + b] = acc[a][b];
-  __syncthreads();
-  cq_factor(sG, sV, nb, &flags[panel]);
+  auto sX = reinterpret_cast<cplx<double>(*)[S::NS]>(sV);
+  cq_factor(sG, sX, nb, &flags[panel]);
   cplx<double>* Tp = Tg + round * tstride + (size_t)panel * nb * nb;
-  for (int idx = t; idx < nb * nb; idx += CQ_THREADS) Tp[idx] = sV[idx / nb][idx % nb];
+  for (int idx = threadIdx.x; idx < nb * nb; idx += S::THREADS) Tp[idx] = sX[idx / nb][idx % nb];
 }
 
 // V <- (V T1) T2 in place on every panel that is not flagged
-__global__ __launch_bounds__(CQ_THREADS) void k_cholqr_apply(cplx<double>* __restrict__ U,
-                                                             const cplx<double>* __restrict__ Tg,
-                                                             const int* __restrict__ flags,
-                                                             int nb, int Vb, int Ef) {
-  __shared__ cplx<double> sV[CQ_CH][CQ_NBMAX + 1];
-  __shared__ cplx<double> sW[CQ_CH][CQ_NBMAX + 1];
-  __shared__ cplx<double> sT[CQ_NBMAX][CQ_NBMAX + 1];
-  __shared__ cplx<double> sT2[CQ_NBMAX][CQ_NBMAX + 1];
+template <int NP>
+__global__ __launch_bounds__(CqShape<NP>::THREADS) void k_cholqr_apply(
+    cplx<double>* __restrict__ U, const cplx<double>* __restrict__ Tg,
+    const int* __restrict__ flags, int nb, int Vb, int Ef) {
+  typedef CqShape<NP> S;
+  __shared__ double sV[CQ_CH * S::LDP];
+  __shared__ double sW[CQ_CH * S::LDP];
+  __shared__ double sT[NP * S::LDP];
+  __shared__ double sT2[NP * S::LDP];
   const int panel = blockIdx.x;
   const int xc = panel >> 1, k = panel & 1;
   const int t = threadIdx.x;
   if (flags[panel]) return;
   const cplx<double>* T1 = Tg + (size_t)panel * nb * nb;
-  const cplx<double>* T2 = T1 + (size_t)gridDim.x * nb * nb;
-  for (int idx = t; idx < nb * nb; idx += CQ_THREADS) {
-    sT[idx / nb][idx % nb] = T1[idx];
-    sT2[idx / nb][idx % nb] = T2[idx];
-  }
+  cq_load_T<NP>(sT, T1, nb);
+  cq_load_T<NP>(sT2, T1 + (size_t)gridDim.x * nb * nb, nb);
   const int rows = Vb * (Ef / 2);
+  cplx<double> reg[S::PER];
+  cq_fetch<NP>(U, reg, 0, rows, nb, Vb, Ef, xc, k);
   for (int r0 = 0; r0 < rows; r0 += CQ_CH) {
-    const int nrow = min(CQ_CH, rows - r0);
+    __syncthreads();  // previous chunk stored
+    cq_stage<NP>(sV, reg);
     __syncthreads();
-    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
+    if (r0 + CQ_CH < rows) cq_fetch<NP>(U, reg, r0 + CQ_CH, rows, nb, Vb, Ef, xc, k);
+    cq_times_T<NP>(sV, sT, sW);
+    __syncthreads();
+    cq_times_T<NP>(sW, sT2, sV);
+    __syncthreads();
+    // the chunk's rows back in place, whole complex values per thread
+    for (int idx = t; idx < CQ_CH * nb; idx += S::THREADS) {
       const int r = idx / nb, i = idx % nb;
-      sV[r][i] = U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i];
-    }
-    __syncthreads();
-    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
-      const int r = idx / nb, j = idx % nb;
-      sW[r][j] = cq_row_times(sV[r], sT, j);
-    }
-    __syncthreads();
-    for (int idx = t; idx < nrow * nb; idx += CQ_THREADS) {
-      const int r = idx / nb, j = idx % nb;
-      U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + j] = cq_row_times(sW[r], sT2, j);
+      if (r0 + r < rows)
+        U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i] = {sV[r * CqShape<NP>::LDP + 2 * i],
+                                                        sV[r * CqShape<NP>::LDP + 2 * i + 1]};
     }
   }
 }
@@ -1001,14 +1101,22 @@ int mgc_aggregation_orthogonalise(mgc_aggregation* a, int passes, void* stream) 
       }
     }
     fill_zero(a->cq_flag, (size_t)panels * sizeof(int), s);
-    auto gram = a->nb > 16 ? k_cholqr_gram<2> : k_cholqr_gram<1>;
+    const bool wide = a->nb > 16;
     for (int round = 0; round < 2; ++round) {
-      gram<<<panels, CQ_THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag, a->nb, a->Vb, a->Ef,
-                                         round);
+      if (wide)
+        k_cholqr_gram<32><<<panels, CqShape<32>::THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag,
+                                                                 a->nb, a->Vb, a->Ef, round);
+      else
+        k_cholqr_gram<16><<<panels, CqShape<16>::THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag,
+                                                                 a->nb, a->Vb, a->Ef, round);
       MGC_LAUNCH_CHECK();
     }
-    k_cholqr_apply<<<panels, CQ_THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag, a->nb, a->Vb,
-                                                 a->Ef);
+    if (wide)
+      k_cholqr_apply<32><<<panels, CqShape<32>::THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag,
+                                                                a->nb, a->Vb, a->Ef);
+    else
+      k_cholqr_apply<16><<<panels, CqShape<16>::THREADS, 0, s>>>(a->basis64, a->cq_T, a->cq_flag,
+                                                                a->nb, a->Vb, a->Ef);
     MGC_LAUNCH_CHECK();
     // panels whose Cholesky broke down: the column sweep (most workgroups exit at once)
     k_orthogonalise<<<panels, MGS_THREADS, 0, s>>>(a->basis64, a->nb, a->Vb, a->Ef, passes,

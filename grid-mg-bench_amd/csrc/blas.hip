@@ -510,7 +510,10 @@ template <typename R> double vnorm2(Blas& b, const cplx<R>* x, size_t n) {
 }
 template <typename R> void vmultidot(Blas& b, const cplx<R>* V, size_t ldv, int k,
                                      const cplx<R>* w, size_t n, cplx<double>* h) {
-  reduce<R, false>(b, V, ldv, k, w, n);
+  MGC_CHECK(k >= 1 && k <= Blas::KMAX, MGC_ERR_ARG, "too many simultaneous reductions");
+  // the multi-row reduction reads w once per MD_ROWS rows
+  vmultidot_dev<R>(b, V, ldv, k, w, n, false, reinterpret_cast<cplx<double>*>(b.d_res));
+  fetch(b, b.d_res, 2 * k);
   for (int j = 0; j < k; ++j) h[j] = {b.h_res[2 * j], b.h_res[2 * j + 1]};
 }
 template <typename R> void vmultiaxpy(Blas& b, const cplx<double>* c, const cplx<R>* V, size_t ldv,

@@ -29,7 +29,7 @@
 
 namespace mgc {
 
-constexpr int CO_SB = 2;  // fine sites per stage
+constexpr int CO_SB = 4;  // fine sites per stage
 
 // Workgroup b -> (coarse site, point).  The points of one coarse site are
 // consecutive in the dispatch sequence of one XCD (b % 8 is the XCD), so the
@@ -66,7 +66,9 @@ struct CoGeom {
     mu = p & 3;
     disp = p < 4 ? 1 : -1;
     // sites of this point: outer face of the block for p < 8, the whole block for p = 8
-    nsite = (p == 8) ? Vb : Vb / blk.L[mu];
+    // (blk.L[mu] by a select: a runtime index would put the geometry in scratch memory)
+    const int Lmu = mu == 0 ? blk.L[0] : mu == 1 ? blk.L[1] : mu == 2 ? blk.L[2] : blk.L[3];
+    nsite = (p == 8) ? Vb : Vb / Lmu;
   }
   // face site f: the coordinates other than mu run lexicographically over the
   // face, coordinate mu is the block's last (+mu) or first (-mu) layer
@@ -89,17 +91,28 @@ struct CoGeom {
   }
 };
 
-// One W item: spins 0..3 at colour ci of W^LW(xb)[., cp], where W^l = H_p P_l u_cp
-// for a hop (p < 8) and D_BB P_l u_cp (inner hops plus the diagonal) for p = 8.
-// The body is instantiated per chirality LW so that every spin index is a
-// compile-time constant (no indexed private arrays).
-template <int NB, int LW>
-__device__ __forceinline__ void coarsen_w_item(const CoGeom& g, const GammaTab& G, int xb, int cp,
-                                               int ci, cplx<double> w[4]) {
+// One W item: W^lw(xb)[e, cp] for all 12 spin-colour elements, where
+// W^l = H_p P_l u_cp for a hop (p < 8) and D_BB P_l u_cp (inner hops plus the
+// diagonal) for p = 8.  The chirality lw is a runtime value, so that lanes
+// of both chiralities run one instruction stream: ws[sl*3 + ci] holds spin
+// 2 lw + sl (chirality lw), wo[k*3 + ci] spin 2 (1 - lw) + k, and the
+// spin-dependent constants are chosen by selects between compile-time values
+// (an index computed from lw would put the accumulators in scratch memory).
+// The hops are taken HB at a time: the loads of the batch (link, and the
+// neighbour's chirality-lw basis values) are issued before any is used.
+// One hop per batch (HB = 1) measured fastest: with two the loads take ~230
+// registers and one wavefront per SIMD; links staged in LDS for the
+// wavefront's site cost a barrier per stage and ran slower.
+template <int NB, int HB>
+__device__ __forceinline__ void coarsen_w_lane(const CoGeom& g, const GammaTab& G, int xb, int lw,
+                                               int cp, cplx<double> ws[6], cplx<double> wo[6]) {
   constexpr int ROW = 12 * NB;
   const int p = g.p;
 #pragma unroll
-  for (int sp = 0; sp < 4; ++sp) w[sp] = {0.0, 0.0};
+  for (int e = 0; e < 6; ++e) {
+    ws[e] = {0.0, 0.0};
+    wo[e] = {0.0, 0.0};
+  }
   // block and fine coordinates of this site
   int bc[4], x = 0;
   {
@@ -111,68 +124,103 @@ __device__ __forceinline__ void coarsen_w_item(const CoGeom& g, const GammaTab& 
       x += (g.cc[d] * g.blk.L[d] + bc[d]) * g.fs[d];
     }
   }
+  const int e0 = 6 * lw;  // first element of chirality lw
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    if (p != 8 && q != p) continue;
-    const int qm = q & 3;
-    const int disp = q < 4 ? 1 : -1;
-    const bool inner = (unsigned)(bc[qm] + disp) < (unsigned)g.blk.L[qm];
-    // self: inner hops only, by the reference's coordinate mask
-    // (x_mu mod b != b-1 for +mu, != 0 for -mu)
-    if (p == 8 && !inner) continue;
-    // fine neighbour y = x + disp*mu (periodic) and its panel row
-    const int fq = g.cc[qm] * g.blk.L[qm] + bc[qm] + disp;
-    const int y = x + disp * g.fs[qm] -
-                  (fq < 0 ? -g.fine.L[qm] * g.fs[qm] : fq >= g.fine.L[qm] ? g.fine.L[qm] * g.fs[qm] : 0);
-    int ry;
-    if (inner) {
-      ry = g.xc * g.Vb + xb + disp * g.bs[qm];
-    } else {
-      const int cq = g.cc[qm] + disp;
-      const int xn = g.xc + disp * g.cs[qm] -
-                     (cq < 0 ? -g.coarse.L[qm] * g.cs[qm]
-                             : cq >= g.coarse.L[qm] ? g.coarse.L[qm] * g.cs[qm] : 0);
-      ry = xn * g.Vb + xb - disp * (g.blk.L[qm] - 1) * g.bs[qm];
+  for (int q0 = 0; q0 < 8; q0 += HB) {
+    if (p != 8 && (p < q0 || p >= q0 + HB)) continue;  // uniform: a hop point uses one hop
+    cplx<double> L[HB][9], B[HB][6];  // link U (row-major), basis (sl, colour)
+    bool use[HB];
+#pragma unroll
+    for (int h = 0; h < HB; ++h) {
+      const int q = q0 + h;
+      const int qm = q & 3;
+      const int disp = q < 4 ? 1 : -1;
+      const bool inner = (unsigned)(bc[qm] + disp) < (unsigned)g.blk.L[qm];
+      // self: inner hops only, by the reference's coordinate mask
+      // (x_mu mod b != b-1 for +mu, != 0 for -mu)
+      use[h] = p == 8 ? inner : q == p;
+      if (!use[h]) continue;
+      // fine neighbour y = x + disp*mu (periodic) and its panel row
+      const int fq = g.cc[qm] * g.blk.L[qm] + bc[qm] + disp;
+      const int y = x + disp * g.fs[qm] -
+                    (fq < 0 ? -g.fine.L[qm] * g.fs[qm]
+                            : fq >= g.fine.L[qm] ? g.fine.L[qm] * g.fs[qm] : 0);
+      int ry;
+      if (inner) {
+        ry = g.xc * g.Vb + xb + disp * g.bs[qm];
+      } else {
+        const int cq = g.cc[qm] + disp;
+        const int xn = g.xc + disp * g.cs[qm] -
+                       (cq < 0 ? -g.coarse.L[qm] * g.cs[qm]
+                               : cq >= g.coarse.L[qm] ? g.coarse.L[qm] * g.cs[qm] : 0);
+        ry = xn * g.Vb + xb - disp * (g.blk.L[qm] - 1) * g.bs[qm];
+      }
+      const cplx<double>* U = g.gauge + ((size_t)qm * g.Vf + (q < 4 ? x : y)) * 9;
+#pragma unroll
+      for (int k = 0; k < 9; ++k) L[h][k] = U[k];
+      const cplx<double>* uy = g.basis + (size_t)ry * ROW + (size_t)e0 * NB + cp;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) B[h][k] = uy[k * NB];  // element e0 + k = 3 sl + j
     }
-    const double sg = q < 4 ? 1.0 : -1.0;
-    cplx<double> L[3];  // row ci of the link matrix
-    if (q < 4) {
-      const cplx<double>* U = g.gauge + ((size_t)qm * g.Vf + x) * 9;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) L[j] = U[ci * 3 + j];
-    } else {
-      const cplx<double>* U = g.gauge + ((size_t)qm * g.Vf + y) * 9;
+    for (int h = 0; h < HB; ++h) {
+      if (!use[h]) continue;
+      const int q = q0 + h;
+      const int qm = q & 3;
+      const double sg = q < 4 ? 1.0 : -1.0;
 #pragma unroll
-      for (int j = 0; j < 3; ++j) L[j] = {U[j * 3 + ci].re, -U[j * 3 + ci].im};
-    }
-    const cplx<double>* uy = g.basis + (size_t)ry * ROW;
+      for (int sl = 0; sl < 2; ++sl) {
+        // gamma image of spin 2 lw + sl: spin s2 of the other chirality,
+        // local index k = s2 - 2 (1 - lw), phase of s2
+        const int s2a = G.col[qm][sl], s2b = G.col[qm][2 + sl];  // lw = 0, lw = 1
+        const int ka = s2a - 2, kb = s2b;
+        const cplx<double> pa = {(double)G.re[qm][s2a], (double)G.im[qm][s2a]};
+        const cplx<double> pb = {(double)G.re[qm][s2b], (double)G.im[qm][s2b]};
+        const cplx<double> ph = lw ? pb : pa;
+        const int kimg = lw ? kb : ka;
 #pragma unroll
-    for (int sl = 0; sl < 2; ++sl) {
-      const int sp = 2 * LW + sl;
-      cplx<double> chi = {0.0, 0.0};
+        for (int ci = 0; ci < 3; ++ci) {
+          cplx<double> chi = {0.0, 0.0};
 #pragma unroll
-      for (int j = 0; j < 3; ++j) cmac(chi, L[j], uy[(sp * 3 + j) * NB + cp]);
-      // same-chirality part: -1/2 chi
-      w[sp].re -= 0.5 * chi.re;
-      w[sp].im -= 0.5 * chi.im;
-      // gamma image: spin s2 = col(sp), + sg/2 * phase(s2) * chi
-      const int s2 = G.col[qm][sp];
-      const cplx<double> ph = {(double)G.re[qm][s2], (double)G.im[qm][s2]};
-      const cplx<double> t = cmul(ph, chi);
-      w[s2].re += 0.5 * sg * t.re;
-      w[s2].im += 0.5 * sg * t.im;
+          for (int j = 0; j < 3; ++j) {
+            // row ci of U (q < 4) or of U^dag (q >= 4)
+            const cplx<double> l = q < 4 ? L[h][ci * 3 + j]
+                                         : cplx<double>{L[h][j * 3 + ci].re, -L[h][j * 3 + ci].im};
+            cmac(chi, l, B[h][sl * 3 + j]);
+          }
+          // same-chirality part: -1/2 chi
+          ws[sl * 3 + ci].re -= 0.5 * chi.re;
+          ws[sl * 3 + ci].im -= 0.5 * chi.im;
+          // gamma image: + sg/2 * phase(s2) * chi at spin s2
+          const cplx<double> t = cmul(ph, chi);
+          const cplx<double> v = {0.5 * sg * t.re, 0.5 * sg * t.im};
+          if (ka == kb) {  // the same local spin for both chiralities (compile time)
+            wo[ka * 3 + ci].re += v.re;
+            wo[ka * 3 + ci].im += v.im;
+          } else {
+            const double f0 = kimg == 0 ? 1.0 : 0.0;
+            wo[ci].re += f0 * v.re;
+            wo[ci].im += f0 * v.im;
+            wo[3 + ci].re += (1.0 - f0) * v.re;
+            wo[3 + ci].im += (1.0 - f0) * v.im;
+          }
+        }
+      }
     }
   }
   if (p == 8) {  // diagonal (mass + clover) term, block-diagonal in chirality
-    const cplx<double>* D = g.diag + ((size_t)x * 2 + LW) * 36;
-    const cplx<double>* ux = g.basis + ((size_t)g.xc * g.Vb + xb) * ROW;
+    const cplx<double>* D = g.diag + ((size_t)x * 2 + lw) * 36;
+    const cplx<double>* ux = g.basis + ((size_t)g.xc * g.Vb + xb) * ROW + (size_t)e0 * NB + cp;
+    cplx<double> v[6];
 #pragma unroll
-    for (int sl = 0; sl < 2; ++sl) {
+    for (int b = 0; b < 6; ++b) v[b] = ux[b * NB];
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
       cplx<double> a = {0.0, 0.0};
 #pragma unroll
-      for (int b = 0; b < 6; ++b) cmac(a, D[(sl * 3 + ci) * 6 + b], ux[(6 * LW + b) * NB + cp]);
-      w[2 * LW + sl].re += a.re;
-      w[2 * LW + sl].im += a.im;
+      for (int b = 0; b < 6; ++b) cmac(a, D[r * 6 + b], v[b]);
+      ws[r].re += a.re;
+      ws[r].im += a.im;
     }
   }
 }
@@ -219,21 +267,23 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
       const cplx<double> v = basis[((size_t)xc * Vb + xb) * ROW + r];
       (&sU[s][0][0])[r] = ok ? v : cplx<double>{0.0, 0.0};
     }
-    // ---- W items (s, l', c', colour i)
-    for (int it = tid; it < CO_SB * 2 * NB * 3; it += THREADS) {
-      const int s = it / (2 * NB * 3);
-      int r = it - s * 2 * NB * 3;
-      const int lw = r / (NB * 3);
-      r -= lw * NB * 3;
-      const int cp = r / 3, ci = r - cp * 3;
-      cplx<double> w[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    // ---- W items (s, l', c'), all 12 elements each
+    for (int it = tid; it < CO_SB * 2 * NB; it += THREADS) {
+      const int s = it / (2 * NB);
+      const int r = it - s * 2 * NB;
+      const int lw = r / NB, cp = r - lw * NB;
+      cplx<double> ws[6], wo[6];
       if (s0 + s < g.nsite) {
-        const int xb = g.face_site(s0 + s);
-        if (lw == 0) coarsen_w_item<NB, 0>(g, G, xb, cp, ci, w);
-        else coarsen_w_item<NB, 1>(g, G, xb, cp, ci, w);
+        coarsen_w_lane<NB, 1>(g, G, g.face_site(s0 + s), lw, cp, ws, wo);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 6; ++e) ws[e] = wo[e] = {0.0, 0.0};
       }
 #pragma unroll
-      for (int sp = 0; sp < 4; ++sp) sW[s][lw][sp * 3 + ci][cp] = w[sp];
+      for (int e = 0; e < 6; ++e) {
+        sW[s][lw][6 * lw + e][cp] = ws[e];
+        sW[s][lw][6 - 6 * lw + e][cp] = wo[e];
+      }
     }
     __syncthreads();
     // ---- accumulate the (k, l) block: acc[c][c'] += sum_{e in k} conj(u(e,c)) W^l(e,c')
@@ -278,7 +328,7 @@ __global__ __launch_bounds__(4 * (NB / TC) * (NB / TC)) void k_coarsen(
 // MFMA 16x16x4 f64 lane layout: A[i = l%16][k = l/16], B[k = l/16][j = l%16],
 // D[i = 4r + l/16][j = l%16] for result r < 4 (rows interleaved, unlike the
 // FP32 16x16x4 form's 4(l/16) + r).
-constexpr int CM_SB = 2;  // fine sites per stage
+constexpr int CM_SB = 4;  // fine sites per stage
 typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 template <int NB>
@@ -290,6 +340,7 @@ __global__ __launch_bounds__(256) void k_coarsen_mfma(
   constexpr int NTL = (NB + 7) / 8;    // n-tiles (columns 2c'+v)
   constexpr int NBP = NB + 1;          // LDS row pitch (doubles) against bank conflicts
   constexpr int ROW = 12 * NB;
+  static_assert(CM_SB == 4 && NB <= 32, "one stage site per wavefront, 2 NB lanes per site");
   __shared__ double sUr[CM_SB][12][NBP], sUi[CM_SB][12][NBP];      // u, with -Im u
   __shared__ double sWr[CM_SB][2][12][NBP], sWi[CM_SB][2][12][NBP];  // W
   const GammaTab G = gamma_table();
@@ -327,28 +378,32 @@ __global__ __launch_bounds__(256) void k_coarsen_mfma(
       sUr[s][e][c] = ok ? u.re : 0.0;
       sUi[s][e][c] = ok ? -u.im : 0.0;
     }
-    // ---- W items (s, l', c', colour i)
-    for (int it = tid; it < CM_SB * 2 * NB * 3; it += 256) {
-      const int s = it / (2 * NB * 3);
-      int r = it - s * 2 * NB * 3;
-      const int lw = r / (NB * 3);
-      r -= lw * NB * 3;
-      const int cp = r / 3, ci = r - cp * 3;
-      cplx<double> wv[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
-      if (s0 + s < g.nsite) {
-        const int xb = g.face_site(s0 + s);
-        if (lw == 0) coarsen_w_item<NB, 0>(g, G, xb, cp, ci, wv);
-        else coarsen_w_item<NB, 1>(g, G, xb, cp, ci, wv);
-      }
+    // ---- W: wavefront w builds site w of the stage, lane l = (lw, c') with
+    // lw = l / NB (both chiralities in one instruction stream; the u staging
+    // above is not read here, so no barrier separates the two)
+    {
+      const int s = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const int lw = l / NB, cp = l - lw * NB;
+      if (l < 2 * NB) {
+        cplx<double> ws[6], wo[6];
+        if (s0 + s < g.nsite) {
+          coarsen_w_lane<NB, 1>(g, G, g.face_site(s0 + s), lw, cp, ws, wo);
+        } else {
 #pragma unroll
-      for (int sp = 0; sp < 4; ++sp) {
-        sWr[s][lw][sp * 3 + ci][cp] = wv[sp].re;
-        sWi[s][lw][sp * 3 + ci][cp] = wv[sp].im;
+          for (int e = 0; e < 6; ++e) ws[e] = wo[e] = {0.0, 0.0};
+        }
+#pragma unroll
+        for (int e = 0; e < 6; ++e) {
+          sWr[s][lw][6 * lw + e][cp] = ws[e].re;
+          sWi[s][lw][6 * lw + e][cp] = ws[e].im;
+          sWr[s][lw][6 - 6 * lw + e][cp] = wo[e].re;
+          sWi[s][lw][6 - 6 * lw + e][cp] = wo[e].im;
+        }
       }
     }
     __syncthreads();
     // ---- the block's GEMM: per site three k-steps of two elements each
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < CM_SB; ++s)
 #pragma unroll
       for (int q = 0; q < 3; ++q) {

@@ -634,6 +634,14 @@ struct MG : MGBase {
     for (int l = 0; l < p.nlevels; ++l) bprecs.push_back(std::make_unique<BLevelPrec>(this, l));
     MGC_HIP(hipMalloc(&bin, BL[0]->op->sh.n * sizeof(cplx<R>)));
     MGC_HIP(hipMalloc(&bout, BL[0]->op->sh.n * sizeof(cplx<R>)));
+    // one apply of every coarse level on zero vectors here, so that the
+    // multi-rhs variant tuner and the quantisation workspace of each shape
+    // run at preparation and not inside the first solve
+    for (int l = 1; l < p.nlevels; ++l) {
+      vzero<R>(BL[l - 1]->rc, BL[l]->op->sh.n, s);
+      BL[l]->op->apply(BL[l - 1]->rc, BL[l - 1]->ec, s);
+    }
+    MGC_HIP(hipStreamSynchronize(s));
     bn = nrhs;
     bprec = coarse_prec;
   }

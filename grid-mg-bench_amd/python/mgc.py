@@ -122,6 +122,7 @@ _SIGS = {
     "mgc_coarse_op_prepare_mrhs": ([_vp, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs_batched": ([_vp, _i, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
+    "mgc_coarse_solve_batched": ([_vp, _i, _vp, _vp, _i, _i, _c.c_double, _i, _i, _vp, _vp], _i),
     "mgc_coarse_op_M_mrhs_noalloc": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
     "mgc_mrhs_set_variant": ([_i], _i),
@@ -467,6 +468,22 @@ def M_mrhs_batched(ops, ins, outs, nrhs, mrhs_prec=F32, in_scales=None, dag=0, s
     call("mgc_coarse_op_M_mrhs_batched", arr(*[o.handle for o in ops]), n,
          arr(*[_p(x) for x in ins]), sc, int(mrhs_prec), arr(*[_p(x) for x in outs]),
          int(nrhs), int(mrhs_prec), int(dag), stream.handle if stream else None)
+
+
+def coarse_solve_batched(ops, bs, xs, nrhs, mrhs_prec=F32, tol=1e-6, maxit=200, restart=20,
+                         stream=None) -> list:
+    """Lockstep GMRES on several coarse lattices, one batched stencil launch per
+    application; b, x per lattice are FP32 [V][nrhs][N].  Returns the stats of
+    every (lattice, rhs) column, lattice-major."""
+    n = len(ops)
+    if not (len(bs) == n and len(xs) == n):
+        raise ValueError("ops, bs and xs must have the same length")
+    arr = ctypes.c_void_p * n
+    st = (SolveStats * (n * int(nrhs)))()
+    call("mgc_coarse_solve_batched", arr(*[o.handle for o in ops]), n, arr(*[_p(b) for b in bs]),
+         arr(*[_p(x) for x in xs]), int(nrhs), int(mrhs_prec), float(tol), int(maxit),
+         int(restart), st, stream.handle if stream else None)
+    return [s.as_dict() for s in st]
 
 
 def quantize(inp, rows, n, prec, out, scales, stream=None):

@@ -515,6 +515,23 @@ int mgc_mg_solve(mgc_mg* mg, const void* b, void* x, double tol, int maxit, int 
  * (src/Benchmark_mrhs_coarsening.cc, include/CoarsenedMatrixUpstreamImprovedDirsaveLutMRHS.h). */
 int mgc_mg_solve_mrhs(mgc_mg* mg, const void* b, void* x, int nrhs, int coarse_prec, double tol,
                       int maxit, int restart, mgc_solve_stats* stats, void* stream);
+/* Coarse solves on up to 8 lattices at once (one nbasis, any volumes): for
+ * each lattice k and rhs r, M_k x_k[., r] = b_k[., r] with b_k, x_k complex
+ * FP32 [V_k][nrhs][nbasis] on the device (x_k: output, zero initial guess).
+ * One lockstep restarted GMRES(restart) runs over all count * nrhs (<= 64)
+ * columns, each with its own Krylov coefficients and stopping test
+ * (relative residual <= tol, at most maxit iterations); every operator
+ * application is a single mgc_coarse_op_M_mrhs_batched launch over the
+ * lattices in mrhs_prec (F32, or F16 / BF16 with FP32 accumulation), so the
+ * small lattices ride along with the large one.  The batched form of the
+ * coarsest GMRES of the reference's preconditioner
+ * (include/MultigridPreconditioner.h:566-641), which solves one lattice at a
+ * time.  stats[k * nrhs + r]: iterations until that column converged,
+ * converged, true_residual = ||b - M x|| / ||b|| recomputed with the FP32
+ * multi-rhs stencil, seconds of the whole call (precond_applies 0). */
+int mgc_coarse_solve_batched(mgc_coarse_op* const* ops, int count, const void* const* b,
+                             void* const* x, int nrhs, int mrhs_prec, double tol, int maxit,
+                             int restart, mgc_solve_stats* stats, void* stream);
 /* the FP64 coarse operator of level >= 1 and the aggregation of level >= 0 (owned by mg) */
 int mgc_mg_coarse_op(mgc_mg* mg, int level, mgc_coarse_op** cop);
 int mgc_mg_aggregation(mgc_mg* mg, int level, mgc_aggregation** agg);

@@ -193,6 +193,39 @@ __global__ void k_import_links(const cplx<double>* __restrict__ src, int layout,
   }
 }
 
+// The same import, one workgroup per (site, point) matrix through LDS: the
+// source is read along its rows (c fastest, contiguous in both layouts) and
+// the column-major destination written along its columns (r fastest), so
+// both sides are coalesced.  For N <= 64.
+template <typename R>
+__global__ __launch_bounds__(256) void k_import_links_tiled(
+    const cplx<double>* __restrict__ src, int layout, int V, int N, cplx<R>* __restrict__ links,
+    cplx<R>* __restrict__ self_dag, cplx<double>* __restrict__ self_rm) {
+  __shared__ cplx<double> sA[64][65];
+  const int x = blockIdx.x / 9, p = blockIdx.x - x * 9;
+  const int t = threadIdx.x;
+  for (int idx = t; idx < N * N; idx += 256) {
+    const int r = idx / N, c = idx - r * N;
+    sA[r][c] = src[ext_link_index(layout, V, N, p, x, r, c)];
+  }
+  __syncthreads();
+  cplx<R>* dst = links + ((size_t)x * 9 + p) * N * N;  // [c][r]
+  for (int idx = t; idx < N * N; idx += 256) {
+    const int c = idx / N, r = idx - c * N;
+    const cplx<double> a = sA[r][c];
+    dst[idx] = {(R)a.re, (R)a.im};
+  }
+  if (p == 8) {
+    // adj(A8) column-major: element (row r, col c) = conj(A8(c, r)), at [c][r]
+    for (int idx = t; idx < N * N; idx += 256) {
+      const int c = idx / N, r = idx - c * N;
+      const cplx<double> b = sA[c][r];
+      self_dag[(size_t)x * N * N + idx] = {(R)b.re, (R)(-b.im)};
+      self_rm[(size_t)x * N * N + idx] = sA[c][r];  // row-major: idx = row c, column r
+    }
+  }
+}
+
 template <typename R>
 __global__ void k_export_links(const cplx<R>* __restrict__ links, int layout, int V, int N,
                                cplx<double>* __restrict__ dst) {
@@ -298,6 +331,168 @@ __global__ __launch_bounds__(256) void k_gauss_jordan(cplx<double>* __restrict__
   if (tid == 0 && s_bad) atomicOr(singular, 1);
 }
 
+// The same Gauss-Jordan inversion for N <= 64 with the matrix in registers:
+// the 256 threads hold it as a 16 x 16 grid of 4 x 4 tiles (thread t: rows
+// 4 (t / 16) .., columns 4 (t % 16) ..), zero-padded to 64 x 64 with an
+// identity block.  Per step k the pivot column and the normalised pivot row
+// pass through LDS; every element gets exactly the operations of
+// k_gauss_jordan (pivot = first largest |M(i,k)|^2, row k times the pivot's
+// inverse, M(i,c) -= M(i,k) M(k,c)), so both give the same bits.  The row
+// interchanges are undone as one column permutation at the store.
+__global__ __launch_bounds__(256) void k_gauss_jordan_reg(cplx<double>* __restrict__ mats, int N,
+                                                          int* __restrict__ singular) {
+  __shared__ cplx<double> s_col[2][64];  // pivot column before the step (double-buffered)
+  __shared__ cplx<double> s_row[64];     // normalised pivot row
+  __shared__ cplx<double> s_rk[64], s_rp[64];  // rows k and pr for the interchange
+  __shared__ int s_perm[64];
+  __shared__ int s_pos[64];
+  __shared__ int s_piv, s_bad;
+  const int t = threadIdx.x;
+  const int ti = t >> 4, tj = t & 15;
+  const size_t base = (size_t)blockIdx.x * N * N;
+  cplx<double> m[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * ti + r, j = 4 * tj + c;
+      m[r][c] = (i < N && j < N) ? mats[base + (size_t)i * N + j]
+                                 : cplx<double>{i == j ? 1.0 : 0.0, 0.0};
+    }
+  if (t == 0) s_bad = 0;
+  for (int k = 0; k < N; ++k) {
+    const int kb = k >> 2, kr = k & 3;
+    cplx<double>* col = s_col[k & 1];
+    // a. the pivot column to LDS
+    if (tj == kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cplx<double> v = m[r][0];
+#pragma unroll
+        for (int c = 1; c < 4; ++c)
+          if (c == kr) v = m[r][c];
+        col[4 * ti + r] = v;
+      }
+    __syncthreads();
+    // b. wavefront 0 finds the pivot row (first largest |v|^2 at or below k)
+    if (t < 64) {
+      double best = -1.0;
+      int bi = k;
+      if (t >= k && t < N) {
+        const cplx<double> v = col[t];
+        best = v.re * v.re + v.im * v.im;
+        bi = t;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ob = __shfl_xor(best, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ob > best || (ob == best && oi < bi)) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if (t == 0) {
+        s_piv = bi;
+        s_perm[k] = bi;
+        if (!(best > 0.0)) s_bad = 1;
+        const cplx<double> a = col[k];  // the pivot column follows the interchange
+        col[k] = col[bi];
+        col[bi] = a;
+      }
+    }
+    __syncthreads();
+    const int pr = s_piv;
+    // c. interchange rows k and pr
+    if (pr != k) {
+      const int pb = pr >> 2, prr = pr & 3;
+      if (ti == kb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (r == kr)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s_rk[4 * tj + c] = m[r][c];
+      if (ti == pb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (r == prr)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) s_rp[4 * tj + c] = m[r][c];
+      __syncthreads();
+      if (ti == kb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (r == kr)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) m[r][c] = s_rp[4 * tj + c];
+      if (ti == pb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (r == prr)
+#pragma unroll
+            for (int c = 0; c < 4; ++c) m[r][c] = s_rk[4 * tj + c];
+    }
+    const cplx<double> piv = col[k];
+    const double den = piv.re * piv.re + piv.im * piv.im;
+    const cplx<double> pinv = den > 0.0 ? cplx<double>{piv.re / den, -piv.im / den}
+                                        : cplx<double>{0.0, 0.0};
+    // d. normalise row k, publish it
+    if (ti == kb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r == kr)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) {
+            const int j = 4 * tj + c;
+            m[r][c] = (j == k) ? pinv : cmul(m[r][c], pinv);
+            s_row[j] = m[r][c];
+          }
+    __syncthreads();
+    // e. eliminate column k from every other row
+    cplx<double> f[4], rk[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) f[r] = col[4 * ti + r];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) rk[c] = s_row[4 * tj + c];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      if (4 * ti + r == k) continue;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (4 * tj + c == k) {
+          const cplx<double> q = cmul(f[r], pinv);
+          m[r][c] = {-q.re, -q.im};
+        } else {
+          const cplx<double> q = cmul(f[r], rk[c]);
+          m[r][c] = {m[r][c].re - q.re, m[r][c].im - q.im};
+        }
+      }
+    }
+  }
+  // the row interchanges, undone last first as column interchanges, make one
+  // permutation: output column j = current column s_pos[j]
+  __syncthreads();
+  if (t == 0) {
+    for (int j = 0; j < N; ++j) s_pos[j] = j;
+    for (int k = N - 1; k >= 0; --k) {
+      const int pr = s_perm[k];
+      const int a = s_pos[k];
+      s_pos[k] = s_pos[pr];
+      s_pos[pr] = a;
+    }
+    // s_perm <- inverse: output column of current column c
+    for (int j = 0; j < N; ++j) s_perm[s_pos[j]] = j;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = 4 * ti + r, j = 4 * tj + c;
+      if (i < N && j < N) mats[base + (size_t)i * N + s_perm[j]] = m[r][c];
+    }
+  if (t == 0 && s_bad) atomicOr(singular, 1);
+}
+
 // row-major FP64 -> column-major (prec) plus its adjoint
 template <typename R>
 __global__ void k_store_inverse(const cplx<double>* __restrict__ rm, int V, int N,
@@ -337,11 +532,15 @@ void invert_self_rowmajor(cplx<double>* rm, int V, int N, hipStream_t s) {
   int* d_flag = nullptr;
   MGC_HIP(hipMalloc(&d_flag, sizeof(int)));
   fill_zero(d_flag, sizeof(int), s);
-  const size_t shmem = (size_t)(N * N + N) * sizeof(cplx<double>);
-  if (shmem > 64 * 1024)
-    MGC_HIP(hipFuncSetAttribute((const void*)k_gauss_jordan,
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-  k_gauss_jordan<<<V, 256, shmem, s>>>(rm, N, d_flag);
+  if (N <= 64) {
+    k_gauss_jordan_reg<<<V, 256, 0, s>>>(rm, N, d_flag);
+  } else {
+    const size_t shmem = (size_t)(N * N + N) * sizeof(cplx<double>);
+    if (shmem > 64 * 1024)
+      MGC_HIP(hipFuncSetAttribute((const void*)k_gauss_jordan,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    k_gauss_jordan<<<V, 256, shmem, s>>>(rm, N, d_flag);
+  }
   MGC_LAUNCH_CHECK();
   int flag = 0;
   MGC_HIP(hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -450,14 +649,19 @@ int mgc_coarse_op_set_links(mgc_coarse_op* op, int layout, const void* links_f64
     MGC_HIP(hipMalloc(&rm, (size_t)V * N * N * sizeof(cplx<double>)));
     const size_t total = (size_t)V * 9 * N * N;
     const unsigned g = grid_for(total, 256, 16384);
-    if (op->prec == MGC_F64)
-      k_import_links<double><<<g, 256, 0, s>>>((const cplx<double>*)links_f64, layout, V, N,
-                                               (cplx<double>*)op->links,
+    const auto* lf = (const cplx<double>*)links_f64;
+    if (op->prec == MGC_F64 && N <= 64)
+      k_import_links_tiled<double><<<V * 9, 256, 0, s>>>(lf, layout, V, N, (cplx<double>*)op->links,
+                                                         (cplx<double>*)op->self_dag, rm);
+    else if (op->prec == MGC_F32 && N <= 64)
+      k_import_links_tiled<float><<<V * 9, 256, 0, s>>>(lf, layout, V, N, (cplx<float>*)op->links,
+                                                        (cplx<float>*)op->self_dag, rm);
+    else if (op->prec == MGC_F64)
+      k_import_links<double><<<g, 256, 0, s>>>(lf, layout, V, N, (cplx<double>*)op->links,
                                                (cplx<double>*)op->self_dag, rm);
     else if (op->prec == MGC_F32)
-      k_import_links<float><<<g, 256, 0, s>>>((const cplx<double>*)links_f64, layout, V, N,
-                                              (cplx<float>*)op->links, (cplx<float>*)op->self_dag,
-                                              rm);
+      k_import_links<float><<<g, 256, 0, s>>>(lf, layout, V, N, (cplx<float>*)op->links,
+                                              (cplx<float>*)op->self_dag, rm);
     else
       throw Error(MGC_ERR_PRECISION,
                   "half-precision operators take their links through an F32/F64 operator");

@@ -748,13 +748,16 @@ __global__ __launch_bounds__(CqShape<NP>::THREADS) void k_cholqr_gram(
   cq_dx4 acc[S::GTPW];
 #pragma unroll
   for (int q = 0; q < S::GTPW; ++q) acc[q] = {0.0, 0.0, 0.0, 0.0};
-  cplx<double> reg[S::PER];
-  cq_fetch<NP>(U, reg, 0, rows, nb, Vb, Ef, xc, k);
-  for (int r0 = 0; r0 < rows; r0 += CQ_CH) {
+  // two register sets: the loads of chunk c + 2 are issued as chunk c is
+  // staged, so each chunk's loads have two chunks of products to arrive
+  cplx<double> ra[S::PER], rb[S::PER];
+  cq_fetch<NP>(U, ra, 0, rows, nb, Vb, Ef, xc, k);
+  cq_fetch<NP>(U, rb, CQ_CH, rows, nb, Vb, Ef, xc, k);
+  auto chunk = [&](int r0, cplx<double>* reg) {
     __syncthreads();  // previous chunk consumed
     cq_stage<NP>(sV, reg);
     __syncthreads();
-    if (r0 + CQ_CH < rows) cq_fetch<NP>(U, reg, r0 + CQ_CH, rows, nb, Vb, Ef, xc, k);
+    if (r0 + 2 * CQ_CH < rows) cq_fetch<NP>(U, reg, r0 + 2 * CQ_CH, rows, nb, Vb, Ef, xc, k);
     const double* src = sV;
     if (round == 1) {
       cq_times_T<NP>(sV, sT, sW);
@@ -762,6 +765,10 @@ __global__ __launch_bounds__(CqShape<NP>::THREADS) void k_cholqr_gram(
       src = sW;
     }
     cq_gram_acc<NP>(src, acc);
+  };
+  for (int r0 = 0; r0 < rows; r0 += 2 * CQ_CH) {
+    chunk(r0, ra);
+    if (r0 + CQ_CH < rows) chunk(r0 + CQ_CH, rb);
   }
   __syncthreads();
   auto sG = reinterpret_cast<cplx<double>(*)[S::NS]>(sW);
@@ -800,13 +807,16 @@ __global__ __launch_bounds__(CqShape<NP>::THREADS) void k_cholqr_apply(
   cq_load_T<NP>(sT, T1, nb);
   cq_load_T<NP>(sT2, T1 + (size_t)gridDim.x * nb * nb, nb);
   const int rows = Vb * (Ef / 2);
-  cplx<double> reg[S::PER];
-  cq_fetch<NP>(U, reg, 0, rows, nb, Vb, Ef, xc, k);
-  for (int r0 = 0; r0 < rows; r0 += CQ_CH) {
+  // loads two chunks ahead, as the Gram rounds (every chunk's rows are read
+  // before any of them is written back, so the in-place update is safe)
+  cplx<double> ra[S::PER], rb[S::PER];
+  cq_fetch<NP>(U, ra, 0, rows, nb, Vb, Ef, xc, k);
+  cq_fetch<NP>(U, rb, CQ_CH, rows, nb, Vb, Ef, xc, k);
+  auto chunk = [&](int r0, cplx<double>* reg) {
     __syncthreads();  // previous chunk stored
     cq_stage<NP>(sV, reg);
     __syncthreads();
-    if (r0 + CQ_CH < rows) cq_fetch<NP>(U, reg, r0 + CQ_CH, rows, nb, Vb, Ef, xc, k);
+    if (r0 + 2 * CQ_CH < rows) cq_fetch<NP>(U, reg, r0 + 2 * CQ_CH, rows, nb, Vb, Ef, xc, k);
     cq_times_T<NP>(sV, sT, sW);
     __syncthreads();
     cq_times_T<NP>(sW, sT2, sV);
@@ -818,6 +828,10 @@ __global__ __launch_bounds__(CqShape<NP>::THREADS) void k_cholqr_apply(
         U[cq_row_off(r0 + r, nb, Vb, Ef, xc, k) + i] = {sV[r * CqShape<NP>::LDP + 2 * i],
                                                         sV[r * CqShape<NP>::LDP + 2 * i + 1]};
     }
+  };
+  for (int r0 = 0; r0 < rows; r0 += 2 * CQ_CH) {
+    chunk(r0, ra);
+    if (r0 + CQ_CH < rows) chunk(r0 + CQ_CH, rb);
   }
 }
 

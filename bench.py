@@ -241,10 +241,12 @@ def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
 # a 1 GiB copy kernel (FETCH x 2.000, WRITE x 1.000).  Each figure is the mean
 # of the variant's two autotuner trials in those runs (the tuner times every
 # variant in dispatch order, twice, six launches a trial), so variants that
-# share a kernel are still told apart.  The command lines and readings are in
-# the profile summary for this kernel under profiles/.
+# share a kernel are still told apart.  Variant 9 (the one the tuner picks)
+# was read again at commit 0f953b4 with the variant forced (852.9 MB, within
+# 0.07% of the first reading).  The command lines and readings are in
+# profiles/k_mrhs_f16_8x8x8x8_n64_nrhs48.md.
 HEADLINE_TRAFFIC_BYTES = {0: 987.5e6, 1: 898.2e6, 2: 996.4e6, 3: 1023.9e6, 4: 925.4e6,
-                          5: 987.6e6, 6: 921.8e6, 7: 1008.1e6, 8: 921.7e6, 9: 852.3e6}
+                          5: 987.6e6, 6: 921.8e6, 7: 1008.1e6, 8: 921.7e6, 9: 852.9e6}
 
 MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 1: "k_mrhs (LDS-staged B', non-temporal link loads)",
@@ -329,14 +331,16 @@ def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, kee
     return res
 
 
-def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10):
+def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10,
+           forms=("fp64_plain", "mixed_fp32", "mixed_fp32_graph")):
     """Three-level K-cycle multigrid solve of the Wilson-clover system
     (m = -0.1, csw = 1.0, random SU(3) field eps = 0.4) on one Gaussian rhs:
     FGMRES(20) outer in FP64 with true-residual recomputation, hierarchy in
     FP64 ("plain form") and in FP32 ("mixed"), the reference's default setup
     (BiCGSTAB 1e-3 / 100 test vectors).  "mixed_fp32_graph" runs the K-cycle
     as a captured HIP graph of device-scalar GMRES with fixed step counts.
-    Setup and solve timed apart."""
+    Setup, the first preconditioner application (graph capture) and the
+    solve timed apart."""
     V = int(np.prod(dims))
     g = mgc.DeviceBuffer(4 * V * 9 * 16)
     mgc.random_gauge(g, dims, 0.4, SEED, 21, stream)
@@ -354,14 +358,24 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10):
                          f"to {tol:g}"}
     for name, prec, graph in (("fp64_plain", mgc.F64, 0), ("mixed_fp32", mgc.F32, 0),
                               ("mixed_fp32_graph", mgc.F32, 2)):
+        if name not in forms:
+            continue
         p = mgc.mg_params(nlevels=len(blocks) + 1, block=blocks, nb=nbs, precision=prec,
                           graph=graph)
         mg = mgc.Multigrid(fop, p)
         t0 = time.perf_counter()
         mg.setup(stream)
         t_setup = time.perf_counter() - t0
+        # one preconditioner application before the timed solve: first-use
+        # work (workspaces; in graph mode the capture and instantiation of
+        # the K-cycle graph) is timed apart
+        t0 = time.perf_counter()
+        mg.precondition(b, x)
+        stream.sync()
+        t_first = time.perf_counter() - t0
         st = mg.solve(b, x, tol=tol, maxit=500, restart=20)
-        out[name] = {"setup_s": t_setup, "solve_s": st["seconds"],
+        out[name] = {"setup_s": t_setup, "first_precondition_s": t_first,
+                     "solve_s": st["seconds"],
                      "outer_iterations": st["iterations"], "converged": bool(st["converged"]),
                      "true_residual": st["true_residual"],
                      "precond_applies": st["precond_applies"]}
@@ -431,7 +445,8 @@ def coarse_batch_leg(mgc, stream, fines=((48, 48, 48, 48), (32, 32, 32, 32), (24
     of FP32 Wilson-clover hierarchies on the given fine lattices, nrhs
     Gaussian rhs each, GMRES(20) to tol with the stencil in mp (FP16 MFMA by
     default) and one multi-lattice launch per application; against the same
-    solves run one lattice at a time."""
+    solves run one lattice at a time.  Both with prepared handles (the
+    solve's state kept between calls), mean of 5 solves after a warm-up."""
     mp = mgc.F16 if mp is None else mp
     mgs, fops, gs, ops, bs, xs, coarse = [], [], [], [], [], [], []
     for k, d in enumerate(fines):
@@ -452,20 +467,26 @@ def coarse_batch_leg(mgc, stream, fines=((48, 48, 48, 48), (32, 32, 32, 32), (24
         mgs.append(mg); fops.append(fop); gs.append(g); ops.append(op); bs.append(b)
         coarse.append(cd)
     stream.sync()
-    mgc.coarse_solve_batched(ops, bs, xs, nrhs, mrhs_prec=mp, tol=tol, maxit=300,
-                             stream=stream)  # warm-up (autotuner, workspaces)
+    # prepared batches (workspaces and Krylov basis kept), timed after a warm-up
+    # solve that also runs the stencil's variant tuner
+    cb = mgc.CoarseBatch(ops, nrhs, mrhs_prec=mp, restart=20, stream=stream)
+    cb.solve(bs, xs, tol=tol, maxit=300)
+    reps = 5
     t0 = time.perf_counter()
-    st = mgc.coarse_solve_batched(ops, bs, xs, nrhs, mrhs_prec=mp, tol=tol, maxit=300,
-                                  stream=stream)
-    t_batch = time.perf_counter() - t0
+    for _ in range(reps):
+        st = cb.solve(bs, xs, tol=tol, maxit=300)
+    t_batch = (time.perf_counter() - t0) / reps
+    cb.close()
     t_sep, st_sep = 0.0, []
     for k in range(len(ops)):
-        mgc.coarse_solve_batched([ops[k]], [bs[k]], [xs[k]], nrhs, mrhs_prec=mp, tol=tol,
-                                 maxit=300, stream=stream)  # warm-up
+        c1 = mgc.CoarseBatch([ops[k]], nrhs, mrhs_prec=mp, restart=20, stream=stream)
+        c1.solve([bs[k]], [xs[k]], tol=tol, maxit=300)
         t0 = time.perf_counter()
-        st_sep += mgc.coarse_solve_batched([ops[k]], [bs[k]], [xs[k]], nrhs, mrhs_prec=mp,
-                                           tol=tol, maxit=300, stream=stream)
-        t_sep += time.perf_counter() - t0
+        for _ in range(reps):
+            s1 = c1.solve([bs[k]], [xs[k]], tol=tol, maxit=300)
+        t_sep += (time.perf_counter() - t0) / reps
+        st_sep += s1
+        c1.close()
     for lst in (xs, bs, gs):
         for b in lst:
             b.close()
@@ -599,8 +620,12 @@ def main() -> int:
                     help="run only the batched multigrid-solve leg and print it (for profiling)")
     ap.add_argument("--only-aux", action="store_true",
                     help="run only the transfer, fine-operator and CoarsenOperator legs")
+    ap.add_argument("--only-mg-small", action="store_true",
+                    help="only run the 16^4 eager / HIP-graph multigrid comparison")
     ap.add_argument("--only-coarse-batch", action="store_true",
                     help="only run the multi-lattice batched coarse-solve leg")
+    ap.add_argument("--only-wilson", action="store_true",
+                    help="only run the Wilson-clover apply legs")
     ap.add_argument("--only-coarsen", action="store_true",
                     help="only run the CoarsenOperator / block-orthonormalisation leg")
     ap.add_argument("--only-mg", action="store_true",
@@ -646,9 +671,18 @@ def main() -> int:
     if args.only_mg_batch:
         print(json.dumps({"mg_batched": batch_leg()}), flush=True)
         return 0
+    if args.only_mg_small:
+        log("leg: multigrid solve 16^4, eager against HIP-graph K-cycle")
+        print(json.dumps({"small_graph": mg_leg(mgc, stream, (16, 16, 16, 16),
+                                                [(4, 4, 4, 4), (2, 2, 2, 2)], [16, 16],
+                                                forms=("mixed_fp32", "mixed_fp32_graph"))}),
+              flush=True)
+        return 0
     if args.only_coarse_batch:
         log("leg: batched coarse solves 12^4 + 8^4 + 6^4, 12 rhs each")
-        print(json.dumps({"coarse_batched": coarse_batch_leg(mgc, stream)}), flush=True)
+        print(json.dumps({"coarse_batched": coarse_batch_leg(mgc, stream),
+                          "coarse_batched_f32": coarse_batch_leg(mgc, stream, mp=mgc.F32,
+                                                                 tol=1e-6)}), flush=True)
         return 0
     if args.only_mg:
         d = tuple(int(v) for v in args.mg_dims.split(","))
@@ -657,6 +691,13 @@ def main() -> int:
                                              [16, 16])}), flush=True)
         return 0
     legs = {}
+    if args.only_wilson:
+        for prec, nm, nrhs in ((mgc.F64, "f64", 1), (mgc.F32, "f32", 1), (mgc.F32, "f32", 12)):
+            log(f"leg: Wilson-clover 32^4 {nm} nrhs={nrhs}")
+            legs[f"wilson_{nm}_nrhs{nrhs}"] = fine_op_leg(mgc, stream, (32, 32, 32, 32), prec, nrhs,
+                                                          args.steps // 4, args.warmup)
+        print(json.dumps({"legs": legs}), flush=True)
+        return 0
     if args.only_coarsen:
         log("leg: CoarsenOperator 32^4 nbasis=64")
         legs["coarsen_f64_n64"] = coarsen_leg(mgc, stream, (32, 32, 32, 32), (4, 4, 4, 4), 32)
@@ -702,8 +743,12 @@ def main() -> int:
         mg = mg_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)], [16, 16])
         if args.mg_batch:
             mg["batched"] = batch_leg()
+        log("leg: multigrid solve 16^4, eager against HIP-graph K-cycle")
+        mg["small_graph"] = mg_leg(mgc, stream, (16, 16, 16, 16), [(4, 4, 4, 4), (2, 2, 2, 2)],
+                                   [16, 16], forms=("mixed_fp32", "mixed_fp32_graph"))
         log("leg: batched coarse solves 12^4 + 8^4 + 6^4, 12 rhs each")
         mg["coarse_batched"] = coarse_batch_leg(mgc, stream)
+        mg["coarse_batched_f32"] = coarse_batch_leg(mgc, stream, mp=mgc.F32, tol=1e-6)
     ref64 = legs.get("M_1rhs_f64_n64")
     line = {
         "metric": METRIC,
@@ -728,6 +773,9 @@ def main() -> int:
             "unit": "GB/s" if head["bound"] == "hbm" else "TFLOP/s",
             "frac": head["hbm_frac"] if head["bound"] == "hbm" else head["mfma_frac"],
             "traffic": HEADLINE_TRAFFIC_BYTES.get(head["kernel_variant"]),
+            "traffic_source": "rocprofv3 FETCH_SIZE / WRITE_SIZE of this kernel variant, "
+                              "calibrated on a 1 GiB copy, read at commit 0f953b4 "
+                              "(profiles/k_mrhs_f16_8x8x8x8_n64_nrhs48.md)",
             "kernel": head["kernel_name"],
             "bytes_per_launch": head["bytes_per_launch"],
             "flops_per_launch": head["flops_per_launch"],

@@ -1,23 +1,30 @@
-// Coarse solves on several lattices at once: one lockstep batched GMRES over
-// the right-hand sides of every lattice, whose operator application is a
-// single multi-lattice launch of the multi-rhs stencil
-// (mgc_coarse_op_M_mrhs_batched).  This is the batched form of the coarsest
+// Coarse solves on several lattices at once, with one multi-lattice launch
+// of the multi-rhs stencil (mgc_coarse_op_M_mrhs_batched) for every operator
+// application of all of them.  This is the batched form of the coarsest
 // solve of the reference's preconditioner (a GMRES on each coarse operator,
 // include/MultigridPreconditioner.h:566-641 and include/VCycleRework.h:84-189,
 // one lattice at a time there).
 //
-// The solver's vectors hold all lattices side by side as extra columns:
-// [Vmax][K * nrhs][N] FP32, lattice k in columns k*nrhs .. k*nrhs + nrhs - 1
-// of its first V_k sites and zero in the padding sites beyond.  The batched
-// BLAS then reduces per (lattice, rhs) column, and the padding stays zero
-// (the operator writes zeros there).  Each application gathers every
-// lattice's columns into its own [V_k][nrhs][N] input (quantised to FP16 /
-// BF16 with one scale per (site, rhs) row for a half-precision stencil), runs
-// the one batched launch, and scatters the outputs back.
+// Each lattice k runs its own lockstep batched GMRES over its nrhs columns
+// (exact-size vectors [V_k][nrhs][N] FP32, so the Krylov BLAS moves no more
+// than a solve of that lattice alone) on its own host thread and stream.
+// Their operator applications meet at a rendezvous: every live lattice
+// deposits its input and output and records an event on its stream; the last
+// to arrive makes a launch stream wait on all those events, quantises the
+// inputs (FP16 / BF16), issues the one batched launch, and records a
+// completion event that every lattice's stream waits on before it goes on.
+// A lattice whose columns have all converged (or that stopped at maxit)
+// leaves the rendezvous, and the remaining ones go on batching without it.
+// The lattices' host-side Krylov steps (one small device-to-host read per
+// Arnoldi step each) overlap one another on their threads.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <type_traits>
+#include <condition_variable>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "bkrylov.hpp"
@@ -25,223 +32,328 @@
 
 namespace mgc {
 
-constexpr int CS_MAX = 8;  // lattices per batched solve (the batched stencil's limit)
+constexpr int CS_MAX = 8;  // lattices per batch (the batched stencil's limit)
 
 static void cs_ok(int st) {
   if (st != MGC_OK) throw Error(st, mgc_last_error());
 }
-static unsigned cs_grid(size_t n) { return grid_for(n, 256, 8192); }
 
-struct CsLayout {
-  int K, nrhs, N, Vmax;
-  int V[CS_MAX];
-  int first[CS_MAX + 1];  // prefix sums of V_k * nrhs (rows of the dense inputs)
-  void* dense[CS_MAX];    // per-lattice [V_k][nrhs][N]: FP32, or half with scales
-  float* scale[CS_MAX];   // per-lattice [V_k][nrhs] (half precision only)
+// The meeting point of the lattices' operator applications.
+class LaunchRendezvous {
+ public:
+  LaunchRendezvous(const std::vector<mgc_coarse_op*>& ops, int nrhs, int mprec)
+      : ops_(ops), K_(int(ops.size())), nrhs_(nrhs), mprec_(mprec), in_(K_), out_(K_),
+        present_(K_, 0), ready_(K_, nullptr) {
+    for (int k = 0; k < K_; ++k)
+      MGC_HIP(hipEventCreateWithFlags(&ready_[k], hipEventDisableTiming));
+    for (int i = 0; i < 2; ++i) MGC_HIP(hipEventCreateWithFlags(&done_[i], hipEventDisableTiming));
+    MGC_HIP(hipStreamCreateWithFlags(&ls_, hipStreamNonBlocking));
+    if (mprec != MGC_F32) {
+      for (int k = 0; k < K_; ++k) {
+        const size_t rows = (size_t)ops[k]->V * nrhs;
+        q_[k].alloc(rows * ops[k]->N * 4);
+        qs_[k].alloc(rows);
+      }
+    }
+  }
+  ~LaunchRendezvous() {
+    for (hipEvent_t e : ready_)
+      if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : done_)
+      if (e) (void)hipEventDestroy(e);
+    if (ls_) (void)hipStreamDestroy(ls_);
+  }
+  LaunchRendezvous(const LaunchRendezvous&) = delete;
+  LaunchRendezvous& operator=(const LaunchRendezvous&) = delete;
+
+  // start of a solve: every lattice takes part
+  void open() {
+    std::lock_guard<std::mutex> l(m_);
+    live_ = K_;
+    arrived_ = 0;
+    std::fill(present_.begin(), present_.end(), 0);
+    error_.clear();
+  }
+
+  // lattice k: y = M_k x (FP32 [V_k][nrhs][N]), ordered on its stream sk
+  void apply(int k, const cplx<float>* x, cplx<float>* y, hipStream_t sk) {
+    MGC_HIP(hipEventRecord(ready_[k], sk));
+    std::unique_lock<std::mutex> l(m_);
+    in_[k] = x;
+    out_[k] = y;
+    present_[k] = 1;
+    const long g = gen_;
+    if (++arrived_ == live_) launch_locked();
+    else cv_.wait(l, [&] { return gen_ != g; });
+    const std::string err = error_;
+    l.unlock();
+    if (!err.empty()) throw Error(MGC_ERR_HIP, "batched coarse apply failed: " + err);
+    // the generation after g + 1 needs this lattice to arrive again, so
+    // done_[g & 1] is not recorded anew before this wait is queued
+    MGC_HIP(hipStreamWaitEvent(sk, done_[g & 1], 0));
+  }
+
+  // run f under the rendezvous lock: library calls from a lattice's thread
+  // that may reach the stencil's tuning tables (shared by all threads) go
+  // one at a time, as the batched launches do
+  template <typename F>
+  void serialised(F&& f) {
+    std::lock_guard<std::mutex> l(m_);
+    f();
+  }
+
+  // a lattice has finished its solve (or failed): the others go on without it
+  void leave() {
+    std::lock_guard<std::mutex> l(m_);
+    --live_;
+    if (live_ > 0 && arrived_ == live_) launch_locked();
+  }
+
+ private:
+  void launch_locked() {
+    try {
+      std::vector<mgc_coarse_op*> ops;
+      std::vector<const void*> ins;
+      std::vector<const float*> scs;
+      std::vector<void*> outs;
+      for (int k = 0; k < K_; ++k) {
+        if (!present_[k]) continue;
+        MGC_HIP(hipStreamWaitEvent(ls_, ready_[k], 0));
+        ops.push_back(ops_[k]);
+        outs.push_back(out_[k]);
+        if (mprec_ == MGC_F32) {
+          ins.push_back(in_[k]);
+        } else {
+          cs_ok(mgc_quantize(in_[k], (size_t)ops_[k]->V * nrhs_, ops_[k]->N, mprec_, q_[k].p,
+                             qs_[k].p, (void*)ls_));
+          ins.push_back(q_[k].p);
+          scs.push_back(qs_[k].p);
+        }
+      }
+      cs_ok(mgc_coarse_op_M_mrhs_batched(ops.data(), int(ops.size()), ins.data(),
+                                         mprec_ == MGC_F32 ? nullptr : scs.data(), mprec_,
+                                         outs.data(), nrhs_, mprec_, 0, (void*)ls_));
+      MGC_HIP(hipEventRecord(done_[gen_ & 1], ls_));
+    } catch (const std::exception& e) {
+      error_ = e.what();
+    }
+    arrived_ = 0;
+    std::fill(present_.begin(), present_.end(), 0);
+    ++gen_;
+    cv_.notify_all();
+  }
+
+  std::vector<mgc_coarse_op*> ops_;
+  int K_, nrhs_, mprec_;
+  std::mutex m_;
+  std::condition_variable cv_;
+  int live_ = 0, arrived_ = 0;
+  long gen_ = 0;
+  std::vector<const cplx<float>*> in_;
+  std::vector<cplx<float>*> out_;
+  std::vector<char> present_;
+  std::vector<hipEvent_t> ready_;
+  hipEvent_t done_[2] = {nullptr, nullptr};
+  hipStream_t ls_ = nullptr;
+  DevBuf<unsigned char> q_[CS_MAX];  // quantised inputs (FP16 / BF16 stencil)
+  DevBuf<float> qs_[CS_MAX];
+  std::string error_;
 };
 
-__device__ __forceinline__ int cs_lattice(const CsLayout& L, int row) {
-  int k = 0;
-#pragma unroll
-  for (int i = 1; i < CS_MAX; ++i)
-    if (i < L.K && row >= L.first[i]) k = i;
-  return k;
-}
-
-// padded columns -> per-lattice dense rows, one wavefront per (lattice, site,
-// rhs) row; KIND = MGC_F32 copies, MGC_F16 / MGC_BF16 quantise the row with
-// scale = max |component| (1 for a zero row), as the stencil's own quantiser
-template <int KIND>
-__global__ __launch_bounds__(256) void k_cs_gather(const float2* __restrict__ padded, CsLayout L) {
-  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= L.first[L.K]) return;
-  const int k = cs_lattice(L, row);
-  const int local = row - L.first[k];
-  const int x = local / L.nrhs, r = local - x * L.nrhs;
-  const float2* src = padded + ((size_t)x * L.K * L.nrhs + (size_t)k * L.nrhs + r) * L.N;
-  if (KIND == MGC_F32) {
-    float2* dst = reinterpret_cast<float2*>(L.dense[k]) + (size_t)local * L.N;
-    for (int c = lane; c < L.N; c += 64) dst[c] = src[c];
-    return;
-  }
-  typedef typename std::conditional<KIND == MGC_F16, _Float16, __bf16>::type T;
-  float m = 0.f;
-  for (int c = lane; c < L.N; c += 64) m = fmaxf(m, fmaxf(fabsf(src[c].x), fabsf(src[c].y)));
-  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  const float s = m > 0.f ? m : 1.f;
-  const float inv = 1.f / s;
-  T* o = reinterpret_cast<T*>(L.dense[k]) + (size_t)local * 2 * L.N;
-  for (int c = lane; c < L.N; c += 64) {
-    o[2 * c] = (T)(src[c].x * inv);
-    o[2 * c + 1] = (T)(src[c].y * inv);
-  }
-  if (lane == 0) L.scale[k][local] = s;
-}
-
-// per-lattice FP32 outputs -> padded columns, zeros in the padding sites;
-// one thread per padded element
-__global__ __launch_bounds__(256) void k_cs_scatter(const CsLayout L, float2* __restrict__ padded,
-                                                   const float2* const* __restrict__ src) {
-  const size_t cols = (size_t)L.K * L.nrhs;
-  const size_t total = (size_t)L.Vmax * cols * L.N;
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
-       i += (size_t)gridDim.x * blockDim.x) {
-    const int c = (int)(i % L.N);
-    const size_t q = i / L.N;
-    const int col = (int)(q % cols);
-    const int x = (int)(q / cols);
-    const int k = col / L.nrhs, r = col - k * L.nrhs;
-    padded[i] = x < L.V[k] ? src[k][((size_t)x * L.nrhs + r) * L.N + c] : float2{0.f, 0.f};
-  }
-}
-
-// the K coarse operators on the padded layout, one batched stencil launch
-struct MultiLatticeOp : BLinOp<float> {
-  std::vector<mgc_coarse_op*> ops;
-  int mprec;
-  CsLayout L{};
-  DevBuf<unsigned char> dense_in;
-  DevBuf<float> scales;
-  DevBuf<cplx<float>> dense_out;
-  DevBuf<const float2*> out_ptrs;
-  std::vector<const void*> in_v;
-  std::vector<const float*> sc_v;
-  std::vector<void*> out_v;
-
-  MultiLatticeOp(mgc_coarse_op* const* o, int K, int nrhs, int mp, hipStream_t s)
-      : ops(o, o + K), mprec(mp) {
-    L.K = K;
-    L.nrhs = nrhs;
-    L.N = o[0]->N;
-    L.Vmax = 0;
-    L.first[0] = 0;
-    for (int k = 0; k < K; ++k) {
-      L.V[k] = o[k]->V;
-      L.Vmax = std::max(L.Vmax, o[k]->V);
-      L.first[k + 1] = L.first[k] + o[k]->V * nrhs;
-    }
-    sh.nrhs = K * nrhs;
-    sh.dof = L.N;
-    sh.n = (size_t)L.Vmax * sh.nrhs * L.N;
-    const size_t rows = (size_t)L.first[K];
-    const size_t elem = mp == MGC_F32 ? 8 : 4;  // bytes per complex in the stencil's input
-    dense_in.alloc(rows * L.N * elem);
-    if (mp != MGC_F32) scales.alloc(rows);
-    dense_out.alloc(rows * L.N);
-    std::vector<const float2*> hp(K);
-    for (int k = 0; k < K; ++k) {
-      L.dense[k] = dense_in.p + (size_t)L.first[k] * L.N * elem;
-      L.scale[k] = mp == MGC_F32 ? nullptr : scales.p + L.first[k];
-      hp[k] = reinterpret_cast<const float2*>(dense_out.p + (size_t)L.first[k] * L.N);
-      in_v.push_back(L.dense[k]);
-      sc_v.push_back(L.scale[k]);
-      out_v.push_back(dense_out.p + (size_t)L.first[k] * L.N);
-    }
-    out_ptrs.alloc(K);
-    MGC_HIP(hipMemcpyAsync(out_ptrs.p, hp.data(), K * sizeof(const float2*),
-                           hipMemcpyHostToDevice, s));
-    MGC_HIP(hipStreamSynchronize(s));
-  }
-
-  void gather(const cplx<float>* padded, hipStream_t s) {
-    const int rows = L.first[L.K];
-    const int g = (rows + 3) / 4;
-    const float2* p = reinterpret_cast<const float2*>(padded);
-    if (mprec == MGC_F32) k_cs_gather<MGC_F32><<<g, 256, 0, s>>>(p, L);
-    else if (mprec == MGC_F16) k_cs_gather<MGC_F16><<<g, 256, 0, s>>>(p, L);
-    else k_cs_gather<MGC_BF16><<<g, 256, 0, s>>>(p, L);
-    MGC_LAUNCH_CHECK();
-  }
-  void scatter(cplx<float>* padded, hipStream_t s) {
-    k_cs_scatter<<<cs_grid(sh.n), 256, 0, s>>>(L, reinterpret_cast<float2*>(padded),
-                                               out_ptrs.p);
-    MGC_LAUNCH_CHECK();
+// lattice k's operator for its solver: the rendezvous
+struct RendezvousOp : BLinOp<float> {
+  LaunchRendezvous* rv;
+  int k;
+  RendezvousOp(LaunchRendezvous* r, int kk, const mgc_coarse_op* op, int nrhs) : rv(r), k(kk) {
+    sh.n = (size_t)op->V * op->N * nrhs;
+    sh.nrhs = nrhs;
+    sh.dof = op->N;
   }
   void apply(const cplx<float>* in, cplx<float>* out, hipStream_t s) override {
-    gather(in, s);
-    cs_ok(mgc_coarse_op_M_mrhs_batched(ops.data(), L.K, in_v.data(),
-                                    mprec == MGC_F32 ? nullptr : sc_v.data(), mprec, out_v.data(),
-                                    L.nrhs, mprec, 0, (void*)s));
-    scatter(out, s);
+    rv->apply(k, in, out, s);
   }
+};
+
+// one lattice's solver state
+struct LatticeSolve {
+  mgc_coarse_op* op;
+  hipStream_t s = nullptr;
+  hipEvent_t fin = nullptr;
+  std::unique_ptr<BBlas> bl;
+  std::unique_ptr<RendezvousOp> A;
+  std::unique_ptr<BGmres<float>> solver;
+  DevBuf<cplx<float>> r;
+  LatticeSolve(LaunchRendezvous* rv, int k, mgc_coarse_op* o, int nrhs, int restart) : op(o) {
+    MGC_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    MGC_HIP(hipEventCreateWithFlags(&fin, hipEventDisableTiming));
+    bl = std::make_unique<BBlas>(s);
+    A = std::make_unique<RendezvousOp>(rv, k, o, nrhs);
+    solver = std::make_unique<BGmres<float>>(A->sh, restart, false);
+    r.alloc(A->sh.n);
+  }
+  ~LatticeSolve() {
+    solver.reset();
+    A.reset();
+    bl.reset();
+    if (fin) (void)hipEventDestroy(fin);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  LatticeSolve(const LatticeSolve&) = delete;
+  LatticeSolve& operator=(const LatticeSolve&) = delete;
+};
+
+// A prepared batch: the rendezvous and every lattice's solver (Krylov basis,
+// BLAS workspaces, stream), kept from one solve to the next.
+struct CoarseBatch {
+  hipStream_t s;
+  int device = 0;
+  int nrhs, mprec;
+  std::unique_ptr<LaunchRendezvous> rv;
+  std::vector<std::unique_ptr<LatticeSolve>> lat;
+  hipEvent_t start = nullptr;
+  CoarseBatch(mgc_coarse_op* const* ops, int count, int nr, int mp, int restart, hipStream_t st)
+      : s(st), nrhs(nr), mprec(mp) {
+    MGC_HIP(hipGetDevice(&device));
+    std::vector<mgc_coarse_op*> v(ops, ops + count);
+    rv = std::make_unique<LaunchRendezvous>(v, nr, mp);
+    for (int k = 0; k < count; ++k)
+      lat.push_back(std::make_unique<LatticeSolve>(rv.get(), k, ops[k], nr, restart));
+    MGC_HIP(hipEventCreateWithFlags(&start, hipEventDisableTiming));
+  }
+  ~CoarseBatch() {
+    lat.clear();
+    rv.reset();
+    if (start) (void)hipEventDestroy(start);
+  }
+  CoarseBatch(const CoarseBatch&) = delete;
+  CoarseBatch& operator=(const CoarseBatch&) = delete;
 };
 
 }  // namespace mgc
 
+struct mgc_coarse_batch {
+  std::unique_ptr<mgc::CoarseBatch> impl;
+};
+
 using namespace mgc;
 
+static void check_batch_ops(mgc_coarse_op* const* ops, int count, int nrhs, int mrhs_prec,
+                            int restart) {
+  MGC_CHECK(ops, MGC_ERR_ARG, "null argument");
+  MGC_CHECK(count >= 1 && count <= CS_MAX, MGC_ERR_ARG, "count must be 1..8");
+  MGC_CHECK(nrhs >= 1 && nrhs <= BBlas::RMAX, MGC_ERR_ARG, "nrhs must be 1..64");
+  MGC_CHECK(mrhs_prec == MGC_F32 || mrhs_prec == MGC_F16 || mrhs_prec == MGC_BF16,
+            MGC_ERR_PRECISION, "multi-rhs precision must be F32, F16 or BF16");
+  MGC_CHECK(restart >= 1 && restart < BBlas::KMAX, MGC_ERR_ARG, "1 <= restart < 64 needed");
+  for (int k = 0; k < count; ++k) {
+    MGC_CHECK(ops[k], MGC_ERR_ARG, "null lattice in the batch");
+    MGC_CHECK(ops[k]->links_set, MGC_ERR_STATE, "links not set");
+    MGC_CHECK(ops[k]->N == ops[0]->N, MGC_ERR_SHAPE,
+              "every lattice in a batch needs the same nbasis");
+    MGC_CHECK(ops[k]->N >= 8 && ops[k]->N <= 64 && ops[k]->N % 8 == 0, MGC_ERR_SHAPE,
+              "multi-rhs stencil supports nbasis 8, 16, ..., 64");
+    for (int j = 0; j < k; ++j)
+      MGC_CHECK(ops[j] != ops[k], MGC_ERR_ARG, "a lattice appears twice in the batch");
+  }
+}
+
+static void batch_solve(CoarseBatch& cb, const void* const* b, void* const* x, double tol,
+                        int maxit, mgc_solve_stats* stats) {
+  const int count = int(cb.lat.size()), nrhs = cb.nrhs;
+  MGC_CHECK(b && x && stats, MGC_ERR_ARG, "null argument");
+  MGC_CHECK(tol > 0.0 && maxit >= 1, MGC_ERR_ARG, "tol > 0 and maxit >= 1 needed");
+  for (int k = 0; k < count; ++k)
+    MGC_CHECK(b[k] && x[k], MGC_ERR_ARG, "null vector in the batch");
+  const auto t0 = std::chrono::steady_clock::now();
+  // the lattices' streams start after the caller's earlier work
+  MGC_HIP(hipEventRecord(cb.start, cb.s));
+  for (auto& L : cb.lat) MGC_HIP(hipStreamWaitEvent(L->s, cb.start, 0));
+  cb.rv->open();
+  std::vector<std::string> err(count);
+  std::vector<BKrylovStats> st(count);
+  std::vector<std::vector<double>> bn(count), rn(count);
+  auto run = [&](int k) {
+    LatticeSolve& L = *cb.lat[k];
+    bool left = false;
+    try {
+      MGC_HIP(hipSetDevice(cb.device));
+      const auto* bk = static_cast<const cplx<float>*>(b[k]);
+      auto* xk = static_cast<cplx<float>*>(x[k]);
+      st[k] = L.solver->solve(*L.bl, *L.A, nullptr, bk, xk, true, tol, maxit);
+      cb.rv->leave();
+      left = true;
+      // b - M x with the stencil of the solve, per rhs
+      bn[k].resize(nrhs);
+      rn[k].resize(nrhs);
+      bnorm2<float>(*L.bl, bk, L.A->sh, bn[k].data());
+      cb.rv->serialised([&] {
+        cs_ok(mgc_coarse_op_M_mrhs(L.op, xk, nullptr, MGC_F32, L.r.p, nrhs, cb.mprec, 0,
+                                   (void*)L.s));
+      });
+      vsub<float>(bk, L.r.p, L.r.p, L.A->sh.n, L.s);
+      bnorm2<float>(*L.bl, L.r.p, L.A->sh, rn[k].data());
+      MGC_HIP(hipEventRecord(L.fin, L.s));
+    } catch (const std::exception& e) {
+      err[k] = e.what();
+      if (!left) cb.rv->leave();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int k = 1; k < count; ++k) th.emplace_back(run, k);
+  run(0);
+  for (auto& t : th) t.join();
+  for (int k = 0; k < count; ++k)
+    if (!err[k].empty()) throw Error(MGC_ERR_HIP, "lattice " + std::to_string(k) + ": " + err[k]);
+  for (auto& L : cb.lat) MGC_HIP(hipStreamWaitEvent(cb.s, L->fin, 0));
+  MGC_HIP(hipStreamSynchronize(cb.s));
+  const double secs =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int k = 0; k < count; ++k)
+    for (int r = 0; r < nrhs; ++r) {
+      mgc_solve_stats& o = stats[k * nrhs + r];
+      o.iterations = st[k].converged[r] ? st[k].iters_rhs[r] : st[k].iters;
+      o.converged = st[k].converged[r];
+      o.precond_applies = 0;
+      o.true_residual = bn[k][r] > 0.0 ? std::sqrt(rn[k][r] / bn[k][r]) : 0.0;
+      o.seconds = secs;
+    }
+}
+
 extern "C" {
+
+int mgc_coarse_batch_create(mgc_coarse_batch** out, mgc_coarse_op* const* ops, int count,
+                            int nrhs, int mrhs_prec, int restart, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(out, MGC_ERR_ARG, "null output handle");
+    *out = nullptr;
+    check_batch_ops(ops, count, nrhs, mrhs_prec, restart);
+    auto h = std::make_unique<mgc_coarse_batch>();
+    h->impl = std::make_unique<CoarseBatch>(ops, count, nrhs, mrhs_prec, restart,
+                                            as_stream(stream));
+    *out = h.release();
+  });
+}
+
+int mgc_coarse_batch_solve(mgc_coarse_batch* cb, const void* const* b, void* const* x, double tol,
+                           int maxit, mgc_solve_stats* stats, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(cb && cb->impl, MGC_ERR_ARG, "null batch handle");
+    MGC_CHECK(as_stream(stream) == cb->impl->s, MGC_ERR_ARG, "use the stream given to create");
+    batch_solve(*cb->impl, b, x, tol, maxit, stats);
+  });
+}
+
+int mgc_coarse_batch_destroy(mgc_coarse_batch* cb) {
+  return guarded([&] { delete cb; });
+}
 
 int mgc_coarse_solve_batched(mgc_coarse_op* const* ops, int count, const void* const* b,
                              void* const* x, int nrhs, int mrhs_prec, double tol, int maxit,
                              int restart, mgc_solve_stats* stats, void* stream) {
   return guarded([&] {
-    MGC_CHECK(ops && b && x && stats, MGC_ERR_ARG, "null argument");
-    MGC_CHECK(count >= 1 && count <= CS_MAX, MGC_ERR_ARG, "count must be 1..8");
-    MGC_CHECK(nrhs >= 1 && count * nrhs <= BBlas::RMAX, MGC_ERR_ARG,
-              "count * nrhs must be 1..64 (columns of one lockstep batch)");
-    MGC_CHECK(mrhs_prec == MGC_F32 || mrhs_prec == MGC_F16 || mrhs_prec == MGC_BF16,
-              MGC_ERR_PRECISION, "multi-rhs precision must be F32, F16 or BF16");
-    MGC_CHECK(tol > 0.0 && maxit >= 1 && restart >= 1 && restart < BBlas::KMAX, MGC_ERR_ARG,
-              "tol > 0, maxit >= 1 and 1 <= restart < 64 needed");
-    for (int k = 0; k < count; ++k) {
-      MGC_CHECK(ops[k] && b[k] && x[k], MGC_ERR_ARG, "null lattice in the batch");
-      MGC_CHECK(ops[k]->links_set, MGC_ERR_STATE, "links not set");
-      MGC_CHECK(ops[k]->N == ops[0]->N, MGC_ERR_SHAPE,
-                "every lattice in a batch needs the same nbasis");
-      MGC_CHECK(ops[k]->N >= 8 && ops[k]->N <= 64 && ops[k]->N % 8 == 0, MGC_ERR_SHAPE,
-                "multi-rhs stencil supports nbasis 8, 16, ..., 64");
-    }
-    hipStream_t s = as_stream(stream);
-    const auto t0 = std::chrono::steady_clock::now();
-    MultiLatticeOp A(ops, count, nrhs, mrhs_prec, s);
-    const size_t n = A.sh.n;
-    DevBuf<cplx<float>> pb(n), px(n), pr(n);
-    // right-hand sides into the padded layout: the dense FP32 inputs are the
-    // scatter's sources
-    {
-      std::vector<const float2*> hp(count);
-      for (int k = 0; k < count; ++k) hp[k] = reinterpret_cast<const float2*>(b[k]);
-      DevBuf<const float2*> dp(count);
-      MGC_HIP(hipMemcpyAsync(dp.p, hp.data(), count * sizeof(const float2*),
-                             hipMemcpyHostToDevice, s));
-      k_cs_scatter<<<cs_grid(n), 256, 0, s>>>(A.L, reinterpret_cast<float2*>(pb.p), dp.p);
-      MGC_LAUNCH_CHECK();
-      MGC_HIP(hipStreamSynchronize(s));
-    }
-    BBlas bl(s);
-    BGmres<float> solver(A.sh, restart, false);
-    const BKrylovStats st = solver.solve(bl, A, nullptr, pb.p, px.p, true, tol, maxit);
-    // residual b - M x recomputed with the FP32 stencil (the F16 / BF16 solve's
-    // operator rounding shows here), per (lattice, rhs) column
-    std::vector<double> bn(A.sh.nrhs), rn(A.sh.nrhs);
-    bnorm2<float>(bl, pb.p, A.sh, bn.data());
-    {
-      MultiLatticeOp A32(ops, count, nrhs, MGC_F32, s);
-      A32.apply(px.p, pr.p, s);
-      vsub<float>(pb.p, pr.p, pr.p, n, s);
-      bnorm2<float>(bl, pr.p, A.sh, rn.data());
-    }
-    // solutions back to the callers' [V_k][nrhs][N] FP32 vectors
-    {
-      CsLayout Lx = A.L;
-      for (int k = 0; k < count; ++k) Lx.dense[k] = x[k];
-      k_cs_gather<MGC_F32><<<(Lx.first[count] + 3) / 4, 256, 0, s>>>(
-          reinterpret_cast<const float2*>(px.p), Lx);
-      MGC_LAUNCH_CHECK();
-    }
-    MGC_HIP(hipStreamSynchronize(s));
-    const double secs =
-        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    for (int c = 0; c < A.sh.nrhs; ++c) {
-      mgc_solve_stats& o = stats[c];
-      o.iterations = st.converged[c] ? st.iters_rhs[c] : st.iters;
-      o.converged = st.converged[c];
-      o.precond_applies = 0;
-      o.true_residual = bn[c] > 0.0 ? std::sqrt(rn[c] / bn[c]) : 0.0;
-      o.seconds = secs;
-    }
+    check_batch_ops(ops, count, nrhs, mrhs_prec, restart);
+    CoarseBatch cb(ops, count, nrhs, mrhs_prec, restart, as_stream(stream));
+    batch_solve(cb, b, x, tol, maxit, stats);
   });
 }
 

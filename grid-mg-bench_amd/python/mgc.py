@@ -123,6 +123,9 @@ _SIGS = {
     "mgc_coarse_op_M_mrhs": ([_vp, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs_batched": ([_vp, _i, _vp, _vp, _i, _vp, _i, _i, _i, _vp], _i),
     "mgc_coarse_solve_batched": ([_vp, _i, _vp, _vp, _i, _i, _c.c_double, _i, _i, _vp, _vp], _i),
+    "mgc_coarse_batch_create": ([_vp, _vp, _i, _i, _i, _i, _vp], _i),
+    "mgc_coarse_batch_solve": ([_vp, _vp, _vp, _c.c_double, _i, _vp, _vp], _i),
+    "mgc_coarse_batch_destroy": ([_vp], _i),
     "mgc_coarse_op_M_mrhs_noalloc": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
     "mgc_mrhs_set_variant": ([_i], _i),
@@ -484,6 +487,42 @@ def coarse_solve_batched(ops, bs, xs, nrhs, mrhs_prec=F32, tol=1e-6, maxit=200, 
          arr(*[_p(x) for x in xs]), int(nrhs), int(mrhs_prec), float(tol), int(maxit),
          int(restart), st, stream.handle if stream else None)
     return [s.as_dict() for s in st]
+
+
+class CoarseBatch:
+    """A prepared lockstep coarse solve over several lattices (mgc_coarse_batch):
+    the workspaces and Krylov basis are kept between solves."""
+
+    def __init__(self, ops, nrhs, mrhs_prec=F32, restart=20, stream=None):
+        self.ops, self.nrhs, self.stream = list(ops), int(nrhs), stream
+        arr = ctypes.c_void_p * len(self.ops)
+        h = _vp()
+        call("mgc_coarse_batch_create", _c.byref(h), arr(*[o.handle for o in self.ops]),
+             len(self.ops), self.nrhs, int(mrhs_prec), int(restart),
+             stream.handle if stream else None)
+        self.handle = h.value
+
+    def solve(self, bs, xs, tol=1e-6, maxit=200) -> list:
+        n = len(self.ops)
+        if not (len(bs) == n and len(xs) == n):
+            raise ValueError("one b and one x per lattice")
+        arr = ctypes.c_void_p * n
+        st = (SolveStats * (n * self.nrhs))()
+        call("mgc_coarse_batch_solve", self.handle, arr(*[_p(b) for b in bs]),
+             arr(*[_p(x) for x in xs]), float(tol), int(maxit), st,
+             self.stream.handle if self.stream else None)
+        return [s.as_dict() for s in st]
+
+    def close(self) -> None:
+        if self.handle:
+            call("mgc_coarse_batch_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def quantize(inp, rows, n, prec, out, scales, stream=None):

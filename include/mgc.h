@@ -517,21 +517,36 @@ int mgc_mg_solve_mrhs(mgc_mg* mg, const void* b, void* x, int nrhs, int coarse_p
                       int maxit, int restart, mgc_solve_stats* stats, void* stream);
 /* Coarse solves on up to 8 lattices at once (one nbasis, any volumes): for
  * each lattice k and rhs r, M_k x_k[., r] = b_k[., r] with b_k, x_k complex
- * FP32 [V_k][nrhs][nbasis] on the device (x_k: output, zero initial guess).
- * One lockstep restarted GMRES(restart) runs over all count * nrhs (<= 64)
- * columns, each with its own Krylov coefficients and stopping test
- * (relative residual <= tol, at most maxit iterations); every operator
- * application is a single mgc_coarse_op_M_mrhs_batched launch over the
- * lattices in mrhs_prec (F32, or F16 / BF16 with FP32 accumulation), so the
- * small lattices ride along with the large one.  The batched form of the
+ * FP32 [V_k][nrhs][nbasis] on the device (x_k: output, zero initial guess),
+ * nrhs <= 64.  Each lattice runs a lockstep restarted GMRES(restart) over its
+ * nrhs columns, each column with its own Krylov coefficients and stopping
+ * test (relative residual <= tol, at most maxit iterations), on a host thread
+ * and stream of its own; the lattices' operator applications meet, and each
+ * meeting is a single mgc_coarse_op_M_mrhs_batched launch over the lattices
+ * still iterating, in mrhs_prec (F32, or F16 / BF16 with FP32
+ * accumulation), so the small lattices ride along with the large one.  The
+ * result of every lattice equals that of a batch of that lattice alone, bit
+ * for bit.  Each lattice may appear once.  The batched form of the
  * coarsest GMRES of the reference's preconditioner
  * (include/MultigridPreconditioner.h:566-641), which solves one lattice at a
  * time.  stats[k * nrhs + r]: iterations until that column converged,
- * converged, true_residual = ||b - M x|| / ||b|| recomputed with the FP32
- * multi-rhs stencil, seconds of the whole call (precond_applies 0). */
+ * converged, true_residual = ||b - M x|| / ||b|| recomputed with the
+ * multi-rhs stencil in mrhs_prec, seconds of the whole call (precond_applies
+ * 0). */
 int mgc_coarse_solve_batched(mgc_coarse_op* const* ops, int count, const void* const* b,
                              void* const* x, int nrhs, int mrhs_prec, double tol, int maxit,
                              int restart, mgc_solve_stats* stats, void* stream);
+/* The same solve with its state kept between calls (operator workspaces, the
+ * padded vectors and the Krylov basis of GMRES(restart)), for a caller that
+ * solves on the same lattices repeatedly (the coarsest level of a K-cycle):
+ * create once, solve with new b / x as often as needed on the stream given to
+ * create, destroy.  The operators must outlive the handle. */
+typedef struct mgc_coarse_batch mgc_coarse_batch;
+int mgc_coarse_batch_create(mgc_coarse_batch** out, mgc_coarse_op* const* ops, int count,
+                            int nrhs, int mrhs_prec, int restart, void* stream);
+int mgc_coarse_batch_solve(mgc_coarse_batch* cb, const void* const* b, void* const* x, double tol,
+                           int maxit, mgc_solve_stats* stats, void* stream);
+int mgc_coarse_batch_destroy(mgc_coarse_batch* cb);
 /* the FP64 coarse operator of level >= 1 and the aggregation of level >= 0 (owned by mg) */
 int mgc_mg_coarse_op(mgc_mg* mg, int level, mgc_coarse_op** cop);
 int mgc_mg_aggregation(mgc_mg* mg, int level, mgc_aggregation** agg);

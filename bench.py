@@ -243,10 +243,11 @@ def coarsen_leg(mgc, stream, fine_dims, blk, nb, reps=3):
 # variant in dispatch order, twice, six launches a trial), so variants that
 # share a kernel are still told apart.  Variant 9 (the one the tuner picks)
 # was read again at commit 0f953b4 with the variant forced (852.9 MB, within
-# 0.07% of the first reading).  The command lines and readings are in
+# 0.07% of the first reading), and variant 10 with it forced when it was added.  The command lines and readings are in
 # profiles/k_mrhs_f16_8x8x8x8_n64_nrhs48.md.
 HEADLINE_TRAFFIC_BYTES = {0: 987.5e6, 1: 898.2e6, 2: 996.4e6, 3: 1023.9e6, 4: 925.4e6,
-                          5: 987.6e6, 6: 921.8e6, 7: 1008.1e6, 8: 921.7e6, 9: 852.9e6}
+                          5: 987.6e6, 6: 921.8e6, 7: 1008.1e6, 8: 921.7e6, 9: 852.9e6,
+                          10: 849.5e6}
 
 MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 1: "k_mrhs (LDS-staged B', non-temporal link loads)",
@@ -259,7 +260,8 @@ MRHS_KERNELS = {0: "k_mrhs (LDS-staged B')",
                 7: "k_mrhs_dma (input copied to LDS, 2 m-tiles per wavefront)",
                 8: "k_mrhs_dma (input copied to LDS, 2 m-tiles per wavefront, non-temporal "
                    "link loads, XCD-contiguous sites)",
-                9: "k_mrhs_xblk (LDS-staged B', non-temporal link loads, XCD-block sites)"}
+                9: "k_mrhs_xblk (LDS-staged B', non-temporal link loads, XCD-block sites)",
+                10: "k_mrhs_xblk2 (as 9, input and link loads two points ahead)"}
 
 
 def mrhs_leg(mgc, stream, dims, N, nrhs, mp, steps, warmup, half_input=True, keep=None):

@@ -151,11 +151,11 @@ static void launch_batched(int mp, int N, int variant, int NT, int blocks, const
   throw Error(MGC_ERR_SHAPE, "multi-rhs stencil supports nbasis 8, 16, ..., 64");
 }
 
-// variants the tuner times: all ten for nbasis 32 and 64, the LDS-staged
+// variants the tuner times: all eleven for nbasis 32 and 64, the LDS-staged
 // ones elsewhere; the batched launch has the LDS-staged ones without the
 // XCD-block order
-static const int ALL_VARIANTS[10] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9};
-static const int LDS_VARIANTS[7] = {0, 1, 3, 4, 5, 6, 9};
+static const int ALL_VARIANTS[11] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+static const int LDS_VARIANTS[8] = {0, 1, 3, 4, 5, 6, 9, 10};
 static const int BATCH_VARIANTS[6] = {0, 1, 3, 4, 5, 6};
 
 // packed lattice extents for the XCD-block site order (0 where it does not
@@ -220,7 +220,7 @@ static int tuned_variant(int mp, int N, int NT, int V, int dims, const uint4* A,
   if (hit != g_mrhs_choice.end()) return hit->second;
   const bool all = mrhs_all_variants(N);
   const int v = time_variants(
-      all ? ALL_VARIANTS : LDS_VARIANTS, all ? 10 : 7,
+      all ? ALL_VARIANTS : LDS_VARIANTS, all ? 11 : 8,
       [&](int v) {
         launch_any(mp, v, N, NT, V, dims, A, as, in, is, out, nbr, nrhs, rhs0, nr, dag, s);
       },

@@ -173,7 +173,7 @@ __device__ __forceinline__ uint4 bprime_perm(uint4 r, unsigned sel, unsigned m) 
 // before the barrier that releases the MFMAs.  For FP16/BF16 the point's
 // product T_p is accumulated from zero and folded into the FP32 result as
 // C += s_link(x,p) * s_in(x+d_p, j) * T_p.
-template <int KIND, int N, int NT, int LAUX>
+template <int KIND, int N, int NT, int LAUX, bool D2 = false>
 __device__ __forceinline__ void mrhs_lds_site(
     const int x, const uint4* __restrict__ A, const float* __restrict__ ascale,
     const void* __restrict__ in_, const float* __restrict__ in_scale, float* __restrict__ out,
@@ -243,14 +243,14 @@ __device__ __forceinline__ void mrhs_lds_site(
 
   const int lane_off = w * NKC * 64 + l;
   const LinkSrc<LAUX> src(A + (size_t)x * 9 * NMT * NKC * 64, 9 * NMT * NKC * 64 * 16, lane_off);
-  uint4 a[NKC], b_n[SPT];
-  auto issue_b = [&](int y) {
+  // 16-byte input chunks of neighbour y into dst (one per staged slot)
+  auto issue_b = [&](uint4* dst, int y) {
     const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<char*>(in + (size_t)y * site_bytes), (short)0, site_bytes, MRHS_BUF_DWORD3);
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
       const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, lane_b, slot_b(q), 0);
-      b_n[q] = make_uint4(v.x, v.y, v.z, v.w);
+      dst[q] = make_uint4(v.x, v.y, v.z, v.w);
     }
   };
 
@@ -258,26 +258,25 @@ __device__ __forceinline__ void mrhs_lds_site(
 #pragma unroll
   for (int nt = 0; nt < NT; ++nt) C[nt] = {0.f, 0.f, 0.f, 0.f};
 
-  // One point: stage the landed input chunks as B', issue the next point's
-  // input loads, release the MFMAs; the link fragment for the next point is
-  // loaded into a[kc] as soon as this point's MFMAs have consumed it.
-  auto point = [&](int q9, auto more_c) {
+  // One point: stage the landed input chunks b as B', issue the input loads
+  // of neighbour yl into b (the point LOOK points ahead), release the MFMAs;
+  // the link fragment a is reloaded for point pl as soon as this point's
+  // MFMAs have consumed it.
+  auto point = [&](int q9, uint4* b, uint4* a, int pl, int yl, auto more_c) {
     constexpr bool more = decltype(more_c)::value;
     const int p = mrhs_point(q9);
-    const int pn = more ? mrhs_point(q9 + 1) : 0;  // never the self point
-    const int yn = more ? __builtin_amdgcn_readfirstlane(s_ys[w][pn]) : 0;
     if (q9 > 0) __syncthreads();  // previous point's MFMA readers are done with sB
 #pragma unroll
     for (int q = 0; q < SPT; ++q) {
       if constexpr (SCALED)
         sB[threadIdx.x + q * THREADS] =
-            bprime_perm(b_n[q], ssel, smask0 ^ (slot_neg(q) ? 0x80008000u : 0u));
+            bprime_perm(b[q], ssel, smask0 ^ (slot_neg(q) ? 0x80008000u : 0u));
       else
-        sB[threadIdx.x + q * THREADS] = bprime<KIND>(b_n[q], u, slot_neg(q));
+        sB[threadIdx.x + q * THREADS] = bprime<KIND>(b[q], u, slot_neg(q));
     }
-    if constexpr (more) issue_b(yn);  // next point's input loads overlap this point's MFMAs
+    if constexpr (more) issue_b(b, yl);  // later point's input loads overlap this point's MFMAs
     __syncthreads();
-    const int anext = (pn * NMT * NKC) * 64;
+    const int anext = (pl * NMT * NKC) * 64;
 
     if constexpr (SCALED) {
       floatx4 Tp[NT];
@@ -306,13 +305,41 @@ __device__ __forceinline__ void mrhs_lds_site(
       }
     }
   };
-
+  auto load_a = [&](uint4* a, int pt) {
 #pragma unroll
-  for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, (8 * NMT * NKC + kc) * 64);
-  issue_b(x);  // the self point comes first
+    for (int kc = 0; kc < NKC; ++kc) a[kc] = src.load(lane_off, (pt * NMT * NKC + kc) * 64);
+  };
+  const std::integral_constant<bool, true> yes{};
+  const std::integral_constant<bool, false> no{};
+  // neighbour of visiting step q (q >= 1: never the self point)
+  auto nb_of = [&](int q) { return __builtin_amdgcn_readfirstlane(s_ys[w][mrhs_point(q)]); };
+
+  if constexpr (!D2) {
+    // one point ahead: one set of input and link registers
+    uint4 a[NKC], b[SPT];
+    load_a(a, 8);
+    issue_b(b, x);  // the self point comes first
 #pragma unroll 1
-  for (int q9 = 0; q9 < 8; ++q9) point(q9, std::integral_constant<bool, true>());
-  point(8, std::integral_constant<bool, false>());
+    for (int q9 = 0; q9 < 8; ++q9) point(q9, b, a, mrhs_point(q9 + 1), nb_of(q9 + 1), yes);
+    point(8, b, a, 0, 0, no);
+  } else {
+    // two points ahead: two register sets, the point loop unrolled by two so
+    // that every set is named at compile time; the loads of point q + 2 are
+    // issued while point q is staged, and wait behind those of point q + 1
+    uint4 a0[NKC], b0[SPT], a1[NKC], b1[SPT];
+    load_a(a0, 8);
+    issue_b(b0, x);
+    load_a(a1, mrhs_point(1));
+    issue_b(b1, nb_of(1));
+#pragma unroll 1
+    for (int q9 = 0; q9 < 6; q9 += 2) {
+      point(q9, b0, a0, mrhs_point(q9 + 2), nb_of(q9 + 2), yes);
+      point(q9 + 1, b1, a1, mrhs_point(q9 + 3), nb_of(q9 + 3), yes);
+    }
+    point(6, b0, a0, mrhs_point(8), nb_of(8), yes);
+    point(7, b1, a1, 0, 0, no);
+    point(8, b0, a0, 0, 0, no);
+  }
 
   // D[m][n]: lane l holds rows 4*(l/16)+i, column l%16
 #pragma unroll
@@ -380,6 +407,29 @@ __global__ __launch_bounds__(mrhs_mtiles(N) * 64, (mrhs_min_waves<KIND, NT>())) 
     int nrhs, int rhs0, int nr, int dag, int dims) {
   mrhs_lds_site<KIND, N, NT, LAUX>(site_of_block_xblk((int)blockIdx.x, dims), A, ascale, in_,
                                   in_scale, out, nbr, nrhs, rhs0, nr, dag);
+}
+
+// Occupancy floor of the two-points-ahead form, which holds a second set of
+// input and link registers: three wavefronts per SIMD (the register budget
+// 168) where the body fits it, as measured from the compiler's resource
+// report; no floor otherwise.
+template <int KIND, int NT>
+constexpr int mrhs_min_waves_d2() {
+  return (KIND == MGC_F32 ? NT <= 2 : NT <= 6) ? 3 : 1;
+}
+
+// XCD-block site order with the input and link loads issued two points
+// ahead (variant 10): arithmetic and order of variant 9, more bytes in
+// flight per workgroup at one wavefront per SIMD less.
+template <int KIND, int N, int NT, int LAUX>
+__global__ __launch_bounds__(mrhs_mtiles(N) * 64, (mrhs_min_waves_d2<KIND, NT>())) void k_mrhs_xblk2(
+    const uint4* __restrict__ A, const float* __restrict__ ascale, const void* __restrict__ in_,
+    const float* __restrict__ in_scale, float* __restrict__ out, const int* __restrict__ nbr,
+    int nrhs, int rhs0, int nr, int dag, int dims, int xchunk) {
+  const int x = dims ? site_of_block_xblk((int)blockIdx.x, dims)
+                     : site_of_block((int)blockIdx.x, xchunk);
+  mrhs_lds_site<KIND, N, NT, LAUX, true>(x, A, ascale, in_, in_scale, out, nbr, nrhs, rhs0, nr,
+                                         dag);
 }
 
 // Register form: no LDS and no barrier.  Each wavefront owns MPW consecutive
@@ -863,13 +913,16 @@ __global__ __launch_bounds__(mrhs_mtiles(N) * 64, (mrhs_persistent_min_waves<KIN
 // 6 = persistent with non-temporal link loads and the XCD-contiguous order,
 // 7 = direct-to-LDS input with two m-tiles per wavefront, 8 = variant 7 with
 // non-temporal link loads and the XCD-contiguous order, 9 = variant 1 with
-// the XCD-block site order (variant 4 where the lattice does not allow it)
-constexpr int MRHS_VARIANTS = 10;
+// the XCD-block site order (variant 4 where the lattice does not allow it),
+// 10 = variant 9 with the loads two points ahead
+constexpr int MRHS_VARIANTS = 11;
 // variant used without tuning: the LDS-staged form, non-temporal links, XCD-block
 // site order (the contiguous XCD order where the lattice does not allow it)
 constexpr int MRHS_DEFAULT_VARIANT = 9;
 static_assert(MRHS_VARIANTS <= 16, "the autotuner times at most 16 variants");
-static int variant_laux(int v) { return (v == 1 || v == 4 || v == 6 || v == 8 || v == 9) ? 2 : 0; }
+static int variant_laux(int v) {
+  return (v == 1 || v == 4 || v == 6 || v == 8 || v == 9 || v == 10) ? 2 : 0;
+}
 static int variant_xchunk(int v, int blocks) {
   return ((v == 3 || v == 4 || v == 6 || v == 8) && blocks % 8 == 0) ? blocks / 8 : 0;
 }
@@ -937,6 +990,11 @@ static void launch_variant(int variant, int V, int dims, const uint4* A, const f
                                                           nr, dag);
       return;
     }
+  }
+  if (variant == 10) {
+    k_mrhs_xblk2<KIND, N, NT, 2><<<g, b, 0, s>>>(A, as, in, is, out, nbr, nrhs, rhs0, nr, dag,
+                                                  dims, dims ? 0 : variant_xchunk(4, V));
+    return;
   }
   if (variant == 9) {
     if (dims) {

@@ -222,7 +222,8 @@ int mgc_coarse_op_M_mrhs_noalloc(mgc_coarse_op* op, const void* in_f32, void* ou
  * non-temporal link loads and the XCD-contiguous order, 7 = input copied
  * straight to LDS with two m-tiles per wavefront, 8 = variant 7 with
  * non-temporal link loads and the XCD-contiguous order, 9 = variant 1 with
- * each XCD taking a 4d block of the lattice (halves of x, y and z, all t);
+ * each XCD taking a 4d block of the lattice (halves of x, y and z, all t),
+ * 10 = variant 9 with the input and link loads issued two points ahead;
  * variants 2, 7 and 8 are built for nbasis 32 and 64 and run as 0, 3 and 4
  * for the other nbasis values; the batched apply
  * uses variants 0, 1 and 3 to 6 and tunes itself when another one is

@@ -81,7 +81,7 @@ def test_mrhs_other_nbasis_match_oracle(gpu, N, mp):
         op.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 @pytest.mark.parametrize("N", [40])
 def test_mrhs_every_kernel_variant_n40(gpu, variant, mp, N):
@@ -105,7 +105,7 @@ def test_mrhs_every_kernel_variant_n40(gpu, variant, mp, N):
     op.close()
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 def test_mrhs_every_kernel_variant(gpu, variant, mp):
     """Each kernel variant the autotuner may pick is checked on its own."""
@@ -129,7 +129,7 @@ def test_mrhs_every_kernel_variant(gpu, variant, mp):
         gpu.call("mgc_mrhs_set_variant", -1)
 
 
-@pytest.mark.parametrize("variant", [7, 8, 9])
+@pytest.mark.parametrize("variant", [7, 8, 9, 10])
 @pytest.mark.parametrize("mp", [F32, F16, BF16])
 @pytest.mark.parametrize("dims,N,nrhs", [((2, 2, 2, 2), 32, 70), ((2, 4, 2, 2), 64, 48),
                                          ((4, 2, 2, 2), 64, 5), ((3, 2, 2, 2), 32, 12)])

@@ -48,3 +48,27 @@ def test_errors_are_status_codes_without_device():
     assert b"null" in L.mgc_last_error()
     assert L.mgc_coarse_op_M(None, None, None, None) == -1
     assert L.mgc_version() >= 10000
+
+
+def test_multigrid_parameter_checks_without_device():
+    """Hierarchy parameters are validated before any device work: a solver
+    type the chosen mode cannot run, an unknown graph mode or basis type is a
+    status code with a message, not a failure later in the set-up."""
+    L = mgc.lib()
+    lop = mgc.make_linop(lambda *a: 0, (4, 4, 4, 4))
+    cases = [
+        (dict(graph=3), b"graph must be 0, 1 or 2"),
+        (dict(graph=2, smoother_solver=mgc.SOLVER_MR), b"graph mode runs GMRES"),
+        (dict(graph=1, coarsest_solver=mgc.SOLVER_BICGSTAB), b"graph mode runs GMRES"),
+        (dict(smoother_solver=mgc.SOLVER_BICGSTAB), b"the smoother is GMRES or MR"),
+        (dict(setup_solver=99), b"unknown setup or coarsest solver type"),
+        (dict(setup_basis_type=7), b"unknown basis-vector type"),
+        (dict(setup_iters=-1), b"negative setup iteration count"),
+    ]
+    for kw, msg in cases:
+        p = mgc.mg_params(nlevels=2, block=[(2, 2, 2, 2)], nb=[4], **kw)
+        h = ctypes.c_void_p()
+        rc = L.mgc_mg_create_linop(ctypes.byref(h), ctypes.byref(lop), ctypes.byref(p))
+        assert rc == -1, kw
+        assert msg in L.mgc_last_error(), (kw, L.mgc_last_error())
+        assert not h.value

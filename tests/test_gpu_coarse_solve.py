@@ -129,3 +129,37 @@ def test_coarse_batch_handle_reuses_state(gpu, problem):
                 assert np.array_equal(a.view(np.float32), r.view(np.float32)), (rep, k)
     finally:
         cb.close()
+
+
+def test_batched_coarse_solve_lattice_leaving_early(gpu, problem):
+    """One lattice far better conditioned than the others (its self link 40
+    times the identity) converges in a couple of iterations and leaves the
+    batch; the others keep batching without it.  Every lattice's result
+    still equals its solve alone, bit for bit."""
+    ops, links, bs, bdev = problem
+    print("[case] batched coarse solve, one lattice leaving early", flush=True)
+    d = LATS[1]
+    V = int(np.prod(d))
+    L = 0.1 / np.sqrt(N) * gauss(9 * V * N * N, 731).reshape(9, V, N, N)
+    L[8] += 40.0 * np.eye(N)[None, :, :]
+    easy = gpu.CoarseOp(d, N, gpu.F64)
+    easy.set_links(gpu.DeviceBuffer.from_numpy(L), gpu.LAYOUT_FLAT, invert=False)
+    batch_ops = [ops[0], easy, ops[2]]
+    batch_b = [bdev[0], bdev[1], bdev[2]]
+    xs = [gpu.DeviceBuffer(int(np.prod(LATS[k])) * NRHS * N * 8) for k in range(3)]
+    st = gpu.coarse_solve_batched(batch_ops, batch_b, xs, NRHS, mrhs_prec=F32, tol=1e-6,
+                                  maxit=300)
+    gpu.synchronize()
+    its = [max(s["iterations"] for s in st[k * NRHS:(k + 1) * NRHS]) for k in range(3)]
+    print(f"  iterations per lattice {its}", flush=True)
+    assert its[1] < min(its[0], its[2])
+    for k in range(3):
+        x1 = gpu.DeviceBuffer(int(np.prod(LATS[k])) * NRHS * N * 8)
+        st1 = gpu.coarse_solve_batched([batch_ops[k]], [batch_b[k]], [x1], NRHS, mrhs_prec=F32,
+                                       tol=1e-6, maxit=300)
+        gpu.synchronize()
+        shp = (int(np.prod(LATS[k])), NRHS, N)
+        assert [s["iterations"] for s in st1] == [s["iterations"] for s in st[k * NRHS:(k + 1) * NRHS]]
+        a = xs[k].to_numpy(np.complex64, shp)
+        b = x1.to_numpy(np.complex64, shp)
+        assert np.array_equal(a.view(np.float32), b.view(np.float32)), k

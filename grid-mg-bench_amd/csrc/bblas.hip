@@ -566,6 +566,161 @@ void bconvert(BBlas& b, const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const 
   MGC_LAUNCH_CHECK();
 }
 
+// ---- BiCGSTAB updates for nrhs systems in lockstep ----
+// Coefficients [3][nrhs] on the device: rows 0 and 1 per kernel below, row 2
+// (.re) 1 for a rhs still iterating; the vectors of a stopped rhs are left
+// untouched, so a breakdown's infinities never reach them.
+
+// p = r + beta (p - omega v); c0 = beta, c1 = omega
+template <typename R>
+__global__ void k_bbicg_p(const cplx<double>* __restrict__ c, const cplx<R>* __restrict__ r,
+                          const cplx<R>* __restrict__ v, cplx<R>* __restrict__ p, size_t n, int C,
+                          int dof, int nrhs) {
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const int q = (int)(mw.m / (unsigned)dof);
+    if (c[2 * nrhs + q].re == 0.0) continue;
+    const cplx<double> be = c[q], om = c[nrhs + q];
+    const cplx<R> vi = v[i], pi = p[i], ri = r[i];
+    const double tr = (double)pi.re - (om.re * vi.re - om.im * vi.im);
+    const double ti = (double)pi.im - (om.re * vi.im + om.im * vi.re);
+    p[i] = {(R)(ri.re + (be.re * tr - be.im * ti)), (R)(ri.im + (be.re * ti + be.im * tr))};
+  }
+}
+
+// s = r - alpha v; c0 = alpha
+template <typename R>
+__global__ void k_bbicg_s(const cplx<double>* __restrict__ c, const cplx<R>* __restrict__ r,
+                          const cplx<R>* __restrict__ v, cplx<R>* __restrict__ s, size_t n, int C,
+                          int dof, int nrhs) {
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const int q = (int)(mw.m / (unsigned)dof);
+    if (c[2 * nrhs + q].re == 0.0) continue;
+    const cplx<double> al = c[q];
+    const cplx<R> vi = v[i], ri = r[i];
+    s[i] = {(R)(ri.re - (al.re * vi.re - al.im * vi.im)),
+            (R)(ri.im - (al.re * vi.im + al.im * vi.re))};
+  }
+}
+
+// x += alpha p + omega s; r = s - omega t; c0 = alpha, c1 = omega
+template <typename R>
+__global__ void k_bbicg_xr(const cplx<double>* __restrict__ c, const cplx<R>* __restrict__ p,
+                           const cplx<R>* __restrict__ s, const cplx<R>* __restrict__ t,
+                           cplx<R>* __restrict__ x, cplx<R>* __restrict__ r, size_t n, int C,
+                           int dof, int nrhs) {
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const int q = (int)(mw.m / (unsigned)dof);
+    if (c[2 * nrhs + q].re == 0.0) continue;
+    const cplx<double> al = c[q], om = c[nrhs + q];
+    const cplx<R> pi = p[i], si = s[i], ti = t[i], xi = x[i];
+    const double xr = (double)xi.re + (al.re * pi.re - al.im * pi.im) +
+                      (om.re * si.re - om.im * si.im);
+    const double xm = (double)xi.im + (al.re * pi.im + al.im * pi.re) +
+                      (om.re * si.im + om.im * si.re);
+    x[i] = {(R)xr, (R)xm};
+    r[i] = {(R)(si.re - (om.re * ti.re - om.im * ti.im)),
+            (R)(si.im - (om.re * ti.im + om.im * ti.re))};
+  }
+}
+
+// i -> (i / C, i % C) along a grid-stride walk, one 64-bit division at the start
+struct DivWalk {
+  size_t q, qstep;
+  unsigned m, step, c;
+  __device__ DivWalk(size_t i0, size_t T, int C)
+      : q(i0 / (size_t)C), qstep(T / (size_t)C), m((unsigned)(i0 % (size_t)C)),
+        step((unsigned)(T % (size_t)C)), c((unsigned)C) {}
+  __device__ void next() {
+    q += qstep;
+    m += step;
+    if (m >= c) {
+      m -= c;
+      ++q;
+    }
+  }
+};
+
+// a set of nrhs vectors [rhs][ld] (each sites x dof) <-> one batched vector
+// [site][rhs][dof]; i walks the batched vector
+template <typename R, bool UNPACK>
+__global__ void k_bpack(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ out, size_t ld,
+                        size_t n, int C, int dof) {
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  DivWalk dw(g, T, C);
+  for (size_t i = g; i < n; i += T, dw.next()) {
+    const unsigned q = dw.m / (unsigned)dof;
+    const size_t f = (size_t)q * ld + dw.q * (size_t)dof + (dw.m - q * (unsigned)dof);
+    if (UNPACK)
+      out[f] = in[i];
+    else
+      out[i] = in[f];
+  }
+}
+
+static void upload_coef(BBlas& b, const cplx<double>* c, int rows, int nrhs) {
+  MGC_HIP(hipEventSynchronize(b.coef_done));
+  std::copy(c, c + (size_t)rows * nrhs, b.h_coef);
+  MGC_HIP(hipMemcpyAsync(b.d_coef, b.h_coef, (size_t)rows * nrhs * sizeof(cplx<double>),
+                         hipMemcpyHostToDevice, b.s));
+  MGC_HIP(hipEventRecord(b.coef_done, b.s));
+}
+
+template <typename R>
+void bbicg_p(BBlas& b, const cplx<double>* c, const cplx<R>* r, const cplx<R>* v, cplx<R>* p,
+             const BShape& sh) {
+  check_shape(sh);
+  upload_coef(b, c, 3, sh.nrhs);
+  k_bbicg_p<R><<<bew_grid(sh.n), 256, 0, b.s>>>(b.d_coef, r, v, p, sh.n, sh.cols(), sh.dof,
+                                                sh.nrhs);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bbicg_s(BBlas& b, const cplx<double>* c, const cplx<R>* r, const cplx<R>* v, cplx<R>* s,
+             const BShape& sh) {
+  check_shape(sh);
+  upload_coef(b, c, 3, sh.nrhs);
+  k_bbicg_s<R><<<bew_grid(sh.n), 256, 0, b.s>>>(b.d_coef, r, v, s, sh.n, sh.cols(), sh.dof,
+                                                sh.nrhs);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bbicg_xr(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* s,
+              const cplx<R>* t, cplx<R>* x, cplx<R>* r, const BShape& sh) {
+  check_shape(sh);
+  upload_coef(b, c, 3, sh.nrhs);
+  k_bbicg_xr<R><<<bew_grid(sh.n), 256, 0, b.s>>>(b.d_coef, p, s, t, x, r, sh.n, sh.cols(),
+                                                 sh.dof, sh.nrhs);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bpack(const cplx<R>* set, size_t ld, cplx<R>* out, const BShape& sh, hipStream_t s) {
+  check_shape(sh);
+  MGC_CHECK(ld * sh.nrhs == sh.n, MGC_ERR_SHAPE, "vector set does not match the batch");
+  k_bpack<R, false><<<bew_grid(sh.n), 256, 0, s>>>(set, out, ld, sh.n, sh.cols(), sh.dof);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bunpack(const cplx<R>* in, cplx<R>* set, size_t ld, const BShape& sh, hipStream_t s) {
+  check_shape(sh);
+  MGC_CHECK(ld * sh.nrhs == sh.n, MGC_ERR_SHAPE, "vector set does not match the batch");
+  k_bpack<R, true><<<bew_grid(sh.n), 256, 0, s>>>(in, set, ld, sh.n, sh.cols(), sh.dof);
+  MGC_LAUNCH_CHECK();
+}
+
 #define INST(R)                                                                               \
   template void bdots_dev<R>(BBlas&, const cplx<R>*, size_t, int, const cplx<R>*,             \
                              const BShape&, bool, cplx<double>*);                             \
@@ -578,6 +733,18 @@ void bconvert(BBlas& b, const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const 
                               cplx<R>*, const BShape&);                                       \
   template void bsubnorm_dev<R>(cplx<double>*, const cplx<R>*, size_t, int, const cplx<R>*,   \
                                 cplx<R>*, const BShape&, hipStream_t);
+INST(double)
+INST(float)
+#undef INST
+#define INST(R)                                                                               \
+  template void bbicg_p<R>(BBlas&, const cplx<double>*, const cplx<R>*, const cplx<R>*,       \
+                           cplx<R>*, const BShape&);                                          \
+  template void bbicg_s<R>(BBlas&, const cplx<double>*, const cplx<R>*, const cplx<R>*,       \
+                           cplx<R>*, const BShape&);                                          \
+  template void bbicg_xr<R>(BBlas&, const cplx<double>*, const cplx<R>*, const cplx<R>*,      \
+                            const cplx<R>*, cplx<R>*, cplx<R>*, const BShape&);               \
+  template void bpack<R>(const cplx<R>*, size_t, cplx<R>*, const BShape&, hipStream_t);       \
+  template void bunpack<R>(const cplx<R>*, cplx<R>*, size_t, const BShape&, hipStream_t);
 INST(double)
 INST(float)
 #undef INST

@@ -68,4 +68,27 @@ void bsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, size_t ldv, int k, const
 template <typename Rs, typename Rd>
 void bconvert(BBlas& b, const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const double* a);
 
+// BiCGSTAB updates with per-rhs coefficients c [3][nrhs] on the host (rows 0
+// and 1 as named, row 2 .re = 1 for a rhs still iterating, 0 for one that
+// has stopped: its vectors are left as they are), each one pass:
+//   bbicg_p : p = r + c0 (p - c1 v)
+//   bbicg_s : s = r - c0 v
+//   bbicg_xr: x += c0 p + c1 s; r = s - c1 t
+template <typename R>
+void bbicg_p(BBlas& b, const cplx<double>* c, const cplx<R>* r, const cplx<R>* v, cplx<R>* p,
+             const BShape& sh);
+template <typename R>
+void bbicg_s(BBlas& b, const cplx<double>* c, const cplx<R>* r, const cplx<R>* v, cplx<R>* s,
+             const BShape& sh);
+template <typename R>
+void bbicg_xr(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* s,
+              const cplx<R>* t, cplx<R>* x, cplx<R>* r, const BShape& sh);
+
+// a set of sh.nrhs vectors [rhs][ld] (ld = sites * dof) into one batched
+// vector [site][rhs][dof], and back
+template <typename R>
+void bpack(const cplx<R>* set, size_t ld, cplx<R>* out, const BShape& sh, hipStream_t s);
+template <typename R>
+void bunpack(const cplx<R>* in, cplx<R>* set, size_t ld, const BShape& sh, hipStream_t s);
+
 }  // namespace mgc

@@ -338,4 +338,141 @@ class BMinRes : public BSolver<R> {
   cplx<double>* d_out_ = nullptr;
 };
 
+// BiCGSTAB for nrhs systems in lockstep: per rhs the recurrence of the
+// single-rhs BiCgStab (the BiCGSTAB solver type of include/Helpers.h:51-53),
+// with its stopping rules per rhs: |r| <= tol |b|, maxit, a breakdown
+// (rho, omega or <rh, v> zero), and for b = 0 (null vectors relaxed from x)
+// exactly maxit steps with no convergence test.  A rhs that has stopped keeps
+// its x and residual while the others go on.  Three host reductions per step
+// ((<rh, r>, |r|^2), <rh, v>, (<s, t>, |t|^2)) serve the whole batch.
+template <typename R>
+class BBiCgStab : public BSolver<R> {
+ public:
+  explicit BBiCgStab(const BShape& sh) : sh_(sh) {
+    MGC_CHECK(sh.nrhs >= 1 && sh.nrhs <= BBlas::RMAX, MGC_ERR_ARG, "batch of 1..64 rhs");
+    // r and rh adjacent, so that one two-row reduction gives <r, r> and <rh, r>
+    MGC_HIP(hipMalloc(&w_, (size_t)6 * sh.n * sizeof(cplx<R>)));
+    MGC_HIP(hipMalloc(&d_out_, (size_t)3 * sh.nrhs * sizeof(cplx<double>)));
+  }
+  ~BBiCgStab() {
+    if (w_) (void)hipFree(w_);
+    if (d_out_) (void)hipFree(d_out_);
+  }
+  BBiCgStab(const BBiCgStab&) = delete;
+  BBiCgStab& operator=(const BBiCgStab&) = delete;
+
+  BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>*, const cplx<R>* b, cplx<R>* x,
+                     bool x_zero, double tol, int maxit) override {
+    typedef std::complex<double> C;
+    hipStream_t st_ = bl.s;
+    const size_t n = sh_.n;
+    const int nr = sh_.nrhs;
+    cplx<R>*r = w_, *rh = w_ + n, *p = w_ + 2 * n, *v = w_ + 3 * n, *sv = w_ + 4 * n,
+           *t = w_ + 5 * n;
+    BKrylovStats st;
+    st.rel_res.assign(nr, 0.0);
+    st.converged.assign(nr, 0);
+    st.iters_rhs.assign(nr, 0);
+    std::vector<double> b2(nr), r2(nr);
+    std::vector<int> live(nr, 1), relax(nr);
+    std::vector<C> rho(nr, 1.0), alpha(nr, 1.0), omega(nr, 1.0), rho1(nr);
+    std::vector<cplx<double>> c(3 * (size_t)nr);
+    bnorm2<R>(bl, b, sh_, b2.data());
+    for (int q = 0; q < nr; ++q) relax[q] = !(b2[q] > 0.0);
+    if (x_zero) {
+      vzero<R>(x, n, st_);
+      vcopy<R>(b, r, n, st_);
+      for (int q = 0; q < nr; ++q)
+        if (relax[q]) {  // b = 0 from x = 0: x = 0 at once
+          live[q] = 0;
+          st.converged[q] = 1;
+        }
+    } else {
+      A.apply(x, t, st_);
+      ++st.applies;
+      vsub<R>(b, t, r, n, st_);
+    }
+    vcopy<R>(r, rh, n, st_);
+    vzero<R>(p, n, st_);
+    vzero<R>(v, n, st_);
+    auto cd = [](C z) { return cplx<double>{z.real(), z.imag()}; };
+    auto dc = [](const double* h) { return C(h[0], h[1]); };
+    // (<r, r>, <rh, r>) per rhs
+    auto fetch_r = [&]() {
+      bdots_dev<R>(bl, r, n, 2, r, sh_, false, d_out_);
+      bfetch(bl, reinterpret_cast<const double*>(d_out_), 4 * nr);
+      for (int q = 0; q < nr; ++q) {
+        r2[q] = bl.h_res[2 * q];
+        rho1[q] = dc(bl.h_res + 2 * (nr + q));
+      }
+    };
+    auto flags = [&]() {
+      for (int q = 0; q < nr; ++q) c[2 * (size_t)nr + q] = {live[q] ? 1.0 : 0.0, 0.0};
+    };
+    fetch_r();
+    while (true) {
+      bool any = false;
+      for (int q = 0; q < nr; ++q) {
+        if (!live[q]) continue;
+        st.rel_res[q] = relax[q] ? std::sqrt(r2[q]) : std::sqrt(r2[q] / b2[q]);
+        st.iters_rhs[q] = st.iters;
+        if (!relax[q] && r2[q] <= tol * tol * b2[q]) {
+          st.converged[q] = 1;
+          live[q] = 0;
+        } else if (st.iters >= maxit || std::abs(rho1[q]) == 0.0 || std::abs(omega[q]) == 0.0) {
+          live[q] = 0;  // iteration limit or breakdown
+        }
+        any = any || live[q];
+      }
+      if (!any) break;
+      for (int q = 0; q < nr; ++q) {
+        if (!live[q]) continue;
+        const C beta = (rho1[q] / rho[q]) * (alpha[q] / omega[q]);
+        rho[q] = rho1[q];
+        c[q] = cd(beta);
+        c[nr + q] = cd(omega[q]);
+      }
+      flags();
+      bbicg_p<R>(bl, c.data(), r, v, p, sh_);  // p = r + beta (p - omega v)
+      A.apply(p, v, st_);
+      bdots_dev<R>(bl, rh, n, 1, v, sh_, false, d_out_);
+      bfetch(bl, reinterpret_cast<const double*>(d_out_), 2 * nr);
+      for (int q = 0; q < nr; ++q) {
+        if (!live[q]) continue;
+        const C rhv = dc(bl.h_res + 2 * q);
+        if (std::abs(rhv) == 0.0) {
+          live[q] = 0;  // breakdown: x and r as they were
+          continue;
+        }
+        alpha[q] = rho[q] / rhv;
+        c[q] = cd(alpha[q]);
+      }
+      flags();
+      bbicg_s<R>(bl, c.data(), r, v, sv, sh_);  // s = r - alpha v
+      A.apply(sv, t, st_);
+      st.applies += 2;
+      // (<t, t>, <s, t>) per rhs: omega = <t, s> / |t|^2 = conj(<s, t>) / |t|^2
+      bdots_dev<R>(bl, sv, n, 1, t, sh_, true, d_out_);
+      bfetch(bl, reinterpret_cast<const double*>(d_out_), 4 * nr);
+      for (int q = 0; q < nr; ++q) {
+        if (!live[q]) continue;
+        const C ts = std::conj(dc(bl.h_res + 2 * q));
+        const double tt = bl.h_res[2 * (nr + q)];
+        omega[q] = tt > 0.0 ? ts / tt : C(0.0);
+        c[q] = cd(alpha[q]);
+        c[nr + q] = cd(omega[q]);
+      }
+      bbicg_xr<R>(bl, c.data(), p, sv, t, x, r, sh_);  // x += alpha p + omega s; r = s - omega t
+      ++st.iters;
+      fetch_r();
+    }
+    return st;
+  }
+
+ private:
+  BShape sh_;
+  cplx<R>* w_ = nullptr;  // r, rh, p, v, s, t
+  cplx<double>* d_out_ = nullptr;
+};
+
 }  // namespace mgc

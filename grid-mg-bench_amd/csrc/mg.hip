@@ -274,6 +274,7 @@ struct MG : MGBase {
     ok(mgc_random_gaussian(tv, (size_t)nb * n, prec_of<R>(), p.seed, 1000 + l, (void*)s));
     auto solver = make_solver<R>(p.setup_solver, n, std::max(1, std::min(p.setup_restart, 63)));
     const bool nullv = p.setup_basis_type == MGC_BASIS_NULL_VECTORS;
+    const int nbatch = setup_batch(l);
     DevBuf<cplx<R>> src(n), e(n);
     for (int it = 0; it < p.setup_iters; ++it) {
       if (p.setup_pre_orthonormalise) orthonormalise_set(tv, nb, n, s);
@@ -283,6 +284,7 @@ struct MG : MGBase {
         MGC_CHECK(nv > 0.0 && std::isfinite(nv), MGC_ERR_STATE,
                   "test vector with zero or non-finite norm in the setup");
         vscale<R>(1.0 / nv, v, n, s);
+        if (nbatch > 1) continue;  // solved below, nbatch at a time
         if (!nullv) {
           vcopy<R>(v, src.p, n, s);
           solver->solve(bl, *lv.op, nullptr, src.p, v, true, p.setup_tol, p.setup_solve_iters);
@@ -298,8 +300,94 @@ struct MG : MGBase {
           vaxpy<R>({1.0, 0.0}, e.p, v, n, s);
         }
       }
+      if (nbatch > 1) generate_batched(l, tv, nbatch, s);
       if (p.setup_post_orthonormalise) orthonormalise_set(tv, nb, n, s);
     }
+  }
+
+  // Right-hand sides per lockstep batch of the set-up solves on level l: 1
+  // (one at a time) unless setup_nrhs asks for more and the level has a
+  // multi-rhs operator (the fine operator up to its max_nrhs; a coarse level
+  // of an FP32 hierarchy, through the FP32 multi-rhs stencil)
+  int setup_batch(int l) const {
+    int b = std::min(p.setup_nrhs, std::min(p.nb[l], (int)BBlas::RMAX));
+    if (l == 0) b = std::min(b, fsrc->lop.max_nrhs);
+    if (l > 0 && !std::is_same<R, float>::value) b = 1;
+    return std::max(b, 1);
+  }
+
+  std::unique_ptr<BLinOp<R>> batched_op(int l, int nr) {
+    if (l == 0) return std::make_unique<BFineLin<R>>(fsrc, nr);
+    if constexpr (std::is_same<R, float>::value)
+      return std::make_unique<BCoarseLin>(L[l - 1]->copR, nr, MGC_F32);
+    throw Error(MGC_ERR_PRECISION, "no multi-rhs operator on an FP64 coarse level");
+  }
+
+  std::unique_ptr<BSolver<R>> batched_solver(const BShape& sh) {
+    switch (p.setup_solver) {
+      case MGC_SOLVER_BICGSTAB: return std::make_unique<BBiCgStab<R>>(sh);
+      case MGC_SOLVER_GMRES:
+      case MGC_SOLVER_FGMRES:
+        return std::make_unique<BGmres<R>>(sh, std::max(1, std::min(p.setup_restart, 63)),
+                                           p.setup_solver == MGC_SOLVER_FGMRES);
+      case MGC_SOLVER_MR: return std::make_unique<BMinRes<R>>(sh);
+    }
+    throw Error(MGC_ERR_ARG, "unknown solver type " + std::to_string(p.setup_solver));
+  }
+
+  // The solves of one test-vector round (the inner loop of Generate,
+  // include/BasisVectors.h:89-139) on nb vectors that are already normalised,
+  // in lockstep batches of up to nbatch: each batch is gathered into one
+  // multi-rhs vector [site][rhs][dof], solved by the batched form of the
+  // set-up solver (per rhs the recurrence and stopping rules of the single-rhs
+  // solver) and scattered back.  Batches are as even as they can be, larger
+  // ones first.
+  void generate_batched(int l, cplx<R>* tv, int nbatch, hipStream_t s) {
+    Level<R>& lv = *L[l];
+    const size_t n = lv.n;
+    const int nb = p.nb[l];
+    const int dof = l == 0 ? 12 : L[l - 1]->copR->N;
+    const bool nullv = p.setup_basis_type == MGC_BASIS_NULL_VECTORS;
+    const int nchunk = (nb + nbatch - 1) / nbatch;
+    DevBuf<cplx<R>> xb((size_t)nbatch * n), bp((size_t)nbatch * n), eb;
+    if (nullv && p.setup_solver != MGC_SOLVER_BICGSTAB) eb.alloc((size_t)nbatch * n);
+    BBlas bb(s);
+    std::unique_ptr<BLinOp<R>> op;
+    std::unique_ptr<BSolver<R>> solver;
+    int k0 = 0;
+    for (int c = 0; c < nchunk; ++c) {
+      const int nr = nb / nchunk + (c < nb % nchunk ? 1 : 0);
+      if (!op || op->sh.nrhs != nr) {
+        solver.reset();  // the previous batch size's workspaces first
+        op = batched_op(l, nr);
+        op->sh.n = n * nr;
+        op->sh.nrhs = nr;
+        op->sh.dof = dof;
+        solver = batched_solver(op->sh);
+      }
+      const BShape& sh = op->sh;
+      cplx<R>* set = tv + (size_t)k0 * n;
+      if (!nullv) {
+        // A psi = v from psi = 0
+        bpack<R>(set, n, bp.p, sh, s);
+        solver->solve(bb, *op, nullptr, bp.p, xb.p, true, p.setup_tol, p.setup_solve_iters);
+      } else if (p.setup_solver == MGC_SOLVER_BICGSTAB) {
+        // rhs 0: BiCGSTAB relaxes psi = v for exactly setup_solve_iters steps
+        bpack<R>(set, n, xb.p, sh, s);
+        vzero<R>(bp.p, sh.n, s);
+        solver->solve(bb, *op, nullptr, bp.p, xb.p, false, p.setup_tol, p.setup_solve_iters);
+      } else {
+        // A psi = 0 from psi = v as psi = v + e with A e = -A v
+        bpack<R>(set, n, xb.p, sh, s);
+        op->apply(xb.p, bp.p, s);
+        vscale<R>(-1.0, bp.p, sh.n, s);
+        solver->solve(bb, *op, nullptr, bp.p, eb.p, true, p.setup_tol, p.setup_solve_iters);
+        vaxpy<R>({1.0, 0.0}, eb.p, xb.p, sh.n, s);
+      }
+      bunpack<R>(xb.p, set, n, sh, s);
+      k0 += nr;
+    }
+    MGC_HIP(hipStreamSynchronize(s));  // the batch buffers are freed on return
   }
 
   // Aggregation::Create (copy + block orthonormalisation) from the level's
@@ -743,6 +831,7 @@ int mgc_mg_params_default(mgc_mg_params* p) {
     p->smoother_solver = MGC_SOLVER_GMRES;
     p->coarsest_solver = MGC_SOLVER_GMRES;
     p->graph = 0;
+    p->setup_nrhs = 0;
     p->smoother_tol = 1e-3;
     p->smoother_iters = 4;
     p->smoother_restart = 4;
@@ -774,6 +863,8 @@ static void check_params(const mgc_mg_params* params) {
   MGC_CHECK(params->setup_refine_iters >= 0 && params->setup_iters >= 0, MGC_ERR_ARG,
             "negative setup iteration count");
   MGC_CHECK(params->graph >= 0 && params->graph <= 2, MGC_ERR_ARG, "graph must be 0, 1 or 2");
+  MGC_CHECK(params->setup_nrhs >= 0 && params->setup_nrhs <= 64, MGC_ERR_ARG,
+            "setup_nrhs must be 0..64");
   MGC_CHECK(!params->graph || (params->smoother_solver == MGC_SOLVER_GMRES &&
                                params->coarsest_solver == MGC_SOLVER_GMRES),
             MGC_ERR_ARG, "graph mode runs GMRES smoothers and a GMRES coarsest solver");

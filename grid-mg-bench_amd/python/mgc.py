@@ -44,7 +44,7 @@ class MGParams(ctypes.Structure):
                 ("setup_pre_orthonormalise", _c.c_int), ("setup_post_orthonormalise", _c.c_int),
                 ("setup_refine_iters", _c.c_int), ("setup_recursive_updates", _c.c_int),
                 ("smoother_solver", _c.c_int), ("coarsest_solver", _c.c_int),
-                ("graph", _c.c_int)]
+                ("graph", _c.c_int), ("setup_nrhs", _c.c_int)]
 
 
 LINOP_APPLY = ctypes.CFUNCTYPE(_c.c_int, _c.c_void_p, _c.c_int, _c.c_int, _c.c_void_p,

@@ -460,6 +460,15 @@ typedef struct mgc_mg_params {
                               coarsest steps) and no host synchronisation; 2 = the same
                               captured once into a HIP graph and replayed (bit-equal to 1).
                               1 and 2 need GMRES smoothers and a created (non-null) stream */
+  int setup_nrhs;          /* test vectors solved together in the setup (no counterpart in the
+                              reference, whose Generate solves them one at a time):
+                              0 or 1 (default) = one at a time; k = 2..64 = in lockstep
+                              batches of up to k through the multi-rhs operator (the fine
+                              operator up to its max_nrhs; coarse levels of an MGC_F32
+                              hierarchy through the FP32 multi-rhs stencil; coarse levels of
+                              an MGC_F64 hierarchy stay one at a time).  Per vector the set-up
+                              solver's own recurrence and stopping rules, so the basis equals
+                              the one-at-a-time basis up to rounding */
 } mgc_mg_params;
 
 typedef struct mgc_solve_stats {

@@ -64,6 +64,8 @@ def test_multigrid_parameter_checks_without_device():
         (dict(setup_solver=99), b"unknown setup or coarsest solver type"),
         (dict(setup_basis_type=7), b"unknown basis-vector type"),
         (dict(setup_iters=-1), b"negative setup iteration count"),
+        (dict(setup_nrhs=65), b"setup_nrhs must be 0..64"),
+        (dict(setup_nrhs=-2), b"setup_nrhs must be 0..64"),
     ]
     for kw, msg in cases:
         p = mgc.mg_params(nlevels=2, block=[(2, 2, 2, 2)], nb=[4], **kw)

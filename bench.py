@@ -650,6 +650,8 @@ def main() -> int:
                     help="only run the Wilson-clover apply legs")
     ap.add_argument("--only-coarsen", action="store_true",
                     help="only run the CoarsenOperator / block-orthonormalisation leg")
+    ap.add_argument("--mg-forms", default="fp64_plain,mixed_fp32,mixed_fp32_graph",
+                    help="forms of the multigrid leg to run (comma-separated)")
     ap.add_argument("--only-mg", action="store_true",
                     help="run only the multigrid-solve leg and print its result (for profiling)")
     ap.add_argument("--mrhs-variant", type=int, default=-1,
@@ -710,7 +712,8 @@ def main() -> int:
         d = tuple(int(v) for v in args.mg_dims.split(","))
         log(f"leg: multigrid solve {d}")
         print(json.dumps({"mg_solve": mg_leg(mgc, stream, d, [(4, 4, 4, 4), (4, 4, 4, 4)],
-                                             [16, 16])}), flush=True)
+                                             [16, 16], forms=tuple(args.mg_forms.split(",")))}),
+              flush=True)
         return 0
     legs = {}
     if args.only_wilson:

@@ -134,11 +134,27 @@ __device__ inline bool site_in_mask(const Lat4& blk, int xb, int kind, int p) {
 // gridDim.z row ranges of rows_per rows; the workgroups then write partial
 // sums to part[((xc * gridDim.y + y) * gridDim.z + z)][RC][2][nb], which
 // k_project_reduce adds in a fixed order.
+// With W = 2 (FP32, even nb) a thread takes the two adjacent basis columns
+// i = 2 i2, 2 i2 + 1 of each row in one 16-byte load, as the FP64 form reads
+// one column in 16 bytes; each column's sum keeps the row order q, q + G, ...
 // ---------------------------------------------------------------------------
 constexpr int RC = 4;  // right-hand sides per workgroup
 
+// W basis entries of one row from p (W = 2: FP32 only, p 16-byte aligned)
+template <typename R, int W>
+__device__ __forceinline__ void project_row(const cplx<R>* p, cplx<R> (&o)[W]) {
+  if constexpr (W == 1) {
+    o[0] = p[0];
+  } else {
+    static_assert(W == 2 && sizeof(R) == 4, "paired columns are an FP32 form");
+    const float4 t = *reinterpret_cast<const float4*>(p);
+    o[0] = {t.x, t.y};
+    o[1] = {t.z, t.w};
+  }
+}
+
 // RCT = right-hand sides a workgroup actually holds (1 for a single rhs)
-template <typename R, int RCT>
+template <typename R, int RCT, int W = 1>
 __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
                                                  const cplx<R>* __restrict__ f,
                                                  cplx<R>* __restrict__ c,
@@ -151,12 +167,15 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
   const int r0 = blockIdx.y * RC;
   const int nr = min(RC, nrhs - r0);
   const int tid = threadIdx.x;
-  const int G = blockDim.x / nb;
-  const int i = tid % nb, g = tid / nb;
+  const int nw = nb / W;  // threads per row
+  const int G = blockDim.x / nw;
+  const int i = (tid % nw) * W, g = tid / nw;  // first column of this thread
   const int half = Ef / 2;
-  cplx<R> acc[RCT][2];
+  cplx<R> acc[RCT][2][W];
 #pragma unroll
-  for (int r = 0; r < RCT; ++r) acc[r][0] = acc[r][1] = {R(0), R(0)};
+  for (int r = 0; r < RCT; ++r)
+#pragma unroll
+    for (int w = 0; w < W; ++w) acc[r][0][w] = acc[r][1][w] = {R(0), R(0)};
   if (g < G) {
     const int rows = Vb * Ef;
     const cplx<R>* Ub = U + (size_t)xc * rows * nb;
@@ -166,9 +185,9 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
     if (kind == 0) {
       // unmasked: PU rows per step, their loads issued together (rows are
       // still accumulated in the order q, q + G, q + 2G, ...)
-      constexpr int PU = (sizeof(R) == 4 ? 8 : 4) * (RCT == 1 ? 2 : 1);
+      constexpr int PU = (sizeof(R) * W == 4 ? 8 : 4) * (RCT == 1 ? 2 : 1);
       for (; q + (PU - 1) * G < qend; q += PU * G) {
-        cplx<R> uu[PU];
+        cplx<R> uu[PU][W];
         const cplx<R>* fp[PU];
         bool lo[PU];
 #pragma unroll
@@ -176,7 +195,7 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
           const int qq = q + v * G;
           const int xb = qq / Ef, e = qq - xb * Ef;
           lo[v] = e < half;
-          uu[v] = Ub[(size_t)qq * nb + i];
+          project_row<R, W>(Ub + (size_t)qq * nb + i, uu[v]);
           fp[v] = f + ((size_t)bf[xb] * nrhs + r0) * Ef + e;
         }
         cplx<R> xv[PU][RCT];
@@ -188,31 +207,40 @@ __global__ __launch_bounds__(256) void k_project(const cplx<R>* __restrict__ U,
         for (int v = 0; v < PU; ++v)
 #pragma unroll
           for (int r = 0; r < RCT; ++r)
-            if (r < nr) {
-              if (lo[v]) cmac_conj(acc[r][0], uu[v], xv[v][r]);
-              else cmac_conj(acc[r][1], uu[v], xv[v][r]);
-            }
+            if (r < nr)
+#pragma unroll
+              for (int w = 0; w < W; ++w) {
+                if (lo[v]) cmac_conj(acc[r][0][w], uu[v][w], xv[v][r]);
+                else cmac_conj(acc[r][1][w], uu[v][w], xv[v][r]);
+              }
       }
     }
     for (; q < qend; q += G) {
       const int xb = q / Ef, e = q - xb * Ef;
       if (kind != 0 && !site_in_mask(blk, xb, kind, point)) continue;
       const int k = e >= half;
-      const cplx<R> u = Ub[(size_t)q * nb + i];
+      cplx<R> u[W];
+      project_row<R, W>(Ub + (size_t)q * nb + i, u);
       const cplx<R>* fr = f + ((size_t)bf[xb] * nrhs + r0) * Ef + e;
 #pragma unroll
       for (int r = 0; r < RCT; ++r)
         if (r < nr) {
           const cplx<R> x = fr[(size_t)r * Ef];
-          if (k == 0) cmac_conj(acc[r][0], u, x);
-          else cmac_conj(acc[r][1], u, x);
+#pragma unroll
+          for (int w = 0; w < W; ++w) {
+            if (k == 0) cmac_conj(acc[r][0][w], u[w], x);
+            else cmac_conj(acc[r][1][w], u[w], x);
+          }
         }
     }
 #pragma unroll
     for (int r = 0; r < RCT; ++r)
       if (r < nr)
 #pragma unroll
-        for (int k = 0; k < 2; ++k) s_red[(((size_t)g * RC + r) * 2 + k) * nb + i] = acc[r][k];
+        for (int k = 0; k < 2; ++k)
+#pragma unroll
+          for (int w = 0; w < W; ++w)
+            s_red[(((size_t)g * RC + r) * 2 + k) * nb + i + w] = acc[r][k][w];
   }
   __syncthreads();
   for (int o = tid; o < nr * 2 * nb; o += blockDim.x) {
@@ -923,7 +951,21 @@ void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int n
                                                             nb);
     } else {
       const cplx<float>* b32 = agg_basis32(a, s);
-      if (nrhs == 1)
+      // even nb: two adjacent columns per thread in one 16-byte load
+      const bool paired = nb % 2 == 0;
+      const int block2 = (256 / (nb / 2 > 0 ? nb / 2 : 1)) * (nb / 2);
+      const size_t sh2 = (size_t)(paired ? block2 / (nb / 2) : 0) * RC * 2 * nb * es;
+      if (paired && nrhs == 1)
+        k_project<float, 1, 2><<<grid, block2, sh2, s>>>(b32, (const cplx<float>*)in,
+                                                         (cplx<float>*)out, a->blk_fine, nb,
+                                                         a->Vb, a->Ef, nrhs, a->blk, kind, point,
+                                                         rows_per, (cplx<float>*)part);
+      else if (paired)
+        k_project<float, RC, 2><<<grid, block2, sh2, s>>>(b32, (const cplx<float>*)in,
+                                                          (cplx<float>*)out, a->blk_fine, nb,
+                                                          a->Vb, a->Ef, nrhs, a->blk, kind, point,
+                                                          rows_per, (cplx<float>*)part);
+      else if (nrhs == 1)
         k_project<float, 1><<<grid, block, sh, s>>>(b32, (const cplx<float>*)in,
                                                     (cplx<float>*)out, a->blk_fine, nb, a->Vb,
                                                     a->Ef, nrhs, a->blk, kind, point, rows_per,

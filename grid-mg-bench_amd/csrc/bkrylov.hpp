@@ -131,7 +131,7 @@ class BGmres : public BSolver<R> {
         // classical Gram-Schmidt with selective re-orthogonalisation (as the
         // single-rhs solver): h1 and |w|^2 at rows 0 .. j+1 of d_h_; the
         // second projection at rows j+2 .. 2j+3 when some live rhs has
-        // |w'|^2 = |w|^2 - |h1|^2 < |w|^2 / 2
+        // |w'|^2 = |w|^2 - |h1|^2 < eta |w|^2
         cplx<double>* dh1 = d_h_;
         cplx<double>* dh2 = d_h_ + (size_t)(j + 2) * nr;
         cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
@@ -147,7 +147,7 @@ class BGmres : public BSolver<R> {
           }
           wn2[r] = bl.h_res[2 * ((size_t)(j + 1) * nr + r)];
           hn[r] = wn2[r] - sum1[r] > 0.0 ? std::sqrt(wn2[r] - sum1[r]) : 0.0;
-          if (live[r] && sum1[r] > 0.5 * wn2[r]) reorth = true;
+          if (live[r] && wn2[r] - sum1[r] < eta_ * wn2[r]) reorth = true;
         }
         if (!reorth) {
           bsubnorm_dev<R>(dh1, V_, n, j + 1, w_, vnext, sh_, s);
@@ -243,10 +243,17 @@ class BGmres : public BSolver<R> {
 
   const BShape& shape() const { return sh_; }
 
+  // re-orthogonalisation threshold eta, as Gmres::set_reorth_eta
+  void set_reorth_eta(double eta) {
+    MGC_CHECK(eta > 0.0 && eta <= 1.0, MGC_ERR_ARG, "re-orthogonalisation threshold in (0, 1]");
+    eta_ = eta;
+  }
+
  private:
   BShape sh_;
   int m_;
   bool flexible_;
+  double eta_ = 0.5;
   cplx<R>* V_ = nullptr;
   cplx<R>* Z_ = nullptr;
   cplx<R>* w_ = nullptr;

@@ -125,7 +125,7 @@ class Gmres : public Solver<R> {
         // Classical Gram-Schmidt with device-resident coefficients and
         // selective re-orthogonalisation: h1 = V^H w and |w|^2 at
         // d_h_[0 .. j+1]; the second projection (h2, |w'|^2 at
-        // d_h_[j+2 .. 2j+3]) only when |w'|^2 = |w|^2 - |h1|^2 < |w|^2 / 2.
+        // d_h_[j+2 .. 2j+3]) only when |w'|^2 = |w|^2 - |h1|^2 < eta |w|^2.
         cplx<double>* dh1 = d_h_;
         cplx<double>* dh2 = d_h_ + (j + 2);
         cplx<R>* vnext = V_ + (size_t)(j + 1) * n;
@@ -138,7 +138,7 @@ class Gmres : public Solver<R> {
         }
         const double wn2 = bl.h_res[2 * (j + 1)];
         double hn;
-        if (sum1 <= 0.5 * wn2) {
+        if (wn2 - sum1 >= eta_ * wn2) {
           // one projection suffices: v = (w - V h1) / sqrt(|w|^2 - |h1|^2)
           vsubnorm_dev<R>(dh1, V_, n, j + 1, w_, vnext, n, s);
           hn = std::sqrt(wn2 - sum1);
@@ -212,11 +212,21 @@ class Gmres : public Solver<R> {
   }
 
   size_t n() const { return n_; }
+  // Re-orthogonalisation threshold eta (default 1/2, the Daniel-Gragg-
+  // Kaufman-Stewart test).  One classical projection leaves the new vector
+  // orthogonal to the basis to about u / sqrt(eta) (u the unit roundoff) and
+  // its Pythagorean norm accurate to about u / eta relative, so FP64 solves
+  // can take a small eta and skip most second projections.
+  void set_reorth_eta(double eta) {
+    MGC_CHECK(eta > 0.0 && eta <= 1.0, MGC_ERR_ARG, "re-orthogonalisation threshold in (0, 1]");
+    eta_ = eta;
+  }
 
  private:
   size_t n_;
   int m_;
   bool flexible_;
+  double eta_ = 0.5;
   cplx<R>* V_ = nullptr;
   cplx<R>* Z_ = nullptr;
   cplx<R>* w_ = nullptr;

@@ -45,6 +45,11 @@ template <typename R> constexpr int prec_of() { return sizeof(R) == 8 ? MGC_F64 
 // up to this threshold one FP64 projection leaves the new vector orthogonal
 // to the basis to ~1e-14 and its Pythagorean norm accurate to ~1e-12.
 constexpr double OUTER_REORTH_ETA = 1e-4;
+// The same for the GMRES smoothers, K-cycle wrappers and coarsest solvers of
+// the preconditioner: 1e-4 in FP64; 1e-2 in FP32, where one projection then
+// leaves the few-step smoother basis orthogonal to ~1e-6 and its norms
+// accurate to ~1e-5 (the inner products are FP64 sums over FP32 vectors).
+template <typename R> constexpr double hier_reorth_eta() { return sizeof(R) == 8 ? 1e-4 : 1e-2; }
 
 static void ok(int st) {
   if (st != MGC_OK) throw Error(st, mgc_last_error());
@@ -111,10 +116,14 @@ struct CoarseLin : LinOp<R> {
 
 // the solver types of include/Helpers.h:38-56 (unpreconditioned but FGMRES)
 template <typename R>
-std::unique_ptr<Solver<R>> make_solver(int type, size_t n, int restart) {
+std::unique_ptr<Solver<R>> make_solver(int type, size_t n, int restart, double eta = 0.5) {
   switch (type) {
-    case MGC_SOLVER_GMRES: return std::make_unique<Gmres<R>>(n, restart, false);
-    case MGC_SOLVER_FGMRES: return std::make_unique<Gmres<R>>(n, restart, true);
+    case MGC_SOLVER_GMRES:
+    case MGC_SOLVER_FGMRES: {
+      auto g = std::make_unique<Gmres<R>>(n, restart, type == MGC_SOLVER_FGMRES);
+      g->set_reorth_eta(eta);
+      return g;
+    }
     case MGC_SOLVER_BICGSTAB: return std::make_unique<BiCgStab<R>>(n);
     case MGC_SOLVER_MR: return std::make_unique<MinRes<R>>(n);
   }
@@ -491,11 +500,14 @@ struct MG : MGBase {
           lv.smoother = std::make_unique<DGmres<R>>(lv.n, p.coarsest_restart, false);
         }
       } else if (l + 1 < p.nlevels) {
-        lv.smoother = make_solver<R>(p.smoother_solver, lv.n, p.smoother_restart);
+        lv.smoother = make_solver<R>(p.smoother_solver, lv.n, p.smoother_restart,
+                                     hier_reorth_eta<R>());
         if (l + 2 < p.nlevels)
-          lv.wrapper = std::make_unique<Gmres<R>>(L[l + 1]->n, p.wrapper_restart, true);
+          lv.wrapper = make_solver<R>(MGC_SOLVER_FGMRES, L[l + 1]->n, p.wrapper_restart,
+                                      hier_reorth_eta<R>());
       } else {
-        lv.smoother = make_solver<R>(p.coarsest_solver, lv.n, p.coarsest_restart);
+        lv.smoother = make_solver<R>(p.coarsest_solver, lv.n, p.coarsest_restart,
+                                     hier_reorth_eta<R>());
       }
       MGC_HIP(hipMalloc(&lv.t1, lv.n * sizeof(cplx<R>)));
       MGC_HIP(hipMalloc(&lv.t2, lv.n * sizeof(cplx<R>)));
@@ -714,7 +726,9 @@ struct MG : MGBase {
         if (type == MGC_SOLVER_MR) return std::make_unique<BMinRes<R>>(sh);
         MGC_CHECK(type == MGC_SOLVER_GMRES, MGC_ERR_ARG,
                   "the batched solve takes GMRES or MR smoothers and coarsest solvers");
-        return std::make_unique<BGmres<R>>(sh, restart, false);
+        auto g = std::make_unique<BGmres<R>>(sh, restart, false);
+        g->set_reorth_eta(hier_reorth_eta<R>());
+        return g;
       };
       if (l + 1 < p.nlevels) {
         st->smoother = bsolver(p.smoother_solver, p.smoother_restart);
@@ -726,8 +740,10 @@ struct MG : MGBase {
       }
       BL.push_back(std::move(st));
     }
-    for (int l = 0; l + 2 < p.nlevels; ++l)
+    for (int l = 0; l + 2 < p.nlevels; ++l) {
       BL[l]->wrapper = std::make_unique<BGmres<R>>(BL[l + 1]->op->sh, p.wrapper_restart, true);
+      BL[l]->wrapper->set_reorth_eta(hier_reorth_eta<R>());
+    }
     for (int l = 0; l < p.nlevels; ++l) bprecs.push_back(std::make_unique<BLevelPrec>(this, l));
     MGC_HIP(hipMalloc(&bin, BL[0]->op->sh.n * sizeof(cplx<R>)));
     MGC_HIP(hipMalloc(&bout, BL[0]->op->sh.n * sizeof(cplx<R>)));

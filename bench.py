@@ -340,7 +340,8 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10,
     FGMRES(20) outer in FP64 with true-residual recomputation, hierarchy in
     FP64 ("plain form") and in FP32 ("mixed"), the reference's default setup
     (BiCGSTAB 1e-3 / 100 test vectors).  "mixed_fp32_graph" runs the K-cycle
-    as a captured HIP graph of device-scalar GMRES with fixed step counts.
+    as a captured HIP graph of device-scalar GMRES with fixed step counts
+    (smoother 4, K-cycle wrapper 1, coarsest 6).
     Setup, the first preconditioner application (graph capture) and the
     solve timed apart.  The plain form generates its test vectors one at a
     time, as the reference does; the mixed forms solve them setup_nrhs at a
@@ -375,8 +376,10 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10,
             mg.setup(stream)
             t_setup1 = time.perf_counter() - t0
             mg.close()
+        # graph form: fixed step counts 4 (smoother), 1 (K-cycle wrapper), 6 (coarsest)
+        steps = dict(wrapper_restart=1, coarsest_restart=6) if graph else {}
         p = mgc.mg_params(nlevels=len(blocks) + 1, block=blocks, nb=nbs, precision=prec,
-                          graph=graph, setup_nrhs=snrhs)
+                          graph=graph, setup_nrhs=snrhs, **steps)
         mg = mgc.Multigrid(fop, p)
         t0 = time.perf_counter()
         mg.setup(stream)
@@ -399,6 +402,9 @@ def mg_leg(mgc, stream, dims, blocks, nbs, tol=1e-10,
                      "precond_applies": st["precond_applies"]}
         if t_setup1 is not None:
             out[name]["setup_one_at_a_time_s"] = t_setup1
+        if graph:
+            out[name]["graph_steps"] = {"smoother": p.smoother_iters, "wrapper": p.wrapper_restart,
+                                        "coarsest": p.coarsest_restart}
         mg.close()
     for buf in (g, b, x):
         buf.close()

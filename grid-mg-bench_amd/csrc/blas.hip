@@ -128,6 +128,24 @@ __global__ __launch_bounds__(256) void k_dot_part(const cplx<R>* V, size_t ldv,
   }
 }
 
+// k_dot_final, skipped (whole grid, nothing written) when *skip is nonzero
+__global__ __launch_bounds__(256) void k_dot_final_unless(const int* __restrict__ skip,
+                                                          const double* part, int nparts,
+                                                          double* res) {
+  if (*skip) return;
+  const int j = blockIdx.x;
+  double re = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+    re += part[((size_t)j * nparts + i) * 2];
+    im += part[((size_t)j * nparts + i) * 2 + 1];
+  }
+  const double2 t = block_sum2(re, im);
+  if (threadIdx.x == 0) {
+    res[2 * j] = t.x;
+    res[2 * j + 1] = t.y;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_dot_final(const double* part, int nparts, double* res) {
   const int j = blockIdx.x;
   double re = 0.0, im = 0.0;
@@ -284,7 +302,9 @@ constexpr int MSUB_STAGED = 16;
 template <typename R, int KT, bool PAIRED>
 __global__ __launch_bounds__(256) void k_msub_mdot(const cplx<double>* __restrict__ c,
                                                    const cplx<R>* V, size_t ldv, int k,
-                                                   cplx<R>* w, size_t n, double* part) {
+                                                   cplx<R>* w, size_t n, double* part,
+                                                   const int* __restrict__ skip) {
+  if (skip && *skip) return;  // uniform: the whole grid returns, w and part untouched
   constexpr int KS = KT > 8 ? MSUB_STAGED : 0;
   using Slot = typename std::conditional<PAIRED, float4, cplx<R>>::type;
   __shared__ Slot s_v[KS > 0 ? KS : 1][256];
@@ -569,29 +589,32 @@ template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, si
 }
 template <typename R, int KT>
 static void launch_msub_mdot(Blas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv,
-                             int k, cplx<R>* w, size_t n, int np) {
+                             int k, cplx<R>* w, size_t n, int np, const int* skip) {
   const dim3 g(np);
   if constexpr (std::is_same<R, float>::value) {
     if (pairable(V, w, ldv, n)) {
-      k_msub_mdot<R, KT, true><<<g, 256, 0, b.s>>>(d_c, V, ldv, k, w, n, b.d_part);
+      k_msub_mdot<R, KT, true><<<g, 256, 0, b.s>>>(d_c, V, ldv, k, w, n, b.d_part, skip);
       return;
     }
   }
-  k_msub_mdot<R, KT, false><<<g, 256, 0, b.s>>>(d_c, V, ldv, k, w, n, b.d_part);
+  k_msub_mdot<R, KT, false><<<g, 256, 0, b.s>>>(d_c, V, ldv, k, w, n, b.d_part, skip);
 }
 
 template <typename R> bool vmultisub_dot_dev(Blas& b, const cplx<double>* d_c, const cplx<R>* V,
                                              size_t ldv, int k, cplx<R>* w, size_t n,
-                                             cplx<double>* d_h) {
+                                             cplx<double>* d_h, const int* skip) {
   if (k < 1 || k > 24) return false;
   // the grid of vmultidot_dev, so that the partial sums are the same
   const bool paired = std::is_same<R, float>::value && pairable(V, w, ldv, n);
   const int np = nparts_for(paired ? n / 2 : n);
-  if (k <= 8) launch_msub_mdot<R, 8>(b, d_c, V, ldv, k, w, n, np);
-  else if (k <= 16) launch_msub_mdot<R, 16>(b, d_c, V, ldv, k, w, n, np);
-  else launch_msub_mdot<R, 24>(b, d_c, V, ldv, k, w, n, np);
+  if (k <= 8) launch_msub_mdot<R, 8>(b, d_c, V, ldv, k, w, n, np, skip);
+  else if (k <= 16) launch_msub_mdot<R, 16>(b, d_c, V, ldv, k, w, n, np, skip);
+  else launch_msub_mdot<R, 24>(b, d_c, V, ldv, k, w, n, np, skip);
   MGC_LAUNCH_CHECK();
-  k_dot_final<<<k + 1, 256, 0, b.s>>>(b.d_part, np, reinterpret_cast<double*>(d_h));
+  if (skip)
+    k_dot_final_unless<<<k + 1, 256, 0, b.s>>>(skip, b.d_part, np, reinterpret_cast<double*>(d_h));
+  else
+    k_dot_final<<<k + 1, 256, 0, b.s>>>(b.d_part, np, reinterpret_cast<double*>(d_h));
   MGC_LAUNCH_CHECK();
   return true;
 }
@@ -642,7 +665,7 @@ void fetch(Blas& b, const double* d_src, int count) {
   template void vsubnorm_dev<R>(cplx<double>*, const cplx<R>*, size_t, int, const cplx<R>*,   \
                                 cplx<R>*, size_t, hipStream_t);                              \
   template bool vmultisub_dot_dev<R>(Blas&, const cplx<double>*, const cplx<R>*, size_t, int,  \
-                                     cplx<R>*, size_t, cplx<double>*);
+                                     cplx<R>*, size_t, cplx<double>*, const int*);
 INST(double)
 INST(float)
 #undef INST

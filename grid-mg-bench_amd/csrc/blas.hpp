@@ -69,10 +69,13 @@ template <typename R> void vsubnorm_dev(cplx<double>* d_h2, const cplx<R>* V, si
 // Fused re-orthogonalisation pass: w -= sum_j d_c[j] V_j, then d_h[j] =
 // <V_j, w> for j < k and d_h[k].re = |w|^2 of the updated w, with the same
 // results as vmultisub_dev followed by vmultidot_dev(..., true, d_h) but one
-// read of V from memory.  Returns false (and does nothing) for k > 24.
+// read of V from memory.  Returns false (and does nothing) for k > 24.  With
+// skip, a device flag read by the kernels, a nonzero *skip makes the pass
+// change nothing (w and d_h as they were): a device-side decision that a
+// captured graph can replay.
 template <typename R> bool vmultisub_dot_dev(Blas& b, const cplx<double>* d_c, const cplx<R>* V,
                                              size_t ldv, int k, cplx<R>* w, size_t n,
-                                             cplx<double>* d_h);
+                                             cplx<double>* d_h, const int* skip = nullptr);
 // y = x * a with a from the device scalar d_s[0]: mode 0 a = d_s[0], mode 1
 // a = 1/sqrt(d_s[0]), mode 2 a = sqrt(d_s[0]) (a = 0 where d_s[0] <= 0 in modes
 // 1, 2); px / py, when not null, are device slots holding the vector

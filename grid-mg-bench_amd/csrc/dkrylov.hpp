@@ -6,8 +6,11 @@
 // built from it can be captured once into a HIP graph and replayed.
 //
 // Arithmetic per step is that of the host-driven Gmres: classical
-// Gram-Schmidt with the second projection always taken (the host form takes
-// it only when the norm drops below 1/sqrt(2)), the new norm by Pythagoras.
+// Gram-Schmidt with the second projection taken when one projection removed
+// more than 1 - eta of |w|^2, the new norm by Pythagoras.  The test runs on
+// the device (k_dg_reorth_test) and the second projection's kernels read its
+// flag, so the decision is replayed with the graph; without the fused pass
+// (more than 24 basis vectors) the second projection is always taken.
 // A step whose new norm is not positive marks the Krylov space exhausted;
 // the solution then uses the columns before it.
 #pragma once
@@ -26,7 +29,30 @@ struct DGmresState {
   cplx<double> negy[Blas::KMAX];  // -y, for x += sum y_i Z_i by the multi-subtract
   double r2[2];                   // |r0|^2 (from vnorm2_dev)
   int jmax;                       // Arnoldi columns in use
+  int skip;                       // 1: this step's second projection is not taken
 };
+
+// Re-orthogonalisation test of step j on the device: with h1 at dh1[0..j]
+// and |w|^2 at dh1[j+1].re, a second projection is needed when
+// |w|^2 - |h1|^2 < eta |w|^2.  Without it, dh2 takes h1 and |w|^2 (so that
+// vsubnorm_dev(dh2) forms v = (w - V h1) / |w'|) and dh1 is zeroed (so that
+// the Hessenberg column dh1 + dh2 is h1); the skipped pass leaves dh2 alone.
+__global__ void k_dg_reorth_test(DGmresState* st, cplx<double>* dh1, cplx<double>* dh2, int j,
+                                 double eta) {
+  if (threadIdx.x != 0) return;
+  double sum1 = 0.0;
+  for (int i = 0; i <= j; ++i) sum1 += dh1[i].re * dh1[i].re + dh1[i].im * dh1[i].im;
+  const double wn2 = dh1[j + 1].re;
+  const int skip = wn2 - sum1 >= eta * wn2 ? 1 : 0;
+  st->skip = skip;
+  if (skip) {
+    for (int i = 0; i <= j; ++i) {
+      dh2[i] = dh1[i];
+      dh1[i] = {0.0, 0.0};
+    }
+    dh2[j + 1] = dh1[j + 1];
+  }
+}
 
 __global__ void k_dg_begin(DGmresState* st) {
   if (threadIdx.x != 0) return;
@@ -139,7 +165,11 @@ class DGmres : public Solver<R> {
       ++st.applies;
       cplx<double>* dh2 = dh_.p + (j + 2);
       vmultidot_dev<R>(bl, V, n, j + 1, w_.p, n, true, dh1);
-      if (!vmultisub_dot_dev<R>(bl, dh1, V, n, j + 1, w_.p, n, dh2)) {
+      if (j + 1 <= 24) {
+        k_dg_reorth_test<<<1, 64, 0, s>>>(st_.p, dh1, dh2, j, eta_);
+        MGC_LAUNCH_CHECK();
+        vmultisub_dot_dev<R>(bl, dh1, V, n, j + 1, w_.p, n, dh2, &st_.p->skip);
+      } else {
         vmultisub_dev<R>(dh1, V, n, j + 1, w_.p, n, s);
         vmultidot_dev<R>(bl, V, n, j + 1, w_.p, n, true, dh2);
       }
@@ -154,10 +184,17 @@ class DGmres : public Solver<R> {
     return st;
   }
 
+  // re-orthogonalisation threshold eta, as Gmres::set_reorth_eta
+  void set_reorth_eta(double eta) {
+    MGC_CHECK(eta > 0.0 && eta <= 1.0, MGC_ERR_ARG, "re-orthogonalisation threshold in (0, 1]");
+    eta_ = eta;
+  }
+
  private:
   size_t n_;
   int m_;
   bool flexible_;
+  double eta_ = 0.5;
   DevBuf<cplx<R>> V_, Z_, w_;
   DevBuf<cplx<double>> dh_;
   DevBuf<DGmresState> st_;

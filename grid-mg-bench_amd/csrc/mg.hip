@@ -492,12 +492,16 @@ struct MG : MGBase {
     for (int l = 0; l < p.nlevels; ++l) {
       Level<R>& lv = *L[l];
       if (p.graph) {
+        auto dg = [&](size_t n, int m, bool flexible) {
+          auto g = std::make_unique<DGmres<R>>(n, m, flexible);
+          g->set_reorth_eta(hier_reorth_eta<R>());
+          return g;
+        };
         if (l + 1 < p.nlevels) {
-          lv.smoother = std::make_unique<DGmres<R>>(lv.n, p.smoother_iters, false);
-          if (l + 2 < p.nlevels)
-            lv.wrapper = std::make_unique<DGmres<R>>(L[l + 1]->n, p.wrapper_restart, true);
+          lv.smoother = dg(lv.n, p.smoother_iters, false);
+          if (l + 2 < p.nlevels) lv.wrapper = dg(L[l + 1]->n, p.wrapper_restart, true);
         } else {
-          lv.smoother = std::make_unique<DGmres<R>>(lv.n, p.coarsest_restart, false);
+          lv.smoother = dg(lv.n, p.coarsest_restart, false);
         }
       } else if (l + 1 < p.nlevels) {
         lv.smoother = make_solver<R>(p.smoother_solver, lv.n, p.smoother_restart,

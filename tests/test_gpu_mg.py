@@ -243,3 +243,34 @@ def test_graph_captured_kcycle_bit_equal_and_solves(gpu, small):
         print(f"  graph={mode}: {st} recomputed residual {res:.3e}", flush=True)
         assert st["converged"] == 1 and res < 1.2e-10
         mgs[mode].close()
+
+
+def test_graph_kcycle_with_few_fixed_steps(gpu, small):
+    """The bench's graph form (one K-cycle wrapper step and six coarsest
+    steps, captured and replayed) is bit-equal to its eager device-scalar
+    form, and its FGMRES reaches 1e-10 in the true residual in at most two
+    outer iterations more than the host-driven K-cycle of the reference's
+    default parameters."""
+    dims, V, fop, b = small
+    s = gpu.Stream()
+    its = {}
+    outs = {}
+    vin = gpu.DeviceBuffer.from_numpy(gauss(V * 12, 310).reshape(V, 12))
+    for mode in (0, 1, 2):
+        print(f"[case] mg graph mode {mode}, wrapper 1 step, coarsest 6", flush=True)
+        steps = dict(wrapper_restart=1, coarsest_restart=6) if mode else {}
+        mg = gpu.Multigrid(fop, gpu.mg_params(precision=1, graph=mode, **SMALL, **steps))
+        mg.setup(s)
+        vout = gpu.DeviceBuffer(V * 12 * 16)
+        mg.precondition(vin, vout)
+        s.sync()
+        outs[mode] = vout.to_numpy(np.complex128, (V, 12))
+        x = gpu.DeviceBuffer(V * 12 * 16)
+        st = mg.solve(b, x, tol=1e-10, maxit=300, restart=20)
+        res = true_residual(gpu, fop, b, x, V)
+        print(f"  graph={mode}: {st} recomputed residual {res:.3e}", flush=True)
+        assert st["converged"] == 1 and res < 1.2e-10
+        its[mode] = st["iterations"]
+        mg.close()
+    assert np.array_equal(outs[1].view(np.float64), outs[2].view(np.float64))
+    assert its[2] <= its[0] + 2

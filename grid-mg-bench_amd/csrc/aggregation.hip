@@ -982,7 +982,11 @@ void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int n
     }
   } else {
     auto lds = [&](int lpr) { return ((size_t)RC * 2 * nb + (size_t)nb * (256 / lpr + 1)) * es; };
-    const int lpr = lds(2) <= 65536 ? 2 : 4;
+    // FP32 with nb <= 32: one lane per row, 256-row tiles (half the barriers
+    // per panel); otherwise two (or four) lanes per row
+    const int lpr = prec == MGC_F32 && nb <= 32 && lds(1) <= 65536 ? 1
+                    : lds(2) <= 65536                               ? 2
+                                                                    : 4;
     const size_t sh = lds(lpr);
     MGC_CHECK(sh <= 65536, MGC_ERR_SHAPE, "promote: nbasis too large for the LDS tile");
     // register prefetch depth: entries per thread of one tile, rounded up to a power of 2
@@ -1006,7 +1010,9 @@ void agg_transfer(mgc_aggregation* a, const void* in, void* out, int prec, int n
         PROMOTE(double, 4, a->basis64)
       }
     } else if (prec == MGC_F32) {
-      if (lpr == 2) {
+      if (lpr == 1) {
+        PROMOTE(float, 1, agg_basis32(a, s))
+      } else if (lpr == 2) {
         PROMOTE(float, 2, agg_basis32(a, s))
       } else {
         PROMOTE(float, 4, agg_basis32(a, s))

@@ -468,7 +468,8 @@ typedef struct mgc_mg_params {
                               hierarchy through the FP32 multi-rhs stencil; coarse levels of
                               an MGC_F64 hierarchy stay one at a time).  Per vector the set-up
                               solver's own recurrence and stopping rules, so the basis equals
-                              the one-at-a-time basis up to rounding */
+                              the one-at-a-time basis up to rounding.  Device memory: about
+                              8 k level vectors while a level's batches run (BiCGSTAB) */
 } mgc_mg_params;
 
 typedef struct mgc_solve_stats {

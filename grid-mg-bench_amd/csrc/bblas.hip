@@ -632,6 +632,65 @@ __global__ void k_bbicg_xr(const cplx<double>* __restrict__ c, const cplx<R>* __
   }
 }
 
+// bbicg_xr fused with the reductions of the next step: x += alpha p + omega s,
+// r = s - omega t, then per rhs <r, r> (row 0) and <rh, r> (row 1) of the
+// updated r, on the reduction grid and with the block sums of k_bdot_part
+// (one element per thread per step).  A stopped rhs keeps x and r and adds
+// nothing to its sums.
+template <typename R>
+__global__ __launch_bounds__(256) void k_bbicg_xr_dots(
+    const cplx<double>* __restrict__ c, const cplx<R>* __restrict__ p,
+    const cplx<R>* __restrict__ s, const cplx<R>* __restrict__ t, const cplx<R>* __restrict__ rh,
+    cplx<R>* __restrict__ x, cplx<R>* __restrict__ r, size_t n, int C, int dof, int nrhs,
+    double* __restrict__ part) {
+  __shared__ double sm[2][256][2];
+  __shared__ int s_rt[256];
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int q = (int)((g % C) / dof);  // this thread's rhs, the same for all its elements
+  s_rt[threadIdx.x] = q;
+  double rr = 0.0, hr = 0.0, hi = 0.0;
+  if (c[2 * nrhs + q].re != 0.0) {
+    const cplx<double> al = c[q], om = c[nrhs + q];
+    for (size_t i = g; i < n; i += T) {
+      const cplx<R> pi = p[i], si = s[i], ti = t[i], xi = x[i];
+      const double xr = (double)xi.re + (al.re * pi.re - al.im * pi.im) +
+                        (om.re * si.re - om.im * si.im);
+      const double xm = (double)xi.im + (al.re * pi.im + al.im * pi.re) +
+                        (om.re * si.im + om.im * si.re);
+      x[i] = {(R)xr, (R)xm};
+      const cplx<R> rn = {(R)(si.re - (om.re * ti.re - om.im * ti.im)),
+                          (R)(si.im - (om.re * ti.im + om.im * ti.re))};
+      r[i] = rn;
+      const double br = rn.re, bi = rn.im;
+      rr += br * br + bi * bi;
+      const cplx<R> a = rh[i];
+      const double ar = a.re, ai = a.im;
+      hr += ar * br + ai * bi;
+      hi += ar * bi - ai * br;
+    }
+  }
+  sm[0][threadIdx.x][0] = rr;
+  sm[0][threadIdx.x][1] = 0.0;
+  sm[1][threadIdx.x][0] = hr;
+  sm[1][threadIdx.x][1] = hi;
+  __syncthreads();
+  // per-rhs sums of this block, threads in ascending order (as k_bdot_part)
+  for (int o = threadIdx.x; o < 2 * nrhs; o += blockDim.x) {
+    const int jj = o / nrhs, rq = o % nrhs;
+    double sr = 0.0, si2 = 0.0;
+    for (int k = 0; k < (int)blockDim.x; ++k) {
+      if (s_rt[k] == rq) {
+        sr += sm[jj][k][0];
+        si2 += sm[jj][k][1];
+      }
+    }
+    double* out = part + ((((size_t)jj * nrhs + rq) * gridDim.x) + blockIdx.x) * 2;
+    out[0] = sr;
+    out[1] = si2;
+  }
+}
+
 // i -> (i / C, i % C) along a grid-stride walk, one 64-bit division at the start
 struct DivWalk {
   size_t q, qstep;
@@ -706,6 +765,28 @@ void bbicg_xr(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* 
 }
 
 template <typename R>
+void bbicg_xr_dots(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* s,
+                   const cplx<R>* t, const cplx<R>* rh, cplx<R>* x, cplx<R>* r,
+                   const BShape& sh, cplx<double>* d_out) {
+  check_shape(sh);
+  upload_coef(b, c, 3, sh.nrhs);
+  const int nblk = bred_blocks(sh, false);
+  const size_t need = (size_t)2 * sh.nrhs * nblk * 2;
+  if (need > b.part_cap) {
+    if (b.d_part) MGC_HIP(hipFree(b.d_part));
+    b.d_part = nullptr;
+    b.part_cap = 0;
+    MGC_HIP(hipMalloc(&b.d_part, need * sizeof(double)));
+    b.part_cap = need;
+  }
+  k_bbicg_xr_dots<R><<<nblk, 256, 0, b.s>>>(b.d_coef, p, s, t, rh, x, r, sh.n, sh.cols(), sh.dof,
+                                            sh.nrhs, b.d_part);
+  MGC_LAUNCH_CHECK();
+  k_bdot_final<<<2 * sh.nrhs, 256, 0, b.s>>>(b.d_part, nblk, reinterpret_cast<double*>(d_out));
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
 void bpack(const cplx<R>* set, size_t ld, cplx<R>* out, const BShape& sh, hipStream_t s) {
   check_shape(sh);
   MGC_CHECK(ld * sh.nrhs == sh.n, MGC_ERR_SHAPE, "vector set does not match the batch");
@@ -743,6 +824,9 @@ INST(float)
                            cplx<R>*, const BShape&);                                          \
   template void bbicg_xr<R>(BBlas&, const cplx<double>*, const cplx<R>*, const cplx<R>*,      \
                             const cplx<R>*, cplx<R>*, cplx<R>*, const BShape&);               \
+  template void bbicg_xr_dots<R>(BBlas&, const cplx<double>*, const cplx<R>*, const cplx<R>*, \
+                                 const cplx<R>*, const cplx<R>*, cplx<R>*, cplx<R>*,           \
+                                 const BShape&, cplx<double>*);                               \
   template void bpack<R>(const cplx<R>*, size_t, cplx<R>*, const BShape&, hipStream_t);       \
   template void bunpack<R>(const cplx<R>*, cplx<R>*, size_t, const BShape&, hipStream_t);
 INST(double)

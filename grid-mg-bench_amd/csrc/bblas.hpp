@@ -83,6 +83,14 @@ void bbicg_s(BBlas& b, const cplx<double>* c, const cplx<R>* r, const cplx<R>* v
 template <typename R>
 void bbicg_xr(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* s,
               const cplx<R>* t, cplx<R>* x, cplx<R>* r, const BShape& sh);
+// bbicg_xr, then per rhs <r, r> and <rh, r> of the updated r into d_out rows
+// 0 and 1 ([2][nrhs] complex, on the device; a stopped rhs gets zeros), in
+// the same pass (bdots_dev's reduction over the updated r, without a second
+// read of r)
+template <typename R>
+void bbicg_xr_dots(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* s,
+                   const cplx<R>* t, const cplx<R>* rh, cplx<R>* x, cplx<R>* r,
+                   const BShape& sh, cplx<double>* d_out);
 
 // a set of sh.nrhs vectors [rhs][ld] (ld = sites * dof) into one batched
 // vector [site][rhs][dof], and back

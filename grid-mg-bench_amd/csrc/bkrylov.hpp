@@ -351,7 +351,8 @@ class BMinRes : public BSolver<R> {
 // (rho, omega or <rh, v> zero), and for b = 0 (null vectors relaxed from x)
 // exactly maxit steps with no convergence test.  A rhs that has stopped keeps
 // its x and residual while the others go on.  Three host reductions per step
-// ((<rh, r>, |r|^2), <rh, v>, (<s, t>, |t|^2)) serve the whole batch.
+// ((<rh, r>, |r|^2), <rh, v>, (<s, t>, |t|^2)) serve the whole batch; the
+// first of them is formed in the pass that updates x and r.
 template <typename R>
 class BBiCgStab : public BSolver<R> {
  public:
@@ -404,9 +405,10 @@ class BBiCgStab : public BSolver<R> {
     vzero<R>(v, n, st_);
     auto cd = [](C z) { return cplx<double>{z.real(), z.imag()}; };
     auto dc = [](const double* h) { return C(h[0], h[1]); };
-    // (<r, r>, <rh, r>) per rhs
-    auto fetch_r = [&]() {
-      bdots_dev<R>(bl, r, n, 2, r, sh_, false, d_out_);
+    // (<r, r>, <rh, r>) per rhs: computed here the first time, by the fused
+    // update of x and r afterwards
+    auto fetch_r = [&](bool compute) {
+      if (compute) bdots_dev<R>(bl, r, n, 2, r, sh_, false, d_out_);
       bfetch(bl, reinterpret_cast<const double*>(d_out_), 4 * nr);
       for (int q = 0; q < nr; ++q) {
         r2[q] = bl.h_res[2 * q];
@@ -416,7 +418,7 @@ class BBiCgStab : public BSolver<R> {
     auto flags = [&]() {
       for (int q = 0; q < nr; ++q) c[2 * (size_t)nr + q] = {live[q] ? 1.0 : 0.0, 0.0};
     };
-    fetch_r();
+    fetch_r(true);
     while (true) {
       bool any = false;
       for (int q = 0; q < nr; ++q) {
@@ -469,9 +471,10 @@ class BBiCgStab : public BSolver<R> {
         c[q] = cd(alpha[q]);
         c[nr + q] = cd(omega[q]);
       }
-      bbicg_xr<R>(bl, c.data(), p, sv, t, x, r, sh_);  // x += alpha p + omega s; r = s - omega t
+      // x += alpha p + omega s; r = s - omega t; (<r, r>, <rh, r>) in the same pass
+      bbicg_xr_dots<R>(bl, c.data(), p, sv, t, rh, x, r, sh_, d_out_);
       ++st.iters;
-      fetch_r();
+      fetch_r(false);
     }
     return st;
   }

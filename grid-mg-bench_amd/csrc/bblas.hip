@@ -202,7 +202,9 @@ template <typename R, int KT, bool PAIRED>
 __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restrict__ c,
                                                    const cplx<R>* V, size_t ldv, int k,
                                                    cplx<R>* w, size_t n, int C, int dof, int nrhs,
-                                                   double* __restrict__ part) {
+                                                   double* __restrict__ part,
+                                                   const int* __restrict__ skip) {
+  if (skip && *skip) return;  // uniform: the whole grid returns, w and part untouched
   constexpr int KS = KT > 8 ? BMSUB_STAGED : 0;
   using Slot = typename std::conditional<PAIRED, float4, cplx<R>>::type;
   constexpr size_t SUM_BYTES = sizeof(double) * BD_ROWS * 256 * 2;
@@ -327,8 +329,8 @@ __global__ __launch_bounds__(256) void k_bmsub_dot(const cplx<double>* __restric
 }
 
 // res[q] = sum over nblk partials of part[q][*], q = row*nrhs + r
-__global__ __launch_bounds__(256) void k_bdot_final(const double* __restrict__ part, int nblk,
-                                                    double* __restrict__ res) {
+__device__ __forceinline__ void bdot_final_body(const double* __restrict__ part, int nblk,
+                                                double* __restrict__ res) {
   __shared__ double sh[2][256];
   const int q = blockIdx.x;
   double re = 0.0, im = 0.0;
@@ -350,6 +352,19 @@ __global__ __launch_bounds__(256) void k_bdot_final(const double* __restrict__ p
     res[2 * q] = sh[0][0];
     res[2 * q + 1] = sh[1][0];
   }
+}
+
+__global__ __launch_bounds__(256) void k_bdot_final(const double* __restrict__ part, int nblk,
+                                                    double* __restrict__ res) {
+  bdot_final_body(part, nblk, res);
+}
+
+// k_bdot_final, skipped (whole grid, nothing written) when *skip is nonzero
+__global__ __launch_bounds__(256) void k_bdot_final_unless(const int* __restrict__ skip,
+                                                           const double* __restrict__ part,
+                                                           int nblk, double* __restrict__ res) {
+  if (*skip) return;
+  bdot_final_body(part, nblk, res);
 }
 
 template <typename R>
@@ -478,21 +493,22 @@ template <typename R> void bnorm2(BBlas& b, const cplx<R>* x, const BShape& sh, 
 
 template <typename R, int KT>
 static void launch_bmsub_dot(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv,
-                             int k, cplx<R>* w, const BShape& sh, bool pairs, int nblk) {
+                             int k, cplx<R>* w, const BShape& sh, bool pairs, int nblk,
+                             const int* skip) {
   if constexpr (std::is_same<R, float>::value) {
     if (pairs) {
       k_bmsub_dot<R, KT, true><<<nblk, 256, 0, b.s>>>(d_c, V, ldv, k, w, sh.n, sh.cols(), sh.dof,
-                                                        sh.nrhs, b.d_part);
+                                                        sh.nrhs, b.d_part, skip);
       return;
     }
   }
   k_bmsub_dot<R, KT, false><<<nblk, 256, 0, b.s>>>(d_c, V, ldv, k, w, sh.n, sh.cols(), sh.dof,
-                                                     sh.nrhs, b.d_part);
+                                                     sh.nrhs, b.d_part, skip);
 }
 
 template <typename R>
 bool bmultisub_dots_dev(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k,
-                        cplx<R>* w, const BShape& sh, cplx<double>* d_out) {
+                        cplx<R>* w, const BShape& sh, cplx<double>* d_out, const int* skip) {
   check_shape(sh);
   if (k < 1 || k > 24) return false;
   bool pairs = false;
@@ -510,11 +526,16 @@ bool bmultisub_dots_dev(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, siz
     MGC_HIP(hipMalloc(&b.d_part, cap * sizeof(double)));
     b.part_cap = cap;
   }
-  if (k <= 8) launch_bmsub_dot<R, 8>(b, d_c, V, ldv, k, w, sh, pairs, nblk);
-  else if (k <= 16) launch_bmsub_dot<R, 16>(b, d_c, V, ldv, k, w, sh, pairs, nblk);
-  else launch_bmsub_dot<R, 24>(b, d_c, V, ldv, k, w, sh, pairs, nblk);
+  if (k <= 8) launch_bmsub_dot<R, 8>(b, d_c, V, ldv, k, w, sh, pairs, nblk, skip);
+  else if (k <= 16) launch_bmsub_dot<R, 16>(b, d_c, V, ldv, k, w, sh, pairs, nblk, skip);
+  else launch_bmsub_dot<R, 24>(b, d_c, V, ldv, k, w, sh, pairs, nblk, skip);
   MGC_LAUNCH_CHECK();
-  k_bdot_final<<<rows * sh.nrhs, 256, 0, b.s>>>(b.d_part, nblk, reinterpret_cast<double*>(d_out));
+  if (skip)
+    k_bdot_final_unless<<<rows * sh.nrhs, 256, 0, b.s>>>(skip, b.d_part, nblk,
+                                                         reinterpret_cast<double*>(d_out));
+  else
+    k_bdot_final<<<rows * sh.nrhs, 256, 0, b.s>>>(b.d_part, nblk,
+                                                  reinterpret_cast<double*>(d_out));
   MGC_LAUNCH_CHECK();
   return true;
 }
@@ -726,6 +747,53 @@ __global__ void k_bpack(const cplx<R>* __restrict__ in, cplx<R>* __restrict__ ou
   }
 }
 
+// y = x * f(r2[q].re) per rhs q with a device factor: mode 1 f = 1/sqrt
+// (0 where r2 <= 0), mode 2 f = sqrt; precision changes from Rs to Rd
+template <typename Rs, typename Rd>
+__global__ void k_bconvert_dev(const cplx<Rs>* __restrict__ x, cplx<Rd>* __restrict__ y,
+                               size_t n, int C, int dof, const cplx<double>* __restrict__ r2,
+                               int mode) {
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const double q = r2[mw.m / (unsigned)dof].re;
+    const double f = mode == 1 ? (q > 0.0 ? 1.0 / sqrt(q) : 0.0) : (q > 0.0 ? sqrt(q) : 0.0);
+    y[i] = {(Rd)((double)x[i].re * f), (Rd)((double)x[i].im * f)};
+  }
+}
+
+// x *= 1 / sqrt(r2[q].re) per rhs q in place (0 where r2 <= 0)
+template <typename R>
+__global__ void k_bscale_dev(cplx<R>* x, size_t n, int C, int dof,
+                             const cplx<double>* __restrict__ r2) {
+  const size_t T = (size_t)gridDim.x * blockDim.x;
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  ModWalk mw(g, T, C);
+  for (size_t i = g; i < n; i += T, mw.next()) {
+    const double q = r2[mw.m / (unsigned)dof].re;
+    const double f = q > 0.0 ? 1.0 / sqrt(q) : 0.0;
+    const cplx<R> v = x[i];
+    x[i] = {(R)((double)v.re * f), (R)((double)v.im * f)};
+  }
+}
+
+template <typename Rs, typename Rd>
+void bconvert_dev(const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const cplx<double>* d_r2,
+                  int mode, hipStream_t s) {
+  check_shape(sh);
+  k_bconvert_dev<Rs, Rd><<<bew_grid(sh.n), 256, 0, s>>>(x, y, sh.n, sh.cols(), sh.dof, d_r2,
+                                                        mode);
+  MGC_LAUNCH_CHECK();
+}
+
+template <typename R>
+void bscale_dev(cplx<R>* x, const BShape& sh, const cplx<double>* d_r2, hipStream_t s) {
+  check_shape(sh);
+  k_bscale_dev<R><<<bew_grid(sh.n), 256, 0, s>>>(x, sh.n, sh.cols(), sh.dof, d_r2);
+  MGC_LAUNCH_CHECK();
+}
+
 static void upload_coef(BBlas& b, const cplx<double>* c, int rows, int nrhs) {
   MGC_HIP(hipEventSynchronize(b.coef_done));
   std::copy(c, c + (size_t)rows * nrhs, b.h_coef);
@@ -807,7 +875,7 @@ void bunpack(const cplx<R>* in, cplx<R>* set, size_t ld, const BShape& sh, hipSt
                              const BShape&, bool, cplx<double>*);                             \
   template void bnorm2<R>(BBlas&, const cplx<R>*, const BShape&, double*);                    \
   template bool bmultisub_dots_dev<R>(BBlas&, const cplx<double>*, const cplx<R>*, size_t, int, \
-                                      cplx<R>*, const BShape&, cplx<double>*);                 \
+                                      cplx<R>*, const BShape&, cplx<double>*, const int*);     \
   template void bmultisub_dev<R>(const cplx<double>*, const cplx<R>*, size_t, int, cplx<R>*,  \
                                  const BShape&, hipStream_t);                                 \
   template void bmultiaxpy<R>(BBlas&, const cplx<double>*, const cplx<R>*, size_t, int,       \
@@ -827,6 +895,7 @@ INST(float)
   template void bbicg_xr_dots<R>(BBlas&, const cplx<double>*, const cplx<R>*, const cplx<R>*, \
                                  const cplx<R>*, const cplx<R>*, cplx<R>*, cplx<R>*,           \
                                  const BShape&, cplx<double>*);                               \
+  template void bscale_dev<R>(cplx<R>*, const BShape&, const cplx<double>*, hipStream_t);     \
   template void bpack<R>(const cplx<R>*, size_t, cplx<R>*, const BShape&, hipStream_t);       \
   template void bunpack<R>(const cplx<R>*, cplx<R>*, size_t, const BShape&, hipStream_t);
 INST(double)
@@ -838,6 +907,12 @@ template void bconvert<float, double>(BBlas&, const cplx<float>*, cplx<double>*,
                                       const double*);
 template void bconvert<double, double>(BBlas&, const cplx<double>*, cplx<double>*,
                                        const BShape&, const double*);
+template void bconvert_dev<double, float>(const cplx<double>*, cplx<float>*, const BShape&,
+                                          const cplx<double>*, int, hipStream_t);
+template void bconvert_dev<float, double>(const cplx<float>*, cplx<double>*, const BShape&,
+                                          const cplx<double>*, int, hipStream_t);
+template void bconvert_dev<double, double>(const cplx<double>*, cplx<double>*, const BShape&,
+                                           const cplx<double>*, int, hipStream_t);
 template void bconvert<float, float>(BBlas&, const cplx<float>*, cplx<float>*, const BShape&,
                                      const double*);
 

@@ -47,10 +47,11 @@ template <typename R> void bnorm2(BBlas& b, const cplx<R>* x, const BShape& sh, 
 // Fused re-orthogonalisation pass: w -= sum_j c[j][r] V_j, then d_out[j][r] =
 // <V_j, w>_r for j < k and d_out[k][r].re = |w|^2_r, as bmultisub_dev followed
 // by bdots_dev(..., true, d_out) but with one read of V; false (nothing done)
-// for k > 24
+// for k > 24.  A nonzero device flag *skip makes the pass change nothing.
 template <typename R>
 bool bmultisub_dots_dev(BBlas& b, const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k,
-                        cplx<R>* w, const BShape& sh, cplx<double>* d_out);
+                        cplx<R>* w, const BShape& sh, cplx<double>* d_out,
+                        const int* skip = nullptr);
 // w -= sum_j c[j][r] V_j (c on the device, [k][nrhs])
 template <typename R>
 void bmultisub_dev(const cplx<double>* d_c, const cplx<R>* V, size_t ldv, int k, cplx<R>* w,
@@ -91,6 +92,17 @@ template <typename R>
 void bbicg_xr_dots(BBlas& b, const cplx<double>* c, const cplx<R>* p, const cplx<R>* s,
                    const cplx<R>* t, const cplx<R>* rh, cplx<R>* x, cplx<R>* r,
                    const BShape& sh, cplx<double>* d_out);
+
+// per-rhs scaling by a device factor from r2 ([nrhs] complex, .re used, as
+// bdots_dev's |w|^2 row leaves it), with no host synchronisation:
+//   bconvert_dev: y = x / sqrt(r2) (mode 1, 0 where r2 <= 0) or x * sqrt(r2)
+//                 (mode 2), with precision change
+//   bscale_dev:   x /= sqrt(r2) in place (0 where r2 <= 0)
+template <typename Rs, typename Rd>
+void bconvert_dev(const cplx<Rs>* x, cplx<Rd>* y, const BShape& sh, const cplx<double>* d_r2,
+                  int mode, hipStream_t s);
+template <typename R>
+void bscale_dev(cplx<R>* x, const BShape& sh, const cplx<double>* d_r2, hipStream_t s);
 
 // a set of sh.nrhs vectors [rhs][ld] (ld = sites * dof) into one batched
 // vector [site][rhs][dof], and back

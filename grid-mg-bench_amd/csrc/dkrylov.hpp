@@ -16,6 +16,7 @@
 #pragma once
 #include <complex>
 
+#include "bkrylov.hpp"
 #include "krylov.hpp"
 
 namespace mgc {
@@ -198,6 +199,217 @@ class DGmres : public Solver<R> {
   DevBuf<cplx<R>> V_, Z_, w_;
   DevBuf<cplx<double>> dh_;
   DevBuf<DGmresState> st_;
+};
+
+// ---------------------------------------------------------------------------
+// Batched device-scalar (F)GMRES: nrhs systems on [site][rhs][dof] vectors,
+// a fixed number m of Arnoldi steps each, the per-rhs scalars (Hessenberg
+// column, Givens rotations, triangular solve) in device memory and updated
+// by single-workgroup kernels with one thread per rhs, so that a batched
+// K-cycle built from it can be captured into a HIP graph.  Per rhs the
+// arithmetic of DGmres; the second projection of a step runs for the whole
+// batch when some rhs still in use needs it (as BGmres decides it on the
+// host), decided on the device.
+// ---------------------------------------------------------------------------
+
+// g = (|r0|, 0, ...), and no Krylov space for a zero residual
+__global__ void k_dbg_begin(cplx<double>* g, int* jmax, const cplx<double>* r2, int nr, int m) {
+  const int r = threadIdx.x;
+  if (r >= nr) return;
+  for (int i = 0; i <= m; ++i) g[(size_t)r * (m + 1) + i] = {0.0, 0.0};
+  const double b2 = r2[r].re;
+  g[(size_t)r * (m + 1)] = {b2 > 0.0 ? sqrt(b2) : 0.0, 0.0};
+  jmax[r] = b2 > 0.0 ? m : 0;
+}
+
+// re-orthogonalisation test of step j for the batch (h1 at dh1[i][r],
+// |w|^2_r at dh1[j+1][r].re): skip the second projection unless some rhs in
+// use has |w|^2 - |h1|^2 < eta |w|^2; when skipped, dh2 takes h1 and |w|^2
+// and dh1 is zeroed, as k_dg_reorth_test does
+__global__ void k_dbg_reorth_test(cplx<double>* dh1, cplx<double>* dh2, const int* jmax, int j,
+                                  double eta, int nr, int* skip) {
+  const int r = threadIdx.x;
+  int need = 0;
+  if (r < nr && j < jmax[r]) {
+    double sum1 = 0.0;
+    for (int i = 0; i <= j; ++i) {
+      const cplx<double> h = dh1[(size_t)i * nr + r];
+      sum1 += h.re * h.re + h.im * h.im;
+    }
+    const double wn2 = dh1[(size_t)(j + 1) * nr + r].re;
+    need = wn2 - sum1 >= eta * wn2 ? 0 : 1;
+  }
+  const int any = __syncthreads_or(need);
+  if (threadIdx.x == 0) *skip = any ? 0 : 1;
+  if (!any && r < nr) {
+    for (int i = 0; i <= j; ++i) {
+      dh2[(size_t)i * nr + r] = dh1[(size_t)i * nr + r];
+      dh1[(size_t)i * nr + r] = {0.0, 0.0};
+    }
+    dh2[(size_t)(j + 1) * nr + r] = dh1[(size_t)(j + 1) * nr + r];
+  }
+}
+
+// column j of each rhs's Hessenberg matrix (h1 + h2, the new norm at
+// dh2[j+1][r].im), the earlier rotations and the new one, as k_dg_step
+__global__ void k_dbg_step(cplx<double>* H, double* cs, cplx<double>* sn, cplx<double>* g,
+                           int* jmax, const cplx<double>* dh1, const cplx<double>* dh2, int j,
+                           int nr, int m) {
+  typedef std::complex<double> C;
+  const int r = threadIdx.x;
+  if (r >= nr || j >= jmax[r]) return;
+  C col[Blas::KMAX + 1];
+  for (int i = 0; i <= j; ++i) {
+    const cplx<double> a = dh1[(size_t)i * nr + r], b = dh2[(size_t)i * nr + r];
+    col[i] = C(a.re + b.re, a.im + b.im);
+  }
+  const double hn = dh2[(size_t)(j + 1) * nr + r].im;
+  double* csr = cs + (size_t)r * m;
+  cplx<double>* snr = sn + (size_t)r * m;
+  for (int i = 0; i < j; ++i) {
+    const C a = col[i], b = col[i + 1];
+    const C sv(snr[i].re, snr[i].im);
+    col[i] = csr[i] * a + sv * b;
+    col[i + 1] = -std::conj(sv) * a + csr[i] * b;
+  }
+  const C a = col[j];
+  const double bb = hn > 0.0 ? hn : 0.0;
+  const double t = sqrt(std::norm(a) + bb * bb);
+  double c;
+  C sv;
+  if (std::abs(a) == 0.0) {
+    c = 0.0;
+    sv = 1.0;
+  } else {
+    c = std::abs(a) / t;
+    sv = (a / std::abs(a)) * (bb / t);
+  }
+  col[j] = c * a + sv * bb;
+  csr[j] = c;
+  snr[j] = {sv.real(), sv.imag()};
+  cplx<double>* gr = g + (size_t)r * (m + 1);
+  const C gj(gr[j].re, gr[j].im);
+  const C g1 = -std::conj(sv) * gj, g0 = c * gj;
+  gr[j] = {g0.real(), g0.imag()};
+  gr[j + 1] = {g1.real(), g1.imag()};
+  cplx<double>* Hr = H + (size_t)r * (m + 1) * m;
+  for (int i = 0; i <= j; ++i) Hr[(size_t)i * m + j] = {col[i].real(), col[i].imag()};
+  if (!(hn > 0.0)) jmax[r] = j + 1;  // the next vector is zero: stop using columns
+}
+
+// per rhs y = H^-1 g over its columns in use, stored negated as negy[i][r]
+__global__ void k_dbg_solve(const cplx<double>* H, const cplx<double>* g, const int* jmax,
+                            cplx<double>* negy, int nr, int m) {
+  typedef std::complex<double> C;
+  const int r = threadIdx.x;
+  if (r >= nr) return;
+  const int jm = jmax[r] < m ? jmax[r] : m;
+  const cplx<double>* Hr = H + (size_t)r * (m + 1) * m;
+  const cplx<double>* gr = g + (size_t)r * (m + 1);
+  C y[Blas::KMAX];
+  for (int i = jm - 1; i >= 0; --i) {
+    C acc(gr[i].re, gr[i].im);
+    for (int k = i + 1; k < jm; ++k)
+      acc -= C(Hr[(size_t)i * m + k].re, Hr[(size_t)i * m + k].im) * y[k];
+    const C d(Hr[(size_t)i * m + i].re, Hr[(size_t)i * m + i].im);
+    y[i] = std::abs(d) > 0.0 ? acc / d : C(0.0);
+  }
+  for (int i = 0; i < m; ++i)
+    negy[(size_t)i * nr + r] = i < jm ? cplx<double>{-y[i].real(), -y[i].imag()}
+                                      : cplx<double>{0.0, 0.0};
+}
+
+template <typename R>
+class DBGmres : public BSolver<R> {
+ public:
+  DBGmres(const BShape& sh, int m, bool flexible) : sh_(sh), m_(m), flexible_(flexible) {
+    MGC_CHECK(m >= 1 && m + 1 <= Blas::KMAX, MGC_ERR_ARG, "Arnoldi steps out of range");
+    MGC_CHECK(sh.nrhs >= 1 && sh.nrhs <= 64, MGC_ERR_ARG, "batch of 1..64 rhs");
+    const int nr = sh.nrhs;
+    V_.alloc((size_t)(m + 1) * sh.n);
+    if (flexible) Z_.alloc((size_t)m * sh.n);
+    w_.alloc(sh.n);
+    dh_.alloc((size_t)(2 * (m + 1) + 2) * nr);
+    H_.alloc((size_t)nr * (m + 1) * m);
+    sn_.alloc((size_t)nr * m);
+    cs_.alloc((size_t)nr * m);
+    g_.alloc((size_t)nr * (m + 1));
+    negy_.alloc((size_t)m * nr);
+    r2_.alloc(nr);
+    jmax_.alloc(nr);
+    skip_.alloc(1);
+  }
+  // re-orthogonalisation threshold eta, as Gmres::set_reorth_eta
+  void set_reorth_eta(double eta) {
+    MGC_CHECK(eta > 0.0 && eta <= 1.0, MGC_ERR_ARG, "re-orthogonalisation threshold in (0, 1]");
+    eta_ = eta;
+  }
+
+  // m steps from x (or 0) for every rhs; tol and maxit are not used
+  BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>* P, const cplx<R>* b, cplx<R>* x,
+                     bool x_zero, double, int) override {
+    hipStream_t s = bl.s;
+    const size_t n = sh_.n;
+    const int nr = sh_.nrhs, m = m_;
+    cplx<R>* V = V_.p;
+    BKrylovStats st;
+    st.rel_res.assign(nr, 0.0);
+    st.converged.assign(nr, 0);
+    st.iters_rhs.assign(nr, m);
+    if (x_zero) {
+      vzero<R>(x, n, s);
+      vcopy<R>(b, V, n, s);
+    } else {
+      A.apply(x, w_.p, s);
+      ++st.applies;
+      vsub<R>(b, w_.p, V, n, s);
+    }
+    bdots_dev<R>(bl, V, n, 0, V, sh_, true, r2_.p);  // |r0|^2 per rhs
+    k_dbg_begin<<<1, 64, 0, s>>>(g_.p, jmax_.p, r2_.p, nr, m);
+    MGC_LAUNCH_CHECK();
+    bscale_dev<R>(V, sh_, r2_.p, s);  // v0 = r0 / |r0|
+    cplx<double>* dh1 = dh_.p;
+    for (int j = 0; j < m; ++j) {
+      cplx<R>* vj = V + (size_t)j * n;
+      const cplx<R>* z = vj;
+      if (flexible_) {
+        cplx<R>* zj = Z_.p + (size_t)j * n;
+        if (P) P->apply(vj, zj, s);
+        else vcopy<R>(vj, zj, n, s);
+        z = zj;
+      }
+      A.apply(z, w_.p, s);
+      ++st.applies;
+      cplx<double>* dh2 = dh_.p + (size_t)(j + 2) * nr;
+      bdots_dev<R>(bl, V, n, j + 1, w_.p, sh_, true, dh1);
+      if (j + 1 <= 24) {
+        k_dbg_reorth_test<<<1, 64, 0, s>>>(dh1, dh2, jmax_.p, j, eta_, nr, skip_.p);
+        MGC_LAUNCH_CHECK();
+        bmultisub_dots_dev<R>(bl, dh1, V, n, j + 1, w_.p, sh_, dh2, skip_.p);
+      } else {
+        bmultisub_dev<R>(dh1, V, n, j + 1, w_.p, sh_, s);
+        bdots_dev<R>(bl, V, n, j + 1, w_.p, sh_, true, dh2);
+      }
+      bsubnorm_dev<R>(dh2, V, n, j + 1, w_.p, V + (size_t)(j + 1) * n, sh_, s);
+      k_dbg_step<<<1, 64, 0, s>>>(H_.p, cs_.p, sn_.p, g_.p, jmax_.p, dh1, dh2, j, nr, m);
+      MGC_LAUNCH_CHECK();
+      ++st.iters;
+    }
+    k_dbg_solve<<<1, 64, 0, s>>>(H_.p, g_.p, jmax_.p, negy_.p, nr, m);
+    MGC_LAUNCH_CHECK();
+    bmultisub_dev<R>(negy_.p, flexible_ ? Z_.p : V, n, m, x, sh_, s);  // x += sum y_i Z_i
+    return st;
+  }
+
+ private:
+  BShape sh_;
+  int m_;
+  bool flexible_;
+  double eta_ = 0.5;
+  DevBuf<cplx<R>> V_, Z_, w_;
+  DevBuf<cplx<double>> dh_, H_, sn_, g_, negy_, r2_;
+  DevBuf<double> cs_;
+  DevBuf<int> jmax_, skip_;
 };
 
 }  // namespace mgc

@@ -676,7 +676,7 @@ struct MG : MGBase {
   struct BLevelState {
     std::unique_ptr<BLinOp<R>> op;
     std::unique_ptr<BSolver<R>> smoother;
-    std::unique_ptr<BGmres<R>> wrapper;
+    std::unique_ptr<BSolver<R>> wrapper;
     cplx<R>* rc = nullptr;  // next-level residual / correction, batched
     cplx<R>* ec = nullptr;
     ~BLevelState() {
@@ -698,8 +698,16 @@ struct MG : MGBase {
   std::unique_ptr<BBlas> bbl;
   cplx<R>* bin = nullptr;
   cplx<R>* bout = nullptr;
+  // graph mode: per-rhs |in|^2 on the device and the captured batched K-cycle
+  DevBuf<cplx<double>> bnin;
+  hipGraph_t bgraph = nullptr;
+  hipGraphExec_t bgexec = nullptr;
 
   void free_batch() {
+    if (bgexec) (void)hipGraphExecDestroy(bgexec);
+    if (bgraph) (void)hipGraphDestroy(bgraph);
+    bgexec = nullptr;
+    bgraph = nullptr;
     BL.clear();
     bprecs.clear();
     if (bin) (void)hipFree(bin);
@@ -730,12 +738,17 @@ struct MG : MGBase {
         if (type == MGC_SOLVER_MR) return std::make_unique<BMinRes<R>>(sh);
         MGC_CHECK(type == MGC_SOLVER_GMRES, MGC_ERR_ARG,
                   "the batched solve takes GMRES or MR smoothers and coarsest solvers");
+        if (p.graph) {  // device-scalar, fixed step counts (checked GMRES in graph mode)
+          auto g = std::make_unique<DBGmres<R>>(sh, restart, false);
+          g->set_reorth_eta(hier_reorth_eta<R>());
+          return g;
+        }
         auto g = std::make_unique<BGmres<R>>(sh, restart, false);
         g->set_reorth_eta(hier_reorth_eta<R>());
         return g;
       };
       if (l + 1 < p.nlevels) {
-        st->smoother = bsolver(p.smoother_solver, p.smoother_restart);
+        st->smoother = bsolver(p.smoother_solver, p.graph ? p.smoother_iters : p.smoother_restart);
         const size_t nc = L[l + 1]->n * (size_t)nrhs;
         MGC_HIP(hipMalloc(&st->rc, nc * sizeof(cplx<R>)));
         MGC_HIP(hipMalloc(&st->ec, nc * sizeof(cplx<R>)));
@@ -745,8 +758,15 @@ struct MG : MGBase {
       BL.push_back(std::move(st));
     }
     for (int l = 0; l + 2 < p.nlevels; ++l) {
-      BL[l]->wrapper = std::make_unique<BGmres<R>>(BL[l + 1]->op->sh, p.wrapper_restart, true);
-      BL[l]->wrapper->set_reorth_eta(hier_reorth_eta<R>());
+      if (p.graph) {
+        auto g = std::make_unique<DBGmres<R>>(BL[l + 1]->op->sh, p.wrapper_restart, true);
+        g->set_reorth_eta(hier_reorth_eta<R>());
+        BL[l]->wrapper = std::move(g);
+      } else {
+        auto g = std::make_unique<BGmres<R>>(BL[l + 1]->op->sh, p.wrapper_restart, true);
+        g->set_reorth_eta(hier_reorth_eta<R>());
+        BL[l]->wrapper = std::move(g);
+      }
     }
     for (int l = 0; l < p.nlevels; ++l) bprecs.push_back(std::make_unique<BLevelPrec>(this, l));
     MGC_HIP(hipMalloc(&bin, BL[0]->op->sh.n * sizeof(cplx<R>)));
@@ -778,8 +798,46 @@ struct MG : MGBase {
     lv.smoother->solve(*bbl, *lv.op, nullptr, in, out, false, p.smoother_tol, p.smoother_iters);
   }
 
+  // graph mode: the per-rhs norms and conversions on the device around the
+  // batched K-cycle on bin / bout, which graph = 2 captures into a HIP graph
+  // at the first call (after one eager application that makes every
+  // first-use allocation) and replays from then on
+  void precondition_batch_graph(const cplx<double>* in, cplx<double>* out, hipStream_t s) {
+    MGC_CHECK(s != nullptr, MGC_ERR_ARG,
+              "graph mode needs a created stream (the null stream cannot be captured)");
+    const BShape sh64 = BL[0]->op->sh;
+    if (!bnin.p) bnin.alloc(BBlas::RMAX);
+    bdots_dev<double>(*bbl, in, 0, 0, in, sh64, true, bnin.p);  // |in|^2 per rhs
+    bconvert_dev<double, R>(in, bin, sh64, bnin.p, 1, s);         // in / |in|
+    if (p.graph == 1) {
+      kcycle_b(0, bin, bout, s);
+    } else {
+      if (!bgexec) {
+        kcycle_b(0, bin, bout, s);  // first-use allocations happen outside the capture
+        MGC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          kcycle_b(0, bin, bout, s);
+        } catch (...) {
+          hipGraph_t g = nullptr;
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        MGC_HIP(hipStreamEndCapture(s, &bgraph));
+        MGC_HIP(hipGraphInstantiate(&bgexec, bgraph, nullptr, nullptr, 0));
+      }
+      MGC_HIP(hipGraphLaunch(bgexec, s));
+    }
+    bconvert_dev<R, double>(bout, out, sh64, bnin.p, 2, s);  // * |in|
+  }
+
   void precondition64_batch(const cplx<double>* in, cplx<double>* out, hipStream_t s) override {
     MGC_CHECK(bn > 0, MGC_ERR_STATE, "batched preconditioner not prepared");
+    if (p.graph) {
+      precondition_batch_graph(in, out, s);
+      ++precond_applies;
+      return;
+    }
     BShape sh64 = BL[0]->op->sh;
     std::vector<double> nin(bn), inv(bn);
     bnorm2<double>(*bbl, in, sh64, nin.data());

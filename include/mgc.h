@@ -453,7 +453,8 @@ typedef struct mgc_mg_params {
   int setup_recursive_updates;   /* setupRecursiveUpdates (default 1) */
   int smoother_solver;     /* smootherSolverType: MGC_SOLVER_GMRES (default) or _MR */
   int coarsest_solver;     /* coarsestSolverType: MGC_SOLVER_GMRES (default), _BICGSTAB, _MR */
-  int graph;               /* K-cycle of mgc_mg_precondition / mgc_mg_solve (no counterpart in
+  int graph;               /* K-cycle of mgc_mg_precondition / mgc_mg_solve and of the batched
+                              mgc_mg_solve_mrhs (no counterpart in
                               the reference): 0 (default) = host-driven Krylov solvers;
                               1 = device-scalar GMRES with fixed step counts (smoother_iters
                               smoother steps, wrapper_restart FGMRES steps, coarsest_restart

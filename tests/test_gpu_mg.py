@@ -274,3 +274,37 @@ def test_graph_kcycle_with_few_fixed_steps(gpu, small):
         mg.close()
     assert np.array_equal(outs[1].view(np.float64), outs[2].view(np.float64))
     assert its[2] <= its[0] + 2
+
+
+def test_batched_graph_kcycle_bit_equal_and_solves(gpu, small):
+    """Graph mode of the batched solve: the device-scalar batched K-cycle
+    issued eagerly (graph = 1) and captured once into a HIP graph and
+    replayed (graph = 2) make the same preconditioner, so the two lockstep
+    solves give the same x bit for bit with the same iteration counts; every
+    rhs reaches 1e-10 in the recomputed true residual, in at most two outer
+    iterations more than the host-driven batched solve."""
+    dims, V, fop, _ = small
+    nrhs = 3
+    s = gpu.Stream()
+    bs = gauss(V * nrhs * 12, 95).reshape(V, nrhs, 12)
+    b = gpu.DeviceBuffer.from_numpy(bs)
+    xs, its = {}, {}
+    for mode in (0, 1, 2):
+        print(f"[case] batched mg solve graph mode {mode}, nrhs={nrhs}", flush=True)
+        steps = dict(wrapper_restart=1, coarsest_restart=6) if mode else {}
+        mg = gpu.Multigrid(fop, gpu.mg_params(precision=1, graph=mode, **MRHS_CFG, **steps))
+        mg.setup(s)
+        x = gpu.DeviceBuffer(V * nrhs * 12 * 16)
+        sts = mg.solve_mrhs(b, x, nrhs, coarse_prec=1, tol=1e-10, maxit=300, restart=20)
+        xs[mode] = x.to_numpy(np.complex128, (V, nrhs, 12))
+        its[mode] = [st["iterations"] for st in sts]
+        print(f"  iterations {its[mode]}", flush=True)
+        for r in range(nrhs):
+            br = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(bs[:, r]))
+            xr = gpu.DeviceBuffer.from_numpy(np.ascontiguousarray(xs[mode][:, r]))
+            res = true_residual(gpu, fop, br, xr, V)
+            assert sts[r]["converged"] == 1 and res <= 1.2e-10, (mode, r, res)
+        mg.close()
+    assert its[1] == its[2]
+    assert np.array_equal(xs[1].view(np.float64), xs[2].view(np.float64))
+    assert max(its[2]) <= max(its[0]) + 2

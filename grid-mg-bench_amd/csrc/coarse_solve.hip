@@ -29,6 +29,7 @@
 
 #include "bkrylov.hpp"
 #include "coarse_op.hpp"
+#include "dkrylov.hpp"
 
 namespace mgc {
 
@@ -229,10 +230,147 @@ struct CoarseBatch {
   CoarseBatch& operator=(const CoarseBatch&) = delete;
 };
 
+// Fixed-step form (no counterpart in the reference): every lattice runs the
+// same number of device-scalar GMRES steps from x = 0 (DBGmres in phases),
+// all of them in lockstep on one stream with one multi-lattice stencil launch
+// per step and no host synchronisation, so that graph = 2 captures the whole
+// solve into a HIP graph at the first call (after one eager solve that makes
+// every first-use allocation and tunes the launch) and replays it after;
+// graph = 1 issues the same work eagerly.  The right-hand sides and
+// solutions pass through staging vectors, copied outside the graph.
+struct FixedCoarseBatch {
+  hipStream_t s;
+  int nrhs, mprec, steps, graph;
+  std::vector<mgc_coarse_op*> ops;
+  std::unique_ptr<BBlas> bl;
+  std::vector<std::unique_ptr<DBGmres<float>>> sol;
+  DevBuf<cplx<float>> bst[CS_MAX], xst[CS_MAX], r[CS_MAX];
+  DevBuf<unsigned char> q[CS_MAX];
+  DevBuf<float> qs[CS_MAX];
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  FixedCoarseBatch(mgc_coarse_op* const* o, int count, int nr, int mp, int st, int gm,
+                   hipStream_t str)
+      : s(str), nrhs(nr), mprec(mp), steps(st), graph(gm), ops(o, o + count) {
+    bl = std::make_unique<BBlas>(s);
+    for (int k = 0; k < count; ++k) {
+      BShape sh;
+      sh.n = (size_t)ops[k]->V * ops[k]->N * nr;
+      sh.nrhs = nr;
+      sh.dof = ops[k]->N;
+      sol.push_back(std::make_unique<DBGmres<float>>(sh, st, false));
+      bst[k].alloc(sh.n);
+      xst[k].alloc(sh.n);
+      r[k].alloc(sh.n);
+      if (mp != MGC_F32) {
+        const size_t rows = (size_t)ops[k]->V * nr;
+        q[k].alloc(rows * ops[k]->N * 4);
+        qs[k].alloc(rows);
+      }
+    }
+  }
+  ~FixedCoarseBatch() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (g) (void)hipGraphDestroy(g);
+  }
+  FixedCoarseBatch(const FixedCoarseBatch&) = delete;
+  FixedCoarseBatch& operator=(const FixedCoarseBatch&) = delete;
+
+  // every lattice's operator on its input, in one launch
+  void apply_all(const cplx<float>* const* in, cplx<float>* const* out) {
+    const int K = int(ops.size());
+    const void* src[CS_MAX];
+    const float* sc[CS_MAX] = {};
+    void* dst[CS_MAX];
+    for (int k = 0; k < K; ++k) {
+      src[k] = in[k];
+      dst[k] = out[k];
+      if (mprec != MGC_F32) {
+        cs_ok(mgc_quantize(in[k], (size_t)ops[k]->V * nrhs, ops[k]->N, mprec, q[k].p, qs[k].p,
+                           (void*)s));
+        src[k] = q[k].p;
+        sc[k] = qs[k].p;
+      }
+    }
+    cs_ok(mgc_coarse_op_M_mrhs_batched(ops.data(), K, src, mprec == MGC_F32 ? nullptr : sc,
+                                       mprec, dst, nrhs, mprec, 0, (void*)s));
+  }
+  // the solve on the staging vectors: no host synchronisation
+  void run() {
+    const int K = int(ops.size());
+    for (int k = 0; k < K; ++k) sol[k]->begin(*bl, bst[k].p, xst[k].p, true);
+    const cplx<float>* in[CS_MAX];
+    cplx<float>* out[CS_MAX];
+    for (int j = 0; j < steps; ++j) {
+      for (int k = 0; k < K; ++k) {
+        in[k] = sol[k]->direction(j, nullptr, s);
+        out[k] = sol[k]->product();
+      }
+      apply_all(in, out);
+      for (int k = 0; k < K; ++k) sol[k]->arnoldi(*bl, j);
+    }
+    for (int k = 0; k < K; ++k) sol[k]->finish(*bl, xst[k].p);
+  }
+  void solve(const void* const* b, void* const* x, double tol, mgc_solve_stats* stats) {
+    const int K = int(ops.size());
+    MGC_CHECK(b && x && stats, MGC_ERR_ARG, "null argument");
+    MGC_CHECK(tol > 0.0, MGC_ERR_ARG, "tol > 0 needed");
+    for (int k = 0; k < K; ++k) MGC_CHECK(b[k] && x[k], MGC_ERR_ARG, "null vector in the batch");
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < K; ++k)
+      MGC_HIP(hipMemcpyAsync(bst[k].p, b[k], sol[k]->shape().n * sizeof(cplx<float>),
+                             hipMemcpyDeviceToDevice, s));
+    if (graph == 1) {
+      run();
+    } else {
+      if (!ge) {
+        run();  // first-use allocations and the launch tuning happen outside the capture
+        MGC_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          run();
+        } catch (...) {
+          hipGraph_t gg = nullptr;
+          (void)hipStreamEndCapture(s, &gg);
+          if (gg) (void)hipGraphDestroy(gg);
+          throw;
+        }
+        MGC_HIP(hipStreamEndCapture(s, &g));
+        MGC_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+      }
+      MGC_HIP(hipGraphLaunch(ge, s));
+    }
+    for (int k = 0; k < K; ++k)
+      MGC_HIP(hipMemcpyAsync(x[k], xst[k].p, sol[k]->shape().n * sizeof(cplx<float>),
+                             hipMemcpyDeviceToDevice, s));
+    MGC_HIP(hipStreamSynchronize(s));
+    const double secs =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // b - M x with the stencil of the solve, per rhs
+    for (int k = 0; k < K; ++k) {
+      const BShape& sh = sol[k]->shape();
+      std::vector<double> bn(nrhs), rn(nrhs);
+      bnorm2<float>(*bl, bst[k].p, sh, bn.data());
+      cs_ok(mgc_coarse_op_M_mrhs(ops[k], xst[k].p, nullptr, MGC_F32, r[k].p, nrhs, mprec, 0,
+                                 (void*)s));
+      vsub<float>(bst[k].p, r[k].p, r[k].p, sh.n, s);
+      bnorm2<float>(*bl, r[k].p, sh, rn.data());
+      for (int c = 0; c < nrhs; ++c) {
+        mgc_solve_stats& o = stats[k * nrhs + c];
+        o.iterations = steps;
+        o.true_residual = bn[c] > 0.0 ? std::sqrt(rn[c] / bn[c]) : 0.0;
+        o.converged = o.true_residual <= tol ? 1 : 0;
+        o.precond_applies = 0;
+        o.seconds = secs;
+      }
+    }
+  }
+};
+
 }  // namespace mgc
 
 struct mgc_coarse_batch {
   std::unique_ptr<mgc::CoarseBatch> impl;
+  std::unique_ptr<mgc::FixedCoarseBatch> fixed;
 };
 
 using namespace mgc;
@@ -334,10 +472,31 @@ int mgc_coarse_batch_create(mgc_coarse_batch** out, mgc_coarse_op* const* ops, i
   });
 }
 
+int mgc_coarse_batch_create_fixed(mgc_coarse_batch** out, mgc_coarse_op* const* ops, int count,
+                                  int nrhs, int mrhs_prec, int steps, int graph, void* stream) {
+  return guarded([&] {
+    MGC_CHECK(out, MGC_ERR_ARG, "null output handle");
+    *out = nullptr;
+    check_batch_ops(ops, count, nrhs, mrhs_prec, steps);
+    MGC_CHECK(graph == 1 || graph == 2, MGC_ERR_ARG, "graph must be 1 or 2");
+    MGC_CHECK(graph == 1 || stream != nullptr, MGC_ERR_ARG,
+              "graph mode needs a created stream (the null stream cannot be captured)");
+    auto h = std::make_unique<mgc_coarse_batch>();
+    h->fixed = std::make_unique<FixedCoarseBatch>(ops, count, nrhs, mrhs_prec, steps, graph,
+                                                  as_stream(stream));
+    *out = h.release();
+  });
+}
+
 int mgc_coarse_batch_solve(mgc_coarse_batch* cb, const void* const* b, void* const* x, double tol,
                            int maxit, mgc_solve_stats* stats, void* stream) {
   return guarded([&] {
-    MGC_CHECK(cb && cb->impl, MGC_ERR_ARG, "null batch handle");
+    MGC_CHECK(cb && (cb->impl || cb->fixed), MGC_ERR_ARG, "null batch handle");
+    if (cb->fixed) {
+      MGC_CHECK(as_stream(stream) == cb->fixed->s, MGC_ERR_ARG, "use the stream given to create");
+      cb->fixed->solve(b, x, tol, stats);
+      return;
+    }
     MGC_CHECK(as_stream(stream) == cb->impl->s, MGC_ERR_ARG, "use the stream given to create");
     batch_solve(*cb->impl, b, x, tol, maxit, stats);
   });

@@ -38,7 +38,7 @@ struct DGmresState {
 // |w|^2 - |h1|^2 < eta |w|^2.  Without it, dh2 takes h1 and |w|^2 (so that
 // vsubnorm_dev(dh2) forms v = (w - V h1) / |w'|) and dh1 is zeroed (so that
 // the Hessenberg column dh1 + dh2 is h1); the skipped pass leaves dh2 alone.
-__global__ void k_dg_reorth_test(DGmresState* st, cplx<double>* dh1, cplx<double>* dh2, int j,
+static __global__ void k_dg_reorth_test(DGmresState* st, cplx<double>* dh1, cplx<double>* dh2, int j,
                                  double eta) {
   if (threadIdx.x != 0) return;
   double sum1 = 0.0;
@@ -55,7 +55,7 @@ __global__ void k_dg_reorth_test(DGmresState* st, cplx<double>* dh1, cplx<double
   }
 }
 
-__global__ void k_dg_begin(DGmresState* st) {
+static __global__ void k_dg_begin(DGmresState* st) {
   if (threadIdx.x != 0) return;
   for (int i = 0; i <= Blas::KMAX; ++i) st->g[i] = {0.0, 0.0};
   const double b2 = st->r2[0];
@@ -66,7 +66,7 @@ __global__ void k_dg_begin(DGmresState* st) {
 // column j of the Hessenberg matrix from the two projections (h1 at dh1[0..j],
 // h2 at dh2[0..j], the new norm at dh2[j+1].im), the earlier rotations, and
 // the rotation that zeroes H[j+1][j]
-__global__ void k_dg_step(DGmresState* st, const cplx<double>* dh1, const cplx<double>* dh2,
+static __global__ void k_dg_step(DGmresState* st, const cplx<double>* dh1, const cplx<double>* dh2,
                           int j) {
   if (threadIdx.x != 0) return;
   typedef std::complex<double> C;
@@ -105,7 +105,7 @@ __global__ void k_dg_step(DGmresState* st, const cplx<double>* dh1, const cplx<d
 }
 
 // y = H^-1 g over the columns in use (m at most), stored negated
-__global__ void k_dg_solve(DGmresState* st, int m) {
+static __global__ void k_dg_solve(DGmresState* st, int m) {
   if (threadIdx.x != 0) return;
   typedef std::complex<double> C;
   const int ld = Blas::KMAX;
@@ -213,7 +213,7 @@ class DGmres : public Solver<R> {
 // ---------------------------------------------------------------------------
 
 // g = (|r0|, 0, ...), and no Krylov space for a zero residual
-__global__ void k_dbg_begin(cplx<double>* g, int* jmax, const cplx<double>* r2, int nr, int m) {
+static __global__ void k_dbg_begin(cplx<double>* g, int* jmax, const cplx<double>* r2, int nr, int m) {
   const int r = threadIdx.x;
   if (r >= nr) return;
   for (int i = 0; i <= m; ++i) g[(size_t)r * (m + 1) + i] = {0.0, 0.0};
@@ -226,7 +226,7 @@ __global__ void k_dbg_begin(cplx<double>* g, int* jmax, const cplx<double>* r2, 
 // |w|^2_r at dh1[j+1][r].re): skip the second projection unless some rhs in
 // use has |w|^2 - |h1|^2 < eta |w|^2; when skipped, dh2 takes h1 and |w|^2
 // and dh1 is zeroed, as k_dg_reorth_test does
-__global__ void k_dbg_reorth_test(cplx<double>* dh1, cplx<double>* dh2, const int* jmax, int j,
+static __global__ void k_dbg_reorth_test(cplx<double>* dh1, cplx<double>* dh2, const int* jmax, int j,
                                   double eta, int nr, int* skip) {
   const int r = threadIdx.x;
   int need = 0;
@@ -252,7 +252,7 @@ __global__ void k_dbg_reorth_test(cplx<double>* dh1, cplx<double>* dh2, const in
 
 // column j of each rhs's Hessenberg matrix (h1 + h2, the new norm at
 // dh2[j+1][r].im), the earlier rotations and the new one, as k_dg_step
-__global__ void k_dbg_step(cplx<double>* H, double* cs, cplx<double>* sn, cplx<double>* g,
+static __global__ void k_dbg_step(cplx<double>* H, double* cs, cplx<double>* sn, cplx<double>* g,
                            int* jmax, const cplx<double>* dh1, const cplx<double>* dh2, int j,
                            int nr, int m) {
   typedef std::complex<double> C;
@@ -298,7 +298,7 @@ __global__ void k_dbg_step(cplx<double>* H, double* cs, cplx<double>* sn, cplx<d
 }
 
 // per rhs y = H^-1 g over its columns in use, stored negated as negy[i][r]
-__global__ void k_dbg_solve(const cplx<double>* H, const cplx<double>* g, const int* jmax,
+static __global__ void k_dbg_solve(const cplx<double>* H, const cplx<double>* g, const int* jmax,
                             cplx<double>* negy, int nr, int m) {
   typedef std::complex<double> C;
   const int r = threadIdx.x;
@@ -348,58 +348,81 @@ class DBGmres : public BSolver<R> {
   // m steps from x (or 0) for every rhs; tol and maxit are not used
   BKrylovStats solve(BBlas& bl, BLinOp<R>& A, BPrec<R>* P, const cplx<R>* b, cplx<R>* x,
                      bool x_zero, double, int) override {
-    hipStream_t s = bl.s;
-    const size_t n = sh_.n;
-    const int nr = sh_.nrhs, m = m_;
-    cplx<R>* V = V_.p;
+    const int nr = sh_.nrhs;
     BKrylovStats st;
     st.rel_res.assign(nr, 0.0);
     st.converged.assign(nr, 0);
-    st.iters_rhs.assign(nr, m);
+    st.iters_rhs.assign(nr, m_);
+    if (!x_zero) {
+      A.apply(x, w_.p, bl.s);
+      ++st.applies;
+    }
+    begin(bl, b, x, x_zero);
+    for (int j = 0; j < m_; ++j) {
+      A.apply(direction(j, P, bl.s), w_.p, bl.s);
+      ++st.applies;
+      arnoldi(bl, j);
+      ++st.iters;
+    }
+    finish(bl, x);
+    return st;
+  }
+
+  // The same solve in phases, for a driver that applies the operators of
+  // several solvers in one launch: begin (with w = A x already formed unless
+  // x_zero), then per step j the operator applied to direction(j) into
+  // product() and arnoldi(j), then finish.
+  void begin(BBlas& bl, const cplx<R>* b, cplx<R>* x, bool x_zero) {
+    hipStream_t s = bl.s;
+    const size_t n = sh_.n;
+    cplx<R>* V = V_.p;
     if (x_zero) {
       vzero<R>(x, n, s);
       vcopy<R>(b, V, n, s);
     } else {
-      A.apply(x, w_.p, s);
-      ++st.applies;
       vsub<R>(b, w_.p, V, n, s);
     }
     bdots_dev<R>(bl, V, n, 0, V, sh_, true, r2_.p);  // |r0|^2 per rhs
-    k_dbg_begin<<<1, 64, 0, s>>>(g_.p, jmax_.p, r2_.p, nr, m);
+    k_dbg_begin<<<1, 64, 0, s>>>(g_.p, jmax_.p, r2_.p, sh_.nrhs, m_);
     MGC_LAUNCH_CHECK();
     bscale_dev<R>(V, sh_, r2_.p, s);  // v0 = r0 / |r0|
-    cplx<double>* dh1 = dh_.p;
-    for (int j = 0; j < m; ++j) {
-      cplx<R>* vj = V + (size_t)j * n;
-      const cplx<R>* z = vj;
-      if (flexible_) {
-        cplx<R>* zj = Z_.p + (size_t)j * n;
-        if (P) P->apply(vj, zj, s);
-        else vcopy<R>(vj, zj, n, s);
-        z = zj;
-      }
-      A.apply(z, w_.p, s);
-      ++st.applies;
-      cplx<double>* dh2 = dh_.p + (size_t)(j + 2) * nr;
-      bdots_dev<R>(bl, V, n, j + 1, w_.p, sh_, true, dh1);
-      if (j + 1 <= 24) {
-        k_dbg_reorth_test<<<1, 64, 0, s>>>(dh1, dh2, jmax_.p, j, eta_, nr, skip_.p);
-        MGC_LAUNCH_CHECK();
-        bmultisub_dots_dev<R>(bl, dh1, V, n, j + 1, w_.p, sh_, dh2, skip_.p);
-      } else {
-        bmultisub_dev<R>(dh1, V, n, j + 1, w_.p, sh_, s);
-        bdots_dev<R>(bl, V, n, j + 1, w_.p, sh_, true, dh2);
-      }
-      bsubnorm_dev<R>(dh2, V, n, j + 1, w_.p, V + (size_t)(j + 1) * n, sh_, s);
-      k_dbg_step<<<1, 64, 0, s>>>(H_.p, cs_.p, sn_.p, g_.p, jmax_.p, dh1, dh2, j, nr, m);
-      MGC_LAUNCH_CHECK();
-      ++st.iters;
-    }
-    k_dbg_solve<<<1, 64, 0, s>>>(H_.p, g_.p, jmax_.p, negy_.p, nr, m);
-    MGC_LAUNCH_CHECK();
-    bmultisub_dev<R>(negy_.p, flexible_ ? Z_.p : V, n, m, x, sh_, s);  // x += sum y_i Z_i
-    return st;
   }
+  const cplx<R>* direction(int j, BPrec<R>* P, hipStream_t s) {
+    cplx<R>* vj = V_.p + (size_t)j * sh_.n;
+    if (!flexible_) return vj;
+    cplx<R>* zj = Z_.p + (size_t)j * sh_.n;
+    if (P) P->apply(vj, zj, s);
+    else vcopy<R>(vj, zj, sh_.n, s);
+    return zj;
+  }
+  cplx<R>* product() { return w_.p; }
+  void arnoldi(BBlas& bl, int j) {
+    hipStream_t s = bl.s;
+    const size_t n = sh_.n;
+    const int nr = sh_.nrhs;
+    cplx<R>* V = V_.p;
+    cplx<double>* dh1 = dh_.p;
+    cplx<double>* dh2 = dh_.p + (size_t)(j + 2) * nr;
+    bdots_dev<R>(bl, V, n, j + 1, w_.p, sh_, true, dh1);
+    if (j + 1 <= 24) {
+      k_dbg_reorth_test<<<1, 64, 0, s>>>(dh1, dh2, jmax_.p, j, eta_, nr, skip_.p);
+      MGC_LAUNCH_CHECK();
+      bmultisub_dots_dev<R>(bl, dh1, V, n, j + 1, w_.p, sh_, dh2, skip_.p);
+    } else {
+      bmultisub_dev<R>(dh1, V, n, j + 1, w_.p, sh_, s);
+      bdots_dev<R>(bl, V, n, j + 1, w_.p, sh_, true, dh2);
+    }
+    bsubnorm_dev<R>(dh2, V, n, j + 1, w_.p, V + (size_t)(j + 1) * n, sh_, s);
+    k_dbg_step<<<1, 64, 0, s>>>(H_.p, cs_.p, sn_.p, g_.p, jmax_.p, dh1, dh2, j, nr, m_);
+    MGC_LAUNCH_CHECK();
+  }
+  void finish(BBlas& bl, cplx<R>* x) {
+    k_dbg_solve<<<1, 64, 0, bl.s>>>(H_.p, g_.p, jmax_.p, negy_.p, sh_.nrhs, m_);
+    MGC_LAUNCH_CHECK();
+    bmultisub_dev<R>(negy_.p, flexible_ ? Z_.p : V_.p, sh_.n, m_, x, sh_, bl.s);  // x += Z y
+  }
+  int steps() const { return m_; }
+  const BShape& shape() const { return sh_; }
 
  private:
   BShape sh_;

@@ -126,6 +126,7 @@ _SIGS = {
     "mgc_coarse_batch_create": ([_vp, _vp, _i, _i, _i, _i, _vp], _i),
     "mgc_coarse_batch_solve": ([_vp, _vp, _vp, _c.c_double, _i, _vp, _vp], _i),
     "mgc_coarse_batch_destroy": ([_vp], _i),
+    "mgc_coarse_batch_create_fixed": ([_vp, _vp, _i, _i, _i, _i, _i, _vp], _i),
     "mgc_coarse_op_M_mrhs_noalloc": ([_vp, _vp, _vp, _i, _i, _i, _vp], _i),
     "mgc_quantize": ([_vp, _sz, _i, _i, _vp, _vp, _vp], _i),
     "mgc_mrhs_set_variant": ([_i], _i),
@@ -493,13 +494,21 @@ class CoarseBatch:
     """A prepared lockstep coarse solve over several lattices (mgc_coarse_batch):
     the workspaces and Krylov basis are kept between solves."""
 
-    def __init__(self, ops, nrhs, mrhs_prec=F32, restart=20, stream=None):
+    def __init__(self, ops, nrhs, mrhs_prec=F32, restart=20, stream=None, fixed_steps=0,
+                 graph=2):
+        """fixed_steps > 0: mgc_coarse_batch_create_fixed with that step count
+        and graph mode (1 eager, 2 captured HIP graph)."""
         self.ops, self.nrhs, self.stream = list(ops), int(nrhs), stream
         arr = ctypes.c_void_p * len(self.ops)
         h = _vp()
-        call("mgc_coarse_batch_create", _c.byref(h), arr(*[o.handle for o in self.ops]),
-             len(self.ops), self.nrhs, int(mrhs_prec), int(restart),
-             stream.handle if stream else None)
+        if fixed_steps:
+            call("mgc_coarse_batch_create_fixed", _c.byref(h),
+                 arr(*[o.handle for o in self.ops]), len(self.ops), self.nrhs, int(mrhs_prec),
+                 int(fixed_steps), int(graph), stream.handle if stream else None)
+        else:
+            call("mgc_coarse_batch_create", _c.byref(h), arr(*[o.handle for o in self.ops]),
+                 len(self.ops), self.nrhs, int(mrhs_prec), int(restart),
+                 stream.handle if stream else None)
         self.handle = h.value
 
     def solve(self, bs, xs, tol=1e-6, maxit=200) -> list:

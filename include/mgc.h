@@ -559,6 +559,19 @@ int mgc_coarse_batch_create(mgc_coarse_batch** out, mgc_coarse_op* const* ops, i
 int mgc_coarse_batch_solve(mgc_coarse_batch* cb, const void* const* b, void* const* x, double tol,
                            int maxit, mgc_solve_stats* stats, void* stream);
 int mgc_coarse_batch_destroy(mgc_coarse_batch* cb);
+/* A prepared batch with a fixed step count (no counterpart in the reference,
+ * whose coarsest solves stop at a tolerance): every lattice runs `steps`
+ * GMRES steps from x = 0 (1 <= steps < 64), all lattices in lockstep on the
+ * stream with one multi-lattice stencil launch per step and no host
+ * synchronisation.  graph = 2 captures the whole solve into a HIP graph at
+ * the first mgc_coarse_batch_solve (after one eager solve) and replays it
+ * after, so the stream must be a created one; graph = 1 runs the same work
+ * eagerly (the same results, bit for bit).  mgc_coarse_batch_solve then
+ * copies b into the handle's staging vectors and x out of them around the
+ * graph, ignores maxit, reports iterations = steps and true_residual
+ * ||b - M x|| / ||b|| per (lattice, rhs), and converged = true_residual <= tol. */
+int mgc_coarse_batch_create_fixed(mgc_coarse_batch** out, mgc_coarse_op* const* ops, int count,
+                                  int nrhs, int mrhs_prec, int steps, int graph, void* stream);
 /* the FP64 coarse operator of level >= 1 and the aggregation of level >= 0 (owned by mg) */
 int mgc_mg_coarse_op(mgc_mg* mg, int level, mgc_coarse_op** cop);
 int mgc_mg_aggregation(mgc_mg* mg, int level, mgc_aggregation** agg);

@@ -163,3 +163,75 @@ def test_batched_coarse_solve_lattice_leaving_early(gpu, problem):
         a = xs[k].to_numpy(np.complex64, shp)
         b = x1.to_numpy(np.complex64, shp)
         assert np.array_equal(a.view(np.float32), b.view(np.float32)), k
+
+
+def fixed_solve(gpu, ops, bdevs, idx, prec, steps, graph, stream):
+    cb = gpu.CoarseBatch([ops[k] for k in idx], NRHS, mrhs_prec=prec, stream=stream,
+                         fixed_steps=steps, graph=graph)
+    try:
+        xs = [gpu.DeviceBuffer(int(np.prod(LATS[k])) * NRHS * N * 8) for k in idx]
+        st = cb.solve([bdevs[k] for k in idx], xs, tol=1e-4)
+        gpu.synchronize()
+        out = [x.to_numpy(np.complex64, (int(np.prod(LATS[k])), NRHS, N)) for x, k in zip(xs, idx)]
+        # a second solve on the same handle (in graph mode a replay) with other rhs
+        b2 = [gpu.DeviceBuffer.from_numpy((2.0 * bdevs[k].to_numpy(np.complex64,
+                                                                  (int(np.prod(LATS[k])), NRHS, N)))
+                                          .astype(np.complex64)) for k in idx]
+        xs2 = [gpu.DeviceBuffer(int(np.prod(LATS[k])) * NRHS * N * 8) for k in idx]
+        cb.solve(b2, xs2, tol=1e-4)
+        gpu.synchronize()
+        out2 = [x.to_numpy(np.complex64, (int(np.prod(LATS[k])), NRHS, N)) for x, k in zip(xs2, idx)]
+    finally:
+        cb.close()
+    return st, out, out2
+
+
+def test_fixed_step_batch_graph_equals_eager_and_separate(gpu, problem):
+    """The fixed-step batch (32 GMRES steps on every lattice in lockstep, one
+    multi-lattice launch per step): the captured and replayed HIP graph gives
+    the x of the eager form bit for bit, on its first solve and on a replay
+    with other right-hand sides; each lattice gets the x of a fixed-step batch
+    of that lattice alone; and 32 steps bring every column's residual, against
+    the FP64 oracle's operator, below 1e-4."""
+    ops, links, bs, bdev = problem
+    s = gpu.Stream()
+    steps = 32
+    print("[case] fixed-step coarse batch F32, graph 1 and 2", flush=True)
+    st1, x1, y1 = fixed_solve(gpu, ops, bdev, [0, 1, 2], F32, steps, 1, s)
+    st2, x2, y2 = fixed_solve(gpu, ops, bdev, [0, 1, 2], F32, steps, 2, s)
+    for k in range(3):
+        assert np.array_equal(x1[k].view(np.float32), x2[k].view(np.float32)), k
+        assert np.array_equal(y1[k].view(np.float32), y2[k].view(np.float32)), k
+    assert all(st["iterations"] == steps for st in st2)
+    worst = 0.0
+    for k, d in enumerate(LATS):
+        Mx = oracle.coarse_mrhs_M(d, N, NRHS, links[k], x2[k].astype(np.complex128))
+        for r in range(NRHS):
+            res = rel(bs[k][:, r].astype(np.complex128), Mx[:, r])
+            worst = max(worst, res)
+            assert res < 1e-4, (k, r, res)
+            assert abs(st2[k * NRHS + r]["true_residual"] - res) < 1e-5, (k, r)
+    print(f"  worst residual after {steps} steps {worst:.3e}", flush=True)
+    for k in range(3):
+        _, xk, _ = fixed_solve(gpu, ops, bdev, [k], F32, steps, 2, s)
+        assert np.array_equal(xk[0].view(np.float32), x2[k].view(np.float32)), k
+
+
+def test_fixed_step_batch_graph_fp16(gpu, problem):
+    ops, links, bs, bdev = problem
+    s = gpu.Stream()
+    print("[case] fixed-step coarse batch F16, graph 2", flush=True)
+    st, xs, _ = fixed_solve(gpu, ops, bdev, [0, 1, 2], F16, 24, 2, s)
+    for k, d in enumerate(LATS):
+        Mx = oracle.coarse_mrhs_M(d, N, NRHS, links[k], xs[k].astype(np.complex128))
+        for r in range(NRHS):
+            res = rel(bs[k][:, r].astype(np.complex128), Mx[:, r])
+            assert res < 1e-2, (k, r, res)
+
+
+def test_fixed_step_batch_rejects_bad_arguments(gpu, problem):
+    ops, links, bs, bdev = problem
+    with pytest.raises(gpu.MgcError, match="graph must be 1 or 2"):
+        gpu.CoarseBatch([ops[0]], NRHS, fixed_steps=8, graph=3, stream=gpu.Stream())
+    with pytest.raises(gpu.MgcError, match="null stream cannot be captured"):
+        gpu.CoarseBatch([ops[0]], NRHS, fixed_steps=8, graph=2, stream=None)

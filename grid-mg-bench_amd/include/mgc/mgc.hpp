@@ -273,6 +273,54 @@ inline void M_mrhs_batched(const std::vector<CoarsenedMatrix*>& ops,
                                      out.data(), nrhs, mrhs_prec, dag, stream));
 }
 
+// Coarse solves of several lattices of one nbasis in one lockstep batch
+// (mgc_coarse_batch_*, no counterpart in the reference, which solves one
+// lattice at a time): GMRES(restart) to a tolerance, or with steps > 0 a
+// fixed number of GMRES steps, captured into a HIP graph (graph = 2, a
+// created stream) or issued eagerly (graph = 1).  Every b and x is
+// [site][nrhs][N] complex FP32; the operators must outlive the batch.
+class CoarseSolveBatch {
+ public:
+  CoarseSolveBatch(const std::vector<CoarsenedMatrix*>& ops, int nrhs, int mrhs_prec = MGC_F32,
+                   int restart = 20, void* stream = nullptr, int steps = 0, int graph = 2)
+      : nrhs_(nrhs), count_((int)ops.size()), stream_(stream) {
+    std::vector<mgc_coarse_op*> h;
+    for (CoarsenedMatrix* op : ops) h.push_back(op->handle());
+    if (steps > 0)
+      check(mgc_coarse_batch_create_fixed(&h_, h.data(), count_, nrhs, mrhs_prec, steps, graph,
+                                          stream));
+    else
+      check(mgc_coarse_batch_create(&h_, h.data(), count_, nrhs, mrhs_prec, restart, stream));
+  }
+  ~CoarseSolveBatch() {
+    if (h_) (void)mgc_coarse_batch_destroy(h_);
+  }
+  CoarseSolveBatch(const CoarseSolveBatch&) = delete;
+  CoarseSolveBatch& operator=(const CoarseSolveBatch&) = delete;
+
+  // one entry per (lattice, rhs), lattice-major
+  std::vector<mgc_solve_stats> Solve(const std::vector<const DeviceField*>& b,
+                                     const std::vector<DeviceField*>& x, double tol,
+                                     int maxit = 300) {
+    if ((int)b.size() != count_ || (int)x.size() != count_)
+      throw Exception(MGC_ERR_ARG, "one b and one x per lattice");
+    std::vector<const void*> bp;
+    std::vector<void*> xp;
+    for (int k = 0; k < count_; ++k) {
+      bp.push_back(b[k]->data());
+      xp.push_back(x[k]->data());
+    }
+    std::vector<mgc_solve_stats> st((size_t)count_ * nrhs_);
+    check(mgc_coarse_batch_solve(h_, bp.data(), xp.data(), tol, maxit, st.data(), stream_));
+    return st;
+  }
+
+ private:
+  mgc_coarse_batch* h_ = nullptr;
+  int nrhs_, count_;
+  void* stream_;
+};
+
 class MultiGridPreconditioner {
  public:
   MultiGridPreconditioner(WilsonCloverOperator& fine, const mgc_mg_params& params,

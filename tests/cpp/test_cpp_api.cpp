@@ -103,6 +103,33 @@ int main() {
     Dc.MooeeInv(wc, uc);
     expect("coarse: MooeeInv Mooee - 1", rel(host(vc), host(uc)), 1e-11);
 
+    // coarse solves in a batch: GMRES to 1e-5, and 24 fixed steps as a HIP graph
+    {
+      const int nrhs = 2;
+      void* cs = nullptr;
+      mgc::check(mgc_stream_create(&cs));
+      {
+        mgc::DeviceField bc((size_t)Vc * nrhs * N, MGC_F32), x1((size_t)Vc * nrhs * N, MGC_F32),
+            x2((size_t)Vc * nrhs * N, MGC_F32);
+        bc.gaussian(seed, 6, cs);
+        mgc::CoarseSolveBatch tolb({&Dc}, nrhs, MGC_F32, 20, cs);
+        mgc::CoarseSolveBatch fixb({&Dc}, nrhs, MGC_F32, 20, cs, 24, 2);
+        const std::vector<mgc_solve_stats> s1 = tolb.Solve({&bc}, {&x1}, 1e-5);
+        const std::vector<mgc_solve_stats> s2 = fixb.Solve({&bc}, {&x2}, 1e-5);
+        double r1 = 0, r2 = 0;
+        for (int r = 0; r < nrhs; ++r) {
+          r1 = std::fmax(r1, s1[r].true_residual);
+          r2 = std::fmax(r2, s2[r].true_residual);
+        }
+        std::printf("coarse batch: GMRES %d iterations residual %.2e; 24 fixed steps residual "
+                    "%.2e\n", s1[0].iterations, r1, r2);
+        expect("coarse batch: GMRES residual", r1, 2e-5);
+        expect("coarse batch: fixed-step graph residual", r2, 1e-4);
+        mgc::check(mgc_stream_synchronize(cs));
+      }  // the batches and fields go before their stream
+      mgc::check(mgc_stream_destroy(cs));
+    }
+
     // two-level hierarchy and a solve to 1e-10
     mgc_mg_params p;
     mgc::check(mgc_mg_params_default(&p));

@@ -105,7 +105,7 @@ __global__ __launch_bounds__(256) void k_quantize(const float2* __restrict__ in,
                                                   size_t rows, void* __restrict__ out,
                                                   float* __restrict__ scales) {
   typedef typename MK<KIND>::T T;
-  // one wavefront per row, lanes over N (N <= 64... generic loop)
+  // one wavefront per row, lanes over N (any N)
   const size_t row = blockIdx.x * (size_t)(blockDim.x / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= rows) return;
@@ -121,6 +121,37 @@ __global__ __launch_bounds__(256) void k_quantize(const float2* __restrict__ in,
     o[2 * c + 1] = (T)(r[c].y * inv);
   }
   if (lane == 0) scales[row] = s;
+}
+
+// The same for even N <= 64 and aligned rows, with the same results (the
+// maximum does not depend on the order it is taken in):
+template <int KIND>
+__global__ __launch_bounds__(256) void k_quantize_pairs(const float2* __restrict__ in, int N,
+                                                        size_t rows, void* __restrict__ out,
+                                                        float* __restrict__ scales) {
+  typedef typename MK<KIND>::T T;
+  // half a wavefront per row: lane c2 (0..31) takes the complex pair
+  // (2 c2, 2 c2 + 1) of the row in one 16-byte load (N even, at most 64),
+  // so a wavefront reads two whole rows contiguously
+  const size_t row = blockIdx.x * (size_t)(blockDim.x / 32) + (threadIdx.x >> 5);
+  const int c2 = threadIdx.x & 31;
+  const bool in_row = row < rows;  // the half-wave of a missing row still joins the shuffles
+  const bool has = in_row && 2 * c2 < N;
+  float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (has) v = reinterpret_cast<const float4*>(in + row * N)[c2];
+  float m = fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w)));
+  for (int off = 16; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));  // within the half
+  if (!has) {
+    if (in_row && c2 == 0) scales[row] = m > 0.f ? m : 1.f;
+    return;
+  }
+  const float s = m > 0.f ? m : 1.f;
+  const float inv = 1.f / s;
+  T q[4] = {(T)(v.x * inv), (T)(v.y * inv), (T)(v.z * inv), (T)(v.w * inv)};
+  uint2 w;
+  __builtin_memcpy(&w, q, 8);
+  reinterpret_cast<uint2*>(reinterpret_cast<T*>(out) + row * 2 * N)[c2] = w;
+  if (c2 == 0) scales[row] = s;
 }
 
 // runtime nbasis -> the instantiation of that nbasis
@@ -298,8 +329,13 @@ static void prepare_mrhs(mgc_coarse_op* op, int mp, hipStream_t s) {
 
 static void quantize(const void* in, size_t rows, int N, int prec, void* out, float* scales,
                      hipStream_t s) {
-  const unsigned g = (unsigned)((rows + 3) / 4);
-  if (prec == MGC_F16)
+  const bool pairs = N % 2 == 0 && N <= 64 && (uintptr_t)in % 16 == 0 && (uintptr_t)out % 8 == 0;
+  const unsigned g = (unsigned)(pairs ? (rows + 7) / 8 : (rows + 3) / 4);
+  if (prec == MGC_F16 && pairs)
+    k_quantize_pairs<MGC_F16><<<g, 256, 0, s>>>((const float2*)in, N, rows, out, scales);
+  else if (prec == MGC_BF16 && pairs)
+    k_quantize_pairs<MGC_BF16><<<g, 256, 0, s>>>((const float2*)in, N, rows, out, scales);
+  else if (prec == MGC_F16)
     k_quantize<MGC_F16><<<g, 256, 0, s>>>((const float2*)in, N, rows, out, scales);
   else if (prec == MGC_BF16)
     k_quantize<MGC_BF16><<<g, 256, 0, s>>>((const float2*)in, N, rows, out, scales);

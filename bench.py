@@ -450,6 +450,11 @@ def mg_batched_leg(mgc, stream, dims, blocks, nbs, nrhs, coarse_prec, tol=1e-10)
     mgg = mgc.Multigrid(fop, pg)
     mgg.setup(stream)
     xg = mgc.DeviceBuffer(V * nrhs * 12 * 16)
+    # one outer iteration first: the capture and instantiation of the graph
+    # (at the first preconditioner application) are timed apart
+    t0 = time.perf_counter()
+    mgg.solve_mrhs(b, xg, nrhs, coarse_prec=coarse_prec, tol=tol, maxit=1, restart=20)
+    t_first_g = time.perf_counter() - t0
     stg = mgg.solve_mrhs(b, xg, nrhs, coarse_prec=coarse_prec, tol=tol, maxit=500, restart=20)
     mgg.close()
     for buf in (g, b, x, b1, x1, xg):
@@ -475,6 +480,7 @@ def mg_batched_leg(mgc, stream, dims, blocks, nbs, nrhs, coarse_prec, tol=1e-10)
         "graph": {
             "graph_steps": {"smoother": pg.smoother_iters, "wrapper": pg.wrapper_restart,
                             "coarsest": pg.coarsest_restart},
+            "first_outer_iteration_s": t_first_g,
             "solve_s": stg[0]["seconds"],
             "solve_s_per_rhs": stg[0]["seconds"] / nrhs,
             "outer_iterations_per_rhs": [st["iterations"] for st in stg],

@@ -31,7 +31,7 @@ BBlas::~BBlas() {
   if (coef_done) (void)hipEventDestroy(coef_done);
 }
 
-static unsigned bew_grid(size_t n) { return grid_for(n, 256, 8192); }
+static unsigned bew_grid(size_t n) { return grid_for(n, 256, 65536); }
 
 static size_t gcd_sz(size_t a, size_t b) {
   while (b) {

@@ -32,7 +32,7 @@ Blas::~Blas() {
   if (coef_done) (void)hipEventDestroy(coef_done);
 }
 
-static unsigned ew_grid(size_t n) { return grid_for(n, 256, 8192); }
+static unsigned ew_grid(size_t n) { return grid_for(n, 256, 65536); }
 
 template <typename R>
 __global__ void k_vzero(cplx<R>* x, size_t n) {

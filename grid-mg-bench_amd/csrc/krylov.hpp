@@ -21,7 +21,10 @@
 #include <complex>
 #include <vector>
 
+#include <memory>
+
 #include "blas.hpp"
+#include "bkrylov.hpp"
 
 namespace mgc {
 
@@ -236,93 +239,46 @@ class Gmres : public Solver<R> {
 
 // Unpreconditioned BiCGSTAB (van der Vorst), the setup solver of
 // include/MultigridParams.h:106-107 and a selectable solver type of
-// include/Helpers.h:38-56.  One host read per scalar.  With b = 0 (the
-// null-vector setup, include/BasisVectors.h:99-105: solve A psi = 0 from
-// psi = v) there is no relative tolerance to meet, so exactly maxit
-// iterations relax the initial guess.
+// include/Helpers.h:38-56: the lockstep BBiCgStab with one right-hand side,
+// so the recurrence, the stopping rules (tolerance, maxit, breakdowns) and
+// the fused update passes (18 vector passes per step) are those of the
+// batched set-up.  With b = 0 (the null-vector setup,
+// include/BasisVectors.h:99-105: solve A psi = 0 from psi = v) there is no
+// relative tolerance to meet, so exactly maxit iterations relax the initial
+// guess.
 template <typename R>
 class BiCgStab : public Solver<R> {
  public:
   explicit BiCgStab(size_t n) : n_(n) {
-    for (int i = 0; i < 6; ++i) MGC_HIP(hipMalloc(&w_[i], n * sizeof(cplx<R>)));
+    sh_.n = n;
+    sh_.nrhs = 1;
+    sh_.dof = n % 2 == 0 ? 2 : 1;  // even: the FP32 reductions take element pairs
+    solver_ = std::make_unique<BBiCgStab<R>>(sh_);
   }
-  ~BiCgStab() {
-    for (int i = 0; i < 6; ++i)
-      if (w_[i]) (void)hipFree(w_[i]);
-  }
-  BiCgStab(const BiCgStab&) = delete;
-  BiCgStab& operator=(const BiCgStab&) = delete;
 
   KrylovStats solve(Blas& bl, LinOp<R>& A, Prec<R>*, const cplx<R>* b, cplx<R>* x, bool x_zero,
                     double tol, int maxit) override {
-    typedef std::complex<double> C;
-    hipStream_t s = bl.s;
-    const size_t n = n_;
-    cplx<R>*r = w_[0], *rh = w_[1], *p = w_[2], *v = w_[3], *sv = w_[4], *t = w_[5];
+    if (!bbl_ || bbl_->s != bl.s) bbl_ = std::make_unique<BBlas>(bl.s);
+    struct One : BLinOp<R> {
+      LinOp<R>* A;
+      void apply(const cplx<R>* in, cplx<R>* out, hipStream_t s) override { A->apply(in, out, s); }
+    } one;
+    one.sh = sh_;
+    one.A = &A;
+    const BKrylovStats b1 = solver_->solve(*bbl_, one, nullptr, b, x, x_zero, tol, maxit);
     KrylovStats st;
-    const double b2 = vnorm2<R>(bl, b, n);
-    const bool relax = !(b2 > 0.0);  // b = 0: fixed iteration count
-    if (relax && x_zero) {
-      vzero<R>(x, n, s);
-      st.converged = true;
-      return st;
-    }
-    if (x_zero) {
-      vzero<R>(x, n, s);
-      vcopy<R>(b, r, n, s);
-    } else {
-      A.apply(x, t, s);
-      ++st.applies;
-      vsub<R>(b, t, r, n, s);
-    }
-    vcopy<R>(r, rh, n, s);
-    vzero<R>(p, n, s);
-    vzero<R>(v, n, s);
-    C rho(1.0), alpha(1.0), omega(1.0);
-    const double stop = tol * tol * b2;
-    double r2 = vnorm2<R>(bl, r, n);
-    auto c = [](C z) { return cplx<double>{z.real(), z.imag()}; };
-    auto d = [](cplx<double> z) { return C(z.re, z.im); };
-    while (true) {
-      st.rel_res = relax ? std::sqrt(r2) : std::sqrt(r2 / b2);
-      if (!relax && r2 <= stop) {
-        st.converged = true;
-        break;
-      }
-      if (st.iters >= maxit) break;
-      const C rho1 = d(vdot<R>(bl, rh, r, n));
-      if (std::abs(rho1) == 0.0 || std::abs(omega) == 0.0) break;  // breakdown
-      const C beta = (rho1 / rho) * (alpha / omega);
-      rho = rho1;
-      // p = r + beta (p - omega v)
-      vaxpy<R>(c(-omega), v, p, n, s);
-      vxpay<R>(r, c(beta), p, n, s);
-      A.apply(p, v, s);
-      const C rhv = d(vdot<R>(bl, rh, v, n));
-      if (std::abs(rhv) == 0.0) break;
-      alpha = rho / rhv;
-      // s = r - alpha v
-      vcopy<R>(r, sv, n, s);
-      vaxpy<R>(c(-alpha), v, sv, n, s);
-      A.apply(sv, t, s);
-      st.applies += 2;
-      const C ts = d(vdot<R>(bl, t, sv, n));
-      const double tt = vnorm2<R>(bl, t, n);
-      omega = tt > 0.0 ? ts / tt : C(0.0);
-      // x += alpha p + omega s ; r = s - omega t
-      vaxpy<R>(c(alpha), p, x, n, s);
-      vaxpy<R>(c(omega), sv, x, n, s);
-      vcopy<R>(sv, r, n, s);
-      vaxpy<R>(c(-omega), t, r, n, s);
-      r2 = vnorm2<R>(bl, r, n);
-      ++st.iters;
-    }
+    st.iters = b1.iters_rhs[0];
+    st.applies = b1.applies;
+    st.rel_res = b1.rel_res[0];
+    st.converged = b1.converged[0] != 0;
     return st;
   }
 
  private:
   size_t n_;
-  cplx<R>* w_[6] = {};
+  BShape sh_;
+  std::unique_ptr<BBiCgStab<R>> solver_;
+  std::unique_ptr<BBlas> bbl_;
 };
 
 // Minimal residual iteration (over-relaxation 1), the MR solver type of

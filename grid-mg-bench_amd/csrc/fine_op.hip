@@ -380,9 +380,9 @@ __device__ __forceinline__ void wilson_copy(uint4* s_rec, const uint4* __restric
 // which consecutive lanes read consecutive chunks (coalesced, unlike a
 // thread reading its own record with a stride of one record per lane).  The
 // chunks go through registers into LDS, each thread then reads its own
-// record there; the copies of the next hop are issued before the arithmetic
-// of the current one.  Hops run self first, then 0..7; the neighbour sites
-// of the next hop are staged in LDS one hop ahead.  Records are padded by
+// record there.  Hops run self first, then 0..7; the neighbour sites of the
+// next hop are staged in LDS one hop ahead, and a hop's link loads are
+// issued before its record copy.  Records are padded by
 // one chunk (13 for FP64, 7 for FP32), so that the records that
 // neighbouring lanes read fall in different banks.  The result leaves through LDS in the
 // same coalesced way.  Threads past the lattice copy site 0 and do no
@@ -487,6 +487,127 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
   }
 }
 
+// Multi-rhs form with the next hop's loads in flight during the current
+// hop's arithmetic (nrhs >= 9, so a tile has at most 28 sites and each of
+// the hop's 9 x S link entries is loaded by one thread).  The tile's 8 x S
+// neighbour sites are read once, coalesced, into LDS.  Each stage then
+// stores the records and link entries that were loaded during the previous
+// stage into LDS, issues the loads of the next stage into registers (six
+// 16-byte record chunks and one link entry per thread), and computes from
+// LDS; the link of a site is loaded once for all its rhs instead of by each
+// rhs's thread.  The self stage's "link" is clover slots 0..8 of block 0.
+// Record layout, output path and per-thread arithmetic are those of
+// k_wilson_lds.
+template <typename R>
+__global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U,
+                                                   const cplx<R>* __restrict__ diag,
+                                                   const cplx<R>* __restrict__ in,
+                                                   cplx<R>* __restrict__ out,
+                                                   const int* __restrict__ nbr, int V, int sv,
+                                                   int nrhs, unsigned mask, int dag,
+                                                   WilsonTiles wt) {
+  constexpr int CPS = 12 * (int)sizeof(cplx<R>) / 16;
+  constexpr int RS = CPS + 1;
+  constexpr int SMAX = 28;  // sites per tile at nrhs >= 9
+  __shared__ uint4 s_rec[256 * RS];
+  __shared__ cplx<R> s_lk[9 * SMAX];  // [k][site]
+  __shared__ int s_nb[9][SMAX];       // [hop][site], hop 8 = the site itself
+  const int T = (int)blockDim.x;
+  const int S = T / nrhs;
+  const int t = (int)threadIdx.x;
+  const int x0 = wilson_tile_site(wt, (int)blockIdx.x);
+  const int sl = t / nrhs;
+  const int x = x0 + sl;
+  const bool live = x < V;
+  const GammaTab G = gamma_table();
+
+  if (t < 8 * S) {  // t = s * 8 + h: consecutive lanes read consecutive entries
+    const int s = t >> 3, h = t & 7;
+    s_nb[h][s] = x0 + s < V ? nbr[(size_t)(x0 + s) * 8 + h] : 0;
+  }
+  if (t < S) s_nb[8][t] = x0 + t < V ? x0 + t : 0;
+  auto next_hop = [&](int i) {
+    for (int k = i + 1; k < 9; ++k) {
+      const int h = k == 0 ? 8 : k - 1;
+      if ((mask >> h) & 1u) return h;
+    }
+    return -1;
+  };
+  // native 4-vectors: copies of uint4 (a union type) are memcpys that keep
+  // the staging array out of registers
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4* in4 = reinterpret_cast<const u32x4*>(in);
+  u32x4* s_rec4 = reinterpret_cast<u32x4*>(s_rec);
+  u32x4 nrec[CPS];
+  cplx<R> nlk = {R(0), R(0)};
+  // loads of hop h into registers: record chunks c = t + k*T of the tile's
+  // records, and link entry (k = t / S, site t % S) for t < 9 S
+  auto issue = [&](int h) {
+#pragma unroll
+    for (int k = 0; k < CPS; ++k) {
+      const int c = t + k * T, q = c / CPS, e = c - q * CPS, s = q / nrhs, r = q - s * nrhs;
+      nrec[k] = in4[((size_t)s_nb[h][s] * nrhs + r) * CPS + e];
+    }
+    if (t < 9 * S) {
+      const int k = t / S, s = t - k * S;
+      const cplx<R>* u = h == 8 ? diag + s_nb[8][s]
+                                : U + (size_t)(h & 3) * 9 * sv + (h < 4 ? s_nb[8][s] : s_nb[h][s]);
+      nlk = u[(size_t)k * sv];
+    }
+  };
+  auto rec = [&](int e) {
+    cplx<R> v = reinterpret_cast<const cplx<R>*>(&s_rec[rec_pos<CPS, RS>(t, e * CPS / 12)])
+        [(e * CPS) % 12 / 6];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    return v;
+  };
+
+  cplx<R> acc[12];
+#pragma unroll
+  for (int e = 0; e < 12; ++e) acc[e] = {R(0), R(0)};
+  __syncthreads();  // s_nb
+  const int h0 = next_hop(-1);
+  if (h0 >= 0) issue(h0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    const int h = i == 0 ? 8 : i - 1;
+    if (!((mask >> h) & 1u)) continue;
+    const int hn = next_hop(i);
+    __syncthreads();  // every thread is done with the previous hop's LDS data
+#pragma unroll
+    for (int k = 0; k < CPS; ++k) {
+      const int c = t + k * T;
+      s_rec4[rec_pos<CPS, RS>(c / CPS, c % CPS)] = nrec[k];
+    }
+    if (t < 9 * S) s_lk[t] = nlk;
+    if (hn >= 0) issue(hn);
+    __syncthreads();
+    if (live) {
+      const cplx<R>* lk = s_lk + sl;
+      if (h == 8)
+        wilson_diag_apply<R>(acc, [&](int l, int s) {
+          return l == 0 && s < 9 ? lk[s * S] : diag[((size_t)l * 18 + s) * sv + x];
+        }, rec);
+      else
+        wilson_hop_apply<R>(acc, G, h, [&](int k) { return lk[k * S]; }, rec);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < 12; ++e) {
+    cplx<R> v = acc[e];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    reinterpret_cast<cplx<R>*>(&s_rec[rec_pos<CPS, RS>(t, e * CPS / 12)])[(e * CPS) % 12 / 6] = v;
+  }
+  __syncthreads();
+  uint4* out4 = reinterpret_cast<uint4*>(out);
+#pragma unroll
+  for (int k = 0; k < CPS; ++k) {
+    const int c = t + k * T, q = c / CPS, e = c - q * CPS, s = q / nrhs, r = q - s * nrhs;
+    if (x0 + s < V) out4[((size_t)(x0 + s) * nrhs + r) * CPS + e] = s_rec[rec_pos<CPS, RS>(q, e)];
+  }
+}
+
 // [4][V][9] -> [4][9][sv] and [V][2][36] -> [2][18][sv] (with precision change);
 // the site stride sv is V plus a pad
 template <typename Rd>
@@ -569,7 +690,8 @@ static WilsonTiles wilson_tiles(const Lat4& L, int nrhs) {
 }
 
 // apply kernel: 0 = k_wilson (each thread reads its own records), 1 =
-// k_wilson_lds (records gathered through LDS); -1 = the default, the LDS form
+// k_wilson_lds (records gathered through LDS), 2 = k_wilson_pf for FP32 with
+// nrhs >= 9 (k_wilson_lds otherwise); -1 = the default, as 2
 static int g_wilson_variant = -1;
 
 static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
@@ -604,7 +726,8 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
       op->gsoa32 = g32;
       op->dsoa32 = d32;
     }
-    auto k = lds ? k_wilson_lds<float> : k_wilson<float>;
+    auto k = !lds ? k_wilson<float>
+             : g_wilson_variant != 1 && nrhs >= 9 ? k_wilson_pf<float> : k_wilson_lds<float>;
     k<<<g, threads, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in, (cplx<float>*)out,
                             op->nbr, op->V, op->sv, nrhs, mask, dag, wt);
   } else {
@@ -646,7 +769,7 @@ int mgc_fine_op_linop(mgc_fine_op* op, mgc_fine_linop* out) {
 
 int mgc_fine_op_set_variant(int variant) {
   return guarded([&] {
-    MGC_CHECK(variant >= -1 && variant <= 1, MGC_ERR_ARG, "unknown fine-operator kernel variant");
+    MGC_CHECK(variant >= -1 && variant <= 2, MGC_ERR_ARG, "unknown fine-operator kernel variant");
     g_wilson_variant = variant;
   });
 }

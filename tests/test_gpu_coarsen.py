@@ -86,12 +86,15 @@ def test_fine_operator_matches_oracle(gpu, prec, nrhs, dims):
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("dims,nrhs", [((4, 4, 2, 6), 1), ((4, 4, 2, 6), 3), ((8, 8, 8, 8), 1),
                                        ((8, 8, 8, 8), 5), ((8, 8, 8, 8), 12),
-                                       ((4, 4, 2, 6), 100), ((4, 4, 4, 8), 256)])
+                                       ((4, 4, 2, 6), 10), ((4, 4, 2, 6), 100),
+                                       ((4, 4, 4, 8), 256)])
 def test_fine_operator_kernel_variants_bit_equal(gpu, prec, dims, nrhs):
-    """The LDS-gathered apply keeps the per-thread arithmetic of the apply in
-    which each thread reads its own records: every kind (M, Mdag, Mdiag, the
-    eight Mdir) gives the same bits.  (4, 4, 2, 6) has partly filled tiles
-    (100 rhs: two sites of 100 threads each), 8^4 and 4x4x4x8 the
+    """The LDS-gathered apply and its prefetching multi-rhs form (FP32, nrhs
+    >= 9) keep the per-thread arithmetic of the apply in which each thread
+    reads its own records: every kind (M, Mdag, Mdiag, the eight Mdir) gives
+    the same bits.  (4, 4, 2, 6) has partly filled tiles (10 rhs: 25 sites a
+    tile, the last one partial; 100 rhs: two sites of 100 threads each),
+    8^4 and 4x4x4x8 the
     XCD-ordered tiles (256 rhs, the largest count: one site per
     workgroup)."""
     print(f"[case] fine variants dims={dims} nrhs={nrhs} prec={prec}", flush=True)
@@ -108,13 +111,14 @@ def test_fine_operator_kernel_variants_bit_equal(gpu, prec, dims, nrhs):
     try:
         for run in kinds:
             out = []
-            for variant in (0, 1):
+            for variant in (0, 1, 2):
                 gpu.call("mgc_fine_op_set_variant", variant)
                 dout.zero()
                 run()
                 gpu.synchronize()
                 out.append(dout.to_numpy(dt, (V, nrhs, 12)))
-            assert np.array_equal(out[0], out[1]), float(np.abs(out[0] - out[1]).max())
+            for o in out[1:]:
+                assert np.array_equal(out[0], o), float(np.abs(out[0] - o).max())
     finally:
         gpu.call("mgc_fine_op_set_variant", -1)
 

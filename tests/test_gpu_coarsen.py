@@ -46,12 +46,13 @@ def test_clover_term_matches_oracle(gpu, dims):
 
 
 @pytest.mark.parametrize("prec", [0, 1])
-@pytest.mark.parametrize("nrhs", [1, 3])
+@pytest.mark.parametrize("nrhs", [1, 3, 12])
 @pytest.mark.parametrize("dims", [(4, 4, 2, 6), (4, 4, 2, 8)])
 def test_fine_operator_matches_oracle(gpu, prec, nrhs, dims):
     """(4, 4, 2, 8) has L3 % 8 == 0, so the apply's tiles are dealt to the
     eight XCDs by time slices (the order used at 32^4 and 48^4); (4, 4, 2, 6)
-    takes the plain tile order."""
+    takes the plain tile order.  FP32 with 12 rhs runs the prefetching
+    multi-rhs kernel."""
     print(f"[case] fine op dims={dims} prec={prec} nrhs={nrhs}", flush=True)
     V = int(np.prod(dims))
     fop, U = fine_setup(gpu, dims, 41)

@@ -309,6 +309,86 @@ __device__ __forceinline__ void wilson_hop_apply(cplx<R>* acc, const GammaTab& G
   }
 }
 
+// One colour row c of the clover and hopping terms, for a thread that holds
+// the four spin components of colour c of one (site, rhs): acc[s] is element
+// s * 3 + c.  Every element gets the same updates in the same order as in
+// wilson_diag_apply and wilson_hop_apply, so the results are bit-identical;
+// c is uniform over a wavefront, so the tests on it are scalar branches.
+template <typename R, typename DK, typename SP>
+__device__ __forceinline__ void wilson_diag_apply_row(cplx<R>* acc, int c, DK&& dk, SP&& sp) {
+  const int e0 = c, e1 = 3 + c;  // this thread's elements of each block
+#pragma unroll
+  for (int l = 0; l < 2; ++l) {
+    cplx<R>& A0 = acc[2 * l];
+    cplx<R>& A1 = acc[2 * l + 1];
+    cplx<R> v[6];
+#pragma unroll
+    for (int b = 0; b < 6; ++b) v[b] = sp(6 * l + b);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool lo0 = 2 * a == e0, hi0 = 2 * a + 1 == e0;
+      const bool lo1 = 2 * a == e1, hi1 = 2 * a + 1 == e1;
+      if (!(lo0 || hi0 || lo1 || hi1)) continue;
+      const cplx<R> dd = dk(l, a);
+      if (lo0) { A0.re += dd.re * v[2 * a].re; A0.im += dd.re * v[2 * a].im; }
+      if (hi0) { A0.re += dd.im * v[2 * a + 1].re; A0.im += dd.im * v[2 * a + 1].im; }
+      if (lo1) { A1.re += dd.re * v[2 * a].re; A1.im += dd.re * v[2 * a].im; }
+      if (hi1) { A1.re += dd.im * v[2 * a + 1].re; A1.im += dd.im * v[2 * a + 1].im; }
+    }
+    int slot = 3;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a + 1; b < 6; ++b, ++slot) {
+        const bool u0 = a == e0, u1 = a == e1, w0 = b == e0, w1 = b == e1;
+        if (!(u0 || u1 || w0 || w1)) continue;
+        const cplx<R> u = dk(l, slot);
+        if (u0) cmac(A0, u, v[b]);
+        if (u1) cmac(A1, u, v[b]);
+        if (w0) cmac_conj(A0, u, v[a]);
+        if (w1) cmac_conj(A1, u, v[a]);
+      }
+  }
+}
+// lk(j): entry j of row c of U_mu(x) (p < 4) or of column c of U_mu(y)
+template <typename R, typename LK, typename SP>
+__device__ __forceinline__ void wilson_hop_apply_row(cplx<R>* acc, const GammaTab& G, int p,
+                                                     LK&& lk, SP&& sp) {
+  const int mu = p & 3;
+  const R sg = p < 4 ? R(1) : R(-1);
+  cplx<R> h[2][3];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const int cs = G.col[mu][s];
+    const cplx<R> ph = {(R)(sg * G.re[mu][s]), (R)(sg * G.im[mu][s])};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const cplx<R> a = sp(s * 3 + j), b = sp(cs * 3 + j);
+      const cplx<R> t = cmul(ph, b);
+      h[s][j] = {a.re - t.re, a.im - t.im};
+    }
+  }
+  cplx<R> L[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const cplx<R> l = lk(j);
+    L[j] = p < 4 ? l : cplx<R>{l.re, -l.im};
+  }
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    cplx<R> chi = {R(0), R(0)};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) cmac(chi, L[j], h[s][j]);
+    const int cs = G.col[mu][s];
+    const cplx<R> ph = {(R)(0.5 * sg * G.re[mu][cs]), (R)(0.5 * sg * G.im[mu][cs])};
+    acc[s].re -= R(0.5) * chi.re;
+    acc[s].im -= R(0.5) * chi.im;
+    const cplx<R> t = cmul(ph, chi);
+    acc[cs].re += t.re;
+    acc[cs].im += t.im;
+  }
+}
+
 // The hopping term with the link read from the site-strided SoA field
 // (element k of U_mu at x in U[(mu * 9 + k) * sv + x]).
 template <typename R, typename SP>
@@ -608,6 +688,118 @@ __global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U
   }
 }
 
+// Single-rhs form with three threads per site: wavefront c of a 192-thread
+// workgroup computes colour row c of 64 sites (tile of 64 sites).  A thread
+// keeps 4 accumulators instead of 12 and loads one row (or column) of each
+// hop's link instead of all nine entries, so it needs far fewer registers
+// than k_wilson_lds's thread per site, and more wavefronts fit on a SIMD.
+// The records are staged in LDS as in k_wilson_lds (each read by the three
+// threads of the site); hops run self first, then 0..7, with the next hop's
+// neighbour sites staged one hop ahead and the link loads issued before the
+// record copy.
+template <typename R>
+__global__ __launch_bounds__(192) void k_wilson_row(const cplx<R>* __restrict__ U,
+                                                    const cplx<R>* __restrict__ diag,
+                                                    const cplx<R>* __restrict__ in,
+                                                    cplx<R>* __restrict__ out,
+                                                    const int* __restrict__ nbr, int V, int sv,
+                                                    unsigned mask, int dag, WilsonTiles wt) {
+  constexpr int CPS = 12 * (int)sizeof(cplx<R>) / 16;
+  constexpr int RS = CPS + 1;
+  constexpr int S = 64, T = 192, NCH = S * CPS / T;  // record chunks per thread
+  static_assert(S * CPS % T == 0, "whole chunks per thread");
+  __shared__ uint4 s_rec[S * RS];
+  __shared__ int s_y[2][S];
+  const int t = (int)threadIdx.x;
+  const int c = __builtin_amdgcn_readfirstlane(t >> 6);  // colour row of this wavefront
+  const int sl = t & 63;
+  const int x0 = wilson_tile_site(wt, (int)blockIdx.x);
+  const int x = x0 + sl;
+  const bool live = x < V;
+  const GammaTab G = gamma_table();
+
+  auto hop_site = [&](int s, int h) {  // h = 8: the site itself
+    const int xs = x0 + s;
+    if (xs >= V) return 0;
+    return h == 8 ? xs : nbr[(size_t)xs * 8 + h];
+  };
+  auto next_hop = [&](int i) {
+    for (int k = i + 1; k < 9; ++k) {
+      const int h = k == 0 ? 8 : k - 1;
+      if ((mask >> h) & 1u) return h;
+    }
+    return -1;
+  };
+  const uint4* in4 = reinterpret_cast<const uint4*>(in);
+  auto rec = [&](int e) {  // element e of this site's staged record
+    cplx<R> v = reinterpret_cast<const cplx<R>*>(&s_rec[rec_pos<CPS, RS>(sl, e * CPS / 12)])
+        [(e * CPS) % 12 / 6];
+    if (dag && e >= 6) { v.re = -v.re; v.im = -v.im; }
+    return v;
+  };
+
+  cplx<R> acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = {R(0), R(0)};
+  int cb = 0;
+  const int h0 = next_hop(-1);
+  if (h0 >= 0) {
+    if (t < S) s_y[0][t] = hop_site(t, h0);
+    __syncthreads();
+  }
+  cplx<R> lk[3];
+  // one stage's loads: this hop's link row (or column), then the records of
+  // its neighbour sites into LDS, the next hop's sites staged one ahead
+  auto begin_stage = [&](int h, int hn) __attribute__((always_inline)) {
+    __syncthreads();  // every thread is done with the previous hop's records
+    const int y = s_y[cb][sl];
+    if (live && h != 8) {
+      const cplx<R>* u = U + (size_t)(h & 3) * 9 * sv + (h < 4 ? x : y);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) lk[j] = u[(size_t)(h < 4 ? c * 3 + j : j * 3 + c) * sv];
+    }
+    const int ynext = hn >= 0 && t < S ? hop_site(t, hn) : 0;
+#pragma unroll
+    for (int k = 0; k < NCH; ++k) {
+      const int cc = t + k * T, q = cc / CPS, e = cc - q * CPS;
+      s_rec[rec_pos<CPS, RS>(q, e)] = in4[(size_t)s_y[cb][q] * CPS + e];
+    }
+    if (hn >= 0 && t < S) s_y[cb ^ 1][t] = ynext;
+    __syncthreads();
+    cb ^= 1;
+  };
+  // the self stage apart from the hop loop, so that the loop body stays
+  // small enough to be unrolled (every hop index a compile-time constant)
+  if ((mask >> 8) & 1u) {
+    begin_stage(8, next_hop(0));
+    if (live)
+      wilson_diag_apply_row<R>(acc, c, [&](int l, int s) {
+        return diag[((size_t)l * 18 + s) * sv + x];
+      }, rec);
+  }
+#pragma unroll
+  for (int p = 0; p < 8; ++p) {
+    if (!((mask >> p) & 1u)) continue;
+    begin_stage(p, next_hop(p + 1));
+    if (live) wilson_hop_apply_row<R>(acc, G, p, [&](int j) { return lk[j]; }, rec);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int e = s * 3 + c;
+    cplx<R> v = acc[s];
+    if (dag && s >= 2) { v.re = -v.re; v.im = -v.im; }
+    reinterpret_cast<cplx<R>*>(&s_rec[rec_pos<CPS, RS>(sl, e * CPS / 12)])[(e * CPS) % 12 / 6] = v;
+  }
+  __syncthreads();
+  uint4* out4 = reinterpret_cast<uint4*>(out);
+#pragma unroll
+  for (int k = 0; k < NCH; ++k) {
+    const int cc = t + k * T, q = cc / CPS, e = cc - q * CPS;
+    if (x0 + q < V) out4[(size_t)(x0 + q) * CPS + e] = s_rec[rec_pos<CPS, RS>(q, e)];
+  }
+}
+
 // [4][V][9] -> [4][9][sv] and [V][2][36] -> [2][18][sv] (with precision change);
 // the site stride sv is V plus a pad
 template <typename Rd>
@@ -669,6 +861,21 @@ static void free_fine(mgc_fine_op* op) {
 
 // Tile geometry of the apply: S sites of nrhs threads each (at most 256
 // threads), XCD-local ordering where the lattice allows it.
+// Tiles of exactly S sites (S a power of two), XCD-local where the lattice
+// allows it.
+static WilsonTiles wilson_tiles_fixed(const Lat4& L, int S) {
+  WilsonTiles w{};
+  w.S = S;
+  w.P = L.L[0] * L.L[1];
+  w.L2 = L.L[2];
+  w.xcd = (L.L[3] % 8 == 0 && w.P % S == 0) ? 1 : 0;
+  if (!w.xcd) return w;
+  w.G = w.P / S;
+  w.TL = L.L[3] / 8;
+  w.ZS = (L.L[2] % 4 == 0) ? 4 : (L.L[2] % 2 == 0 ? 2 : 1);
+  return w;
+}
+
 static WilsonTiles wilson_tiles(const Lat4& L, int nrhs) {
   WilsonTiles w{};
   const int P = L.L[0] * L.L[1];
@@ -691,8 +898,12 @@ static WilsonTiles wilson_tiles(const Lat4& L, int nrhs) {
 
 // apply kernel: 0 = k_wilson (each thread reads its own records), 1 =
 // k_wilson_lds (records gathered through LDS), 2 = k_wilson_pf for FP32 with
-// nrhs >= 9 (k_wilson_lds otherwise); -1 = the default, as 2
+// nrhs >= 9 (k_wilson_lds otherwise), 3 = k_wilson_row for one rhs (as 2
+// otherwise); -1 = the default, as 3
 static int g_wilson_variant = -1;
+static bool wilson_row(int nrhs) {
+  return nrhs == 1 && (g_wilson_variant == 3 || g_wilson_variant == -1);
+}
 
 static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int nrhs,
                        unsigned mask, int dag, hipStream_t s) {
@@ -705,7 +916,13 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
   const unsigned g = (unsigned)((op->V + wt.S - 1) / wt.S);
   const unsigned threads = (unsigned)(wt.S * nrhs);
   const bool lds = g_wilson_variant != 0;
-  if (prec == MGC_F64) {
+  const WilsonTiles wr = wilson_tiles_fixed(op->lat, 64);
+  const unsigned gr = (unsigned)((op->V + 63) / 64);
+  if (prec == MGC_F64 && wilson_row(nrhs)) {
+    k_wilson_row<double><<<gr, 192, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in,
+                                            (cplx<double>*)out, op->nbr, op->V, op->sv, mask,
+                                            dag, wr);
+  } else if (prec == MGC_F64) {
     auto k = lds ? k_wilson_lds<double> : k_wilson<double>;
     k<<<g, threads, 0, s>>>(op->gsoa64, op->dsoa64, (const cplx<double>*)in, (cplx<double>*)out,
                             op->nbr, op->V, op->sv, nrhs, mask, dag, wt);
@@ -725,6 +942,13 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
       build_soa<float>(op, g32, d32, s);
       op->gsoa32 = g32;
       op->dsoa32 = d32;
+    }
+    if (wilson_row(nrhs)) {
+      k_wilson_row<float><<<gr, 192, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in,
+                                             (cplx<float>*)out, op->nbr, op->V, op->sv, mask,
+                                             dag, wr);
+      MGC_LAUNCH_CHECK();
+      return;
     }
     auto k = !lds ? k_wilson<float>
              : g_wilson_variant != 1 && nrhs >= 9 ? k_wilson_pf<float> : k_wilson_lds<float>;
@@ -769,7 +993,7 @@ int mgc_fine_op_linop(mgc_fine_op* op, mgc_fine_linop* out) {
 
 int mgc_fine_op_set_variant(int variant) {
   return guarded([&] {
-    MGC_CHECK(variant >= -1 && variant <= 2, MGC_ERR_ARG, "unknown fine-operator kernel variant");
+    MGC_CHECK(variant >= -1 && variant <= 3, MGC_ERR_ARG, "unknown fine-operator kernel variant");
     g_wilson_variant = variant;
   });
 }

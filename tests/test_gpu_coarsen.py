@@ -112,7 +112,7 @@ def test_fine_operator_kernel_variants_bit_equal(gpu, prec, dims, nrhs):
     try:
         for run in kinds:
             out = []
-            for variant in (0, 1, 2):
+            for variant in (0, 1, 2, 3):
                 gpu.call("mgc_fine_op_set_variant", variant)
                 dout.zero()
                 run()

@@ -568,8 +568,9 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
 }
 
 // Multi-rhs form with the next hop's loads in flight during the current
-// hop's arithmetic (nrhs >= 9, so a tile has at most 28 sites and each of
-// the hop's 9 x S link entries is loaded by one thread).  The tile's 8 x S
+// hop's arithmetic (nrhs >= 5: LPT = 1 link entry per thread for nrhs >= 9,
+// when a tile has at most 28 sites, else LPT = 2, at most 51 sites; the
+// hop's 9 x S link entries are loaded by the first threads).  The tile's 8 x S
 // neighbour sites are read once, coalesced, into LDS.  Each stage then
 // stores the records and link entries that were loaded during the previous
 // stage into LDS, issues the loads of the next stage into registers (six
@@ -578,7 +579,7 @@ __global__ __launch_bounds__(256) void k_wilson_lds(const cplx<R>* __restrict__ 
 // rhs's thread.  The self stage's "link" is clover slots 0..8 of block 0.
 // Record layout, output path and per-thread arithmetic are those of
 // k_wilson_lds.
-template <typename R>
+template <typename R, int LPT>
 __global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U,
                                                    const cplx<R>* __restrict__ diag,
                                                    const cplx<R>* __restrict__ in,
@@ -588,7 +589,7 @@ __global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U
                                                    WilsonTiles wt) {
   constexpr int CPS = 12 * (int)sizeof(cplx<R>) / 16;
   constexpr int RS = CPS + 1;
-  constexpr int SMAX = 28;  // sites per tile at nrhs >= 9
+  constexpr int SMAX = LPT == 1 ? 28 : 51;  // sites per tile at nrhs >= 9 or >= 5
   __shared__ uint4 s_rec[256 * RS];
   __shared__ cplx<R> s_lk[9 * SMAX];  // [k][site]
   __shared__ int s_nb[9][SMAX];       // [hop][site], hop 8 = the site itself
@@ -601,8 +602,8 @@ __global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U
   const bool live = x < V;
   const GammaTab G = gamma_table();
 
-  if (t < 8 * S) {  // t = s * 8 + h: consecutive lanes read consecutive entries
-    const int s = t >> 3, h = t & 7;
+  for (int j = t; j < 8 * S; j += T) {  // j = s * 8 + h: consecutive lanes, consecutive entries
+    const int s = j >> 3, h = j & 7;
     s_nb[h][s] = x0 + s < V ? nbr[(size_t)(x0 + s) * 8 + h] : 0;
   }
   if (t < S) s_nb[8][t] = x0 + t < V ? x0 + t : 0;
@@ -619,20 +620,24 @@ __global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U
   const u32x4* in4 = reinterpret_cast<const u32x4*>(in);
   u32x4* s_rec4 = reinterpret_cast<u32x4*>(s_rec);
   u32x4 nrec[CPS];
-  cplx<R> nlk = {R(0), R(0)};
+  cplx<R> nlk[LPT];
   // loads of hop h into registers: record chunks c = t + k*T of the tile's
-  // records, and link entry (k = t / S, site t % S) for t < 9 S
+  // records, and link entries j = t + m*T (k = j / S, site j % S) for j < 9 S
   auto issue = [&](int h) {
 #pragma unroll
     for (int k = 0; k < CPS; ++k) {
       const int c = t + k * T, q = c / CPS, e = c - q * CPS, s = q / nrhs, r = q - s * nrhs;
       nrec[k] = in4[((size_t)s_nb[h][s] * nrhs + r) * CPS + e];
     }
-    if (t < 9 * S) {
-      const int k = t / S, s = t - k * S;
-      const cplx<R>* u = h == 8 ? diag + s_nb[8][s]
-                                : U + (size_t)(h & 3) * 9 * sv + (h < 4 ? s_nb[8][s] : s_nb[h][s]);
-      nlk = u[(size_t)k * sv];
+#pragma unroll
+    for (int m = 0; m < LPT; ++m) {
+      const int j = t + m * T;
+      if (j < 9 * S) {
+        const int k = j / S, s = j - k * S;
+        const cplx<R>* u = h == 8 ? diag + s_nb[8][s]
+                                  : U + (size_t)(h & 3) * 9 * sv + (h < 4 ? s_nb[8][s] : s_nb[h][s]);
+        nlk[m] = u[(size_t)k * sv];
+      }
     }
   };
   auto rec = [&](int e) {
@@ -659,7 +664,9 @@ __global__ __launch_bounds__(256) void k_wilson_pf(const cplx<R>* __restrict__ U
       const int c = t + k * T;
       s_rec4[rec_pos<CPS, RS>(c / CPS, c % CPS)] = nrec[k];
     }
-    if (t < 9 * S) s_lk[t] = nlk;
+#pragma unroll
+    for (int m = 0; m < LPT; ++m)
+      if (t + m * T < 9 * S) s_lk[t + m * T] = nlk[m];
     if (hn >= 0) issue(hn);
     __syncthreads();
     if (live) {
@@ -898,7 +905,7 @@ static WilsonTiles wilson_tiles(const Lat4& L, int nrhs) {
 
 // apply kernel: 0 = k_wilson (each thread reads its own records), 1 =
 // k_wilson_lds (records gathered through LDS), 2 = k_wilson_pf for FP32 with
-// nrhs >= 9 (k_wilson_lds otherwise), 3 = k_wilson_row for one rhs (as 2
+// nrhs >= 5 (k_wilson_lds otherwise), 3 = k_wilson_row for one rhs (as 2
 // otherwise); -1 = the default, as 3
 static int g_wilson_variant = -1;
 static bool wilson_row(int nrhs) {
@@ -951,7 +958,9 @@ static void fine_apply(mgc_fine_op* op, const void* in, void* out, int prec, int
       return;
     }
     auto k = !lds ? k_wilson<float>
-             : g_wilson_variant != 1 && nrhs >= 9 ? k_wilson_pf<float> : k_wilson_lds<float>;
+             : g_wilson_variant == 1 || nrhs < 5 ? k_wilson_lds<float>
+             : nrhs >= 9                        ? k_wilson_pf<float, 1>
+                                                : k_wilson_pf<float, 2>;
     k<<<g, threads, 0, s>>>(op->gsoa32, op->dsoa32, (const cplx<float>*)in, (cplx<float>*)out,
                             op->nbr, op->V, op->sv, nrhs, mask, dag, wt);
   } else {

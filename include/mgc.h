@@ -328,7 +328,7 @@ int mgc_fine_op_Mdir(mgc_fine_op* op, const void* in, void* out, int dir, int di
  * 0 = each thread reads its own spinor records, 1 = each hop's records
  * gathered by the workgroup through LDS with coalesced loads, 2 = as 1 with
  * each hop's link shared through LDS by the rhs of a site and the next hop's
- * loads issued before the current hop's arithmetic (FP32 with nrhs >= 9; 1
+ * loads issued before the current hop's arithmetic (FP32 with nrhs >= 5; 1
  * otherwise), 3 = as 2, but with one rhs three threads per site, one colour
  * row each; -1 restores the default (3).  All give bit-identical results.
  * No counterpart in the reference: it serves tests and measurements. */

@@ -87,13 +87,14 @@ def test_fine_operator_matches_oracle(gpu, prec, nrhs, dims):
 @pytest.mark.parametrize("prec", [0, 1])
 @pytest.mark.parametrize("dims,nrhs", [((4, 4, 2, 6), 1), ((4, 4, 2, 6), 3), ((8, 8, 8, 8), 1),
                                        ((8, 8, 8, 8), 5), ((8, 8, 8, 8), 12),
-                                       ((4, 4, 2, 6), 10), ((4, 4, 2, 6), 100),
+                                       ((4, 4, 2, 6), 7), ((4, 4, 2, 6), 10), ((4, 4, 2, 6), 100),
                                        ((4, 4, 4, 8), 256)])
 def test_fine_operator_kernel_variants_bit_equal(gpu, prec, dims, nrhs):
-    """The LDS-gathered apply and its prefetching multi-rhs form (FP32, nrhs
-    >= 9) keep the per-thread arithmetic of the apply in which each thread
-    reads its own records: every kind (M, Mdag, Mdiag, the eight Mdir) gives
-    the same bits.  (4, 4, 2, 6) has partly filled tiles (10 rhs: 25 sites a
+    """The LDS-gathered apply, its prefetching multi-rhs form (FP32, nrhs
+    >= 5) and its single-rhs form with one colour row per thread keep the
+    per-thread arithmetic of the apply in which each thread reads its own
+    records: every kind (M, Mdag, Mdiag, the eight Mdir) gives the same bits.
+    (4, 4, 2, 6) has partly filled tiles (7 and 10 rhs: 36 and 25 sites a
     tile, the last one partial; 100 rhs: two sites of 100 threads each),
     8^4 and 4x4x4x8 the
     XCD-ordered tiles (256 rhs, the largest count: one site per

@@ -50,6 +50,20 @@ def test_errors_are_status_codes_without_device():
     assert L.mgc_version() >= 10000
 
 
+def test_kernel_variant_ranges_without_device():
+    """The variant controls only set a process-wide choice: every documented
+    value is accepted and the first value past the range is refused."""
+    L = mgc.lib()
+    try:
+        for v in (-1, 0, 1, 2, 3):
+            assert L.mgc_fine_op_set_variant(v) == 0
+        assert L.mgc_fine_op_set_variant(4) == -1
+        assert b"variant" in L.mgc_last_error()
+        assert L.mgc_fine_op_set_variant(-2) == -1
+    finally:
+        assert L.mgc_fine_op_set_variant(-1) == 0
+
+
 def test_multigrid_parameter_checks_without_device():
     """Hierarchy parameters are validated before any device work: a solver
     type the chosen mode cannot run, an unknown graph mode or basis type is a
